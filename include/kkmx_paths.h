@@ -1,0 +1,78 @@
+/*
+ * kkmx_paths.h -- C-ABI entry points of the hot path, for callers that cannot
+ * instantiate C++ templates (ctypes, cgo, JNI, other languages) and for the
+ * parity tests and the benchmark.
+ *
+ * Each entry point is ordinary Kokkos user code compiled against the MI355X
+ * execution space (kokkos_amd/csrc/paths.hip): it wraps the caller's device
+ * pointers in unmanaged Views and calls parallel_for / parallel_reduce /
+ * parallel_scan.  Pointers named *_dev are device memory (mx_malloc); results
+ * land in device memory asynchronously on the default instance stream
+ * (mx_default_stream), the behaviour of a View result in the reference
+ * (core/src/Kokkos_Parallel_Reduce.hpp:1592-1635: no fence for View results).
+ * Every function returns 0 on success, a negative value for a rejected
+ * argument and a positive HIP status for a runtime failure; kkmx_last_error()
+ * gives the message.
+ *
+ * Reference interfaces:
+ *   kkmx_stream_*           benchmarks/stream/stream-kokkos.cpp:36-77 (set/copy/scale/add/triad functors)
+ *   kkmx_reduce_*           core/src/Kokkos_Parallel_Reduce.hpp:1679-1833 (parallel_reduce with
+ *                           Sum / Min / Max / MinMax reducers, :33-212,536-587)
+ *   kkmx_scan_*             core/src/Kokkos_Parallel.hpp:348-454 (parallel_scan with and without total)
+ *   kkmx_fill_*             algorithms/src/Kokkos_Random.hpp:943-1010 (fill_random) -- seeded inputs,
+ *                           counter-based so host and device produce identical values
+ */
+#ifndef KKMX_PATHS_H
+#define KKMX_PATHS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Kokkos::initialize / finalize for C callers (idempotent). */
+int kkmx_init(int device);
+int kkmx_finalize(void);
+const char* kkmx_last_error(void);
+/* The stream every entry point below is ordered on. */
+void* kkmx_stream(void);
+int kkmx_fence(void);
+
+/* ---- seeded inputs (splitmix64 counter hash of (seed, i)) ---- */
+int kkmx_fill_uniform_f64(double* x_dev, int64_t n, uint64_t seed, double lo, double hi);
+int kkmx_fill_uniform_i64(int64_t* x_dev, int64_t n, uint64_t seed, int64_t lo, int64_t hi);
+int kkmx_fill_uniform_i32(int32_t* x_dev, int64_t n, uint64_t seed, int32_t lo, int32_t hi);
+
+/* ---- STREAM: RangePolicy<IndexType<int>> parallel_for ---- */
+int kkmx_stream_set(double* a_dev, int64_t n, double scalar);
+int kkmx_stream_copy(const double* a_dev, double* b_dev, int64_t n);
+int kkmx_stream_scale(double* b_dev, const double* c_dev, int64_t n, double scalar);
+int kkmx_stream_add(const double* a_dev, const double* b_dev, double* c_dev, int64_t n);
+int kkmx_stream_triad(double* a_dev, const double* b_dev, const double* c_dev, int64_t n, double scalar);
+
+/* ---- parallel_reduce into a device result ---- */
+int kkmx_reduce_sum_f64(const double* x_dev, int64_t n, double* out_dev);
+/* op: 0 = Sum, 1 = Min, 2 = Max */
+int kkmx_reduce_i64(const int64_t* x_dev, int64_t n, int op, int64_t* out_dev);
+int kkmx_reduce_i32(const int32_t* x_dev, int64_t n, int op, int64_t* out_dev);
+/* One pass: out_dev[0] = min, [1] = max, [2] = sum (MinMax + Sum in one struct value). */
+int kkmx_reduce_minmaxsum_i64(const int64_t* x_dev, int64_t n, int64_t* out_dev);
+
+/* ---- parallel_scan ---- */
+/* Exclusive prefix sum of int32 input accumulated in int64; total_dev may be NULL. */
+int kkmx_scan_excl_i32_i64(const int32_t* in_dev, int64_t* out_dev, int64_t n, int64_t* total_dev);
+/* Inclusive prefix sum of int32 input in int32. */
+int kkmx_scan_incl_i32(const int32_t* in_dev, int32_t* out_dev, int64_t n);
+int kkmx_scan_excl_i64(const int64_t* in_dev, int64_t* out_dev, int64_t n, int64_t* total_dev);
+int kkmx_scan_excl_f64(const double* in_dev, double* out_dev, int64_t n, double* total_dev);
+
+/* ---- launch-shape overrides (what the tuning hooks set), -1 = heuristic ---- */
+int kkmx_set_range_config(int pattern, int block, int items, int max_blocks);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KKMX_PATHS_H */

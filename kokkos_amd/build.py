@@ -1,0 +1,89 @@
+"""Build the MI355X execution space's shared libraries in-tree with hipcc.
+
+Libraries (written next to their sources, under kokkos_amd/lib/):
+  libkkmx_rt.so     C-ABI runtime (include/kkmx_runtime.h)
+  libkkmx_paths.so  hot-path C-ABI entry points (include/kkmx_paths.h)
+  further libraries listed in LIBS below.
+
+Usage: python -m kokkos_amd.build [--jobs N] [names...]
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+LIBDIR = PKG / "lib"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("KKMX_OFFLOAD_ARCH", "gfx950")
+
+COMMON = [
+    "-std=c++17",
+    "-O3",
+    "-fPIC",
+    "-shared",
+    f"--offload-arch={ARCH}",
+    # Packed FP32 instructions stay off: hipcc only forms them when allowed.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", str(ROOT / "include"),
+    "-I", str(PKG / "include"),
+    "-Wno-unused-command-line-argument",
+]
+
+# name -> (sources, extra flags, link against runtime)
+LIBS = {
+    "kkmx_rt": (["csrc/runtime.cpp"], [], False),
+    "kkmx_paths": (["csrc/paths.hip"], [], True),
+    "kkmx_peak": (["csrc/peak.hip"], [], True),
+}
+
+
+def lib_path(name: str) -> Path:
+    return LIBDIR / f"lib{name}.so"
+
+
+def build_one(name: str, verbose: bool = False) -> Path:
+    srcs, extra, link_rt = LIBS[name]
+    LIBDIR.mkdir(exist_ok=True)
+    out = lib_path(name)
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [HIPCC, *COMMON, *extra, *[str(PKG / s) for s in srcs], "-o", str(tmp)]
+    if link_rt:
+        cmd += ["-L", str(LIBDIR), "-lkkmx_rt", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"hipcc failed for {name}")
+    os.replace(tmp, out)
+    return out
+
+
+def build(names=None, jobs: int = 4, verbose: bool = False) -> list[Path]:
+    names = list(names or LIBS.keys())
+    outs = []
+    # The runtime first: everything else links against it.
+    if "kkmx_rt" in names:
+        outs.append(build_one("kkmx_rt", verbose))
+        names.remove("kkmx_rt")
+    elif not lib_path("kkmx_rt").exists():
+        outs.append(build_one("kkmx_rt", verbose))
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        for p in ex.map(lambda n: build_one(n, verbose), names):
+            outs.append(p)
+    return outs
+
+
+if __name__ == "__main__":
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    jobs = 4
+    for a in sys.argv[1:]:
+        if a.startswith("--jobs="):
+            jobs = int(a.split("=", 1)[1])
+    for p in build(args or None, jobs=jobs, verbose="--verbose" in sys.argv):
+        print(p)

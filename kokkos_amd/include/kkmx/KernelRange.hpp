@@ -1,0 +1,737 @@
+// RangePolicy kernels for MI355X (gfx950) and the dispatch closures that
+// launch them.
+//
+//  K1 k_range_for     grid-stride loop over tiles of blockDim*U consecutive
+//                     indices; within a tile, round u covers blockDim
+//                     consecutive indices, so every functor call of a
+//                     wavefront touches 64 consecutive elements (coalesced).
+//  K2 k_range_reduce  same traversal with U independent loads in flight per
+//                     lane, ordered DPP wave reduction, LDS combine of the
+//                     wavefronts, one partial per workgroup, and a last-block
+//                     finish behind an agent-scope acq_rel counter that the
+//                     last block resets to zero (no memset between launches).
+//                     The grid size is a function of the range only, so the
+//                     floating-point combination order is reproducible.
+//  K3 k_range_scan    single-pass decoupled look-back scan (Merrill and
+//                     Garland, "Single-pass Parallel Prefix Scan with
+//                     Decoupled Look-back", NVIDIA NVR-2016-002).  Tiles are
+//                     claimed from an atomic counter in launch order, so every
+//                     predecessor of a tile is already running (forward
+//                     progress).  Each wavefront owns 64*IPT consecutive
+//                     indices, handled in IPT coalesced rounds; pass 1 calls
+//                     f(i, v, false) once per index and keeps the per-round
+//                     inclusive wave scans in registers, pass 2 calls
+//                     f(i, prefix, true) once per index.  Tile status words
+//                     carry a launch epoch, so they never need clearing.
+//
+// Reference closures these specialise: core/src/HIP/Kokkos_HIP_ParallelFor_Range.hpp:28-101,
+// Kokkos_HIP_ParallelReduce_Range.hpp:30-337, Kokkos_HIP_ParallelScan_Range.hpp:29-388.
+#ifndef KKMX_KERNEL_RANGE_HPP
+#define KKMX_KERNEL_RANGE_HPP
+
+#include <kkmx/FunctorAnalysis.hpp>
+#include <kkmx/Policy.hpp>
+#include <kkmx/Tools.hpp>
+#include <kkmx/Wave.hpp>
+
+namespace Kokkos {
+namespace Impl {
+
+// ------------------------------------------------------------------ launch helpers
+template <class... Args>
+inline void mx_launch_kernel(const void* fn, unsigned grid, unsigned block, size_t lds,
+                             mx_stream_t stream, Args&... args) {
+  void* argv[] = {static_cast<void*>(&args)..., nullptr};
+  KKMX_SAFE_CALL(mx_launch(fn, grid, 1, 1, block, 1, 1, lds, argv, stream));
+}
+
+// Launch shape of a Range dispatch.  block: lanes per workgroup (multiple of
+// 64, <= 1024); items: indices per lane per tile; max_blocks: 0 = one
+// workgroup per tile, else a grid-stride cap.
+struct RangeConfig {
+  int block      = 256;
+  int items      = 1;
+  int max_blocks = 0;
+};
+
+inline int normalise_items(int u) { return u >= 8 ? 8 : u >= 4 ? 4 : u >= 2 ? 2 : 1; }
+inline int normalise_block(int b) {
+  if (b < 64) b = 64;
+  if (b > 1024) b = 1024;
+  return (b / 64) * 64;
+}
+
+template <class T>
+struct alignas(16) LdsSlots16 {
+  unsigned char bytes[16 * sizeof(T)];
+  __device__ T* get() { return reinterpret_cast<T*>(bytes); }
+};
+
+// ------------------------------------------------------------------ K1
+template <class F, class Tag, class Index, int U>
+__global__ __launch_bounds__(1024) void k_range_for(const F f, const int64_t begin,
+                                                    const int64_t end) {
+  const int64_t bs     = blockDim.x;
+  const int64_t tile   = bs * U;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * tile;
+  for (int64_t base = begin + static_cast<int64_t>(blockIdx.x) * tile; base < end; base += stride) {
+    const int64_t i0 = base + threadIdx.x;
+    if (base + tile <= end) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) call_functor<Tag>(f, static_cast<Index>(i0 + u * bs));
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = i0 + u * bs;
+        if (i < end) call_functor<Tag>(f, static_cast<Index>(i));
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ K2
+template <class CR, class Index, int U>
+__global__ __launch_bounds__(1024) void k_range_reduce(
+    const CR cr, const int64_t begin, const int64_t end, typename CR::value_type* partials,
+    unsigned* counter, typename CR::value_type* result) {
+  using V   = typename CR::value_type;
+  using Tag = typename CR::work_tag;
+  __shared__ LdsSlots16<V> slots;
+  __shared__ int s_last;
+  const auto join = [&cr](V& a, const V& b) { cr.join(a, b); };
+
+  V identity;
+  cr.init(identity);
+  V v = identity;
+
+  const int64_t bs     = blockDim.x;
+  const int64_t tile   = bs * U;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * tile;
+  for (int64_t base = begin + static_cast<int64_t>(blockIdx.x) * tile; base < end; base += stride) {
+    const int64_t i0 = base + threadIdx.x;
+    if (base + tile <= end) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) call_functor<Tag>(cr.functor(), static_cast<Index>(i0 + u * bs), v);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = i0 + u * bs;
+        if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), v);
+      }
+    }
+  }
+  v = block_reduce(v, identity, join, slots.get(), blockDim.x);
+
+  if (gridDim.x == 1) {
+    if (threadIdx.x == 0) {
+      cr.final(v);
+      *result = v;
+    }
+    return;
+  }
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = v;
+    const unsigned prev =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  V w = identity;
+  // Thread t folds partials t, t+bs, ... in order; the block reduction keeps
+  // thread order, so the combination tree depends on the grid size only.
+  for (int64_t j = threadIdx.x; j < gridDim.x; j += bs) join(w, partials[j]);
+  w = block_reduce(w, identity, join, slots.get(), blockDim.x);
+  if (threadIdx.x == 0) {
+    cr.final(w);
+    *result = w;
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Runtime-length (value_type[]) reductions: per-lane partials live in LDS.
+template <class CR, class Index>
+__global__ __launch_bounds__(256) void k_range_reduce_array(
+    const CR cr, const int64_t begin, const int64_t end, typename CR::value_type* partials,
+    unsigned* counter, typename CR::value_type* result) {
+  using V   = typename CR::value_type;
+  using Tag = typename CR::work_tag;
+  extern __shared__ __align__(16) unsigned char kkmx_dyn_lds[];
+  __shared__ int s_last;
+  const unsigned n = cr.value_count();
+  V* mine          = reinterpret_cast<V*>(kkmx_dyn_lds) + threadIdx.x * n;
+  cr.init_array(mine);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = begin + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < end;
+       i += stride)
+    call_functor<Tag>(cr.functor(), static_cast<Index>(i), mine);
+  __syncthreads();
+  // Pairwise tree in LDS, lower thread on the left.
+  for (unsigned s = 1; s < blockDim.x; s <<= 1) {
+    if ((threadIdx.x % (2 * s)) == 0 && threadIdx.x + s < blockDim.x)
+      cr.join_array(mine, reinterpret_cast<V*>(kkmx_dyn_lds) + (threadIdx.x + s) * n);
+    __syncthreads();
+  }
+  V* block_v = reinterpret_cast<V*>(kkmx_dyn_lds);
+  if (gridDim.x == 1) {
+    if (threadIdx.x == 0) {
+      cr.final_array(block_v);
+      for (unsigned k = 0; k < n; ++k) result[k] = block_v[k];
+    }
+    return;
+  }
+  if (threadIdx.x == 0) {
+    for (unsigned k = 0; k < n; ++k) partials[blockIdx.x * n + k] = block_v[k];
+    const unsigned prev =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (threadIdx.x == 0) {
+    cr.init_array(block_v);
+    for (unsigned b = 0; b < gridDim.x; ++b) cr.join_array(block_v, partials + b * n);
+    cr.final_array(block_v);
+    for (unsigned k = 0; k < n; ++k) result[k] = block_v[k];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ------------------------------------------------------------------ K3 tile state
+constexpr uint32_t kScanAgg  = 1;
+constexpr uint32_t kScanIncl = 2;
+KOKKOS_INLINE_FUNCTION uint32_t scan_flag(uint32_t epoch, uint32_t st) {
+  return ((epoch & 0x3fffffffu) << 2) | st;
+}
+
+template <class T, int Kind = (sizeof(T) == 4 ? 1 : (sizeof(T) == 8 ? 2 : 0))>
+struct ScanTileState;
+
+// 4-byte values: one 64-bit word {flag, value}.
+template <class T>
+struct ScanTileState<T, 1> {
+  uint64_t* words;
+  static size_t bytes(int64_t ntiles) { return static_cast<size_t>(ntiles) * 8; }
+  void bind(void* status, void*) { words = static_cast<uint64_t*>(status); }
+  __device__ void publish(int64_t t, uint32_t flag, const T& v) const {
+    uint32_t bits;
+    __builtin_memcpy(&bits, &v, 4);
+    __hip_atomic_store(&words[t], (static_cast<uint64_t>(flag) << 32) | bits, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ uint32_t read(int64_t t, T& v) const {
+    const uint64_t w = __hip_atomic_load(&words[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t bits = static_cast<uint32_t>(w);
+    __builtin_memcpy(&v, &bits, 4);
+    return static_cast<uint32_t>(w >> 32);
+  }
+};
+
+// 8-byte values: two 64-bit words, each {flag, half}; a read is valid only
+// when both halves carry the same flag.
+template <class T>
+struct ScanTileState<T, 2> {
+  uint64_t* words;
+  static size_t bytes(int64_t ntiles) { return static_cast<size_t>(ntiles) * 16; }
+  void bind(void* status, void*) { words = static_cast<uint64_t*>(status); }
+  __device__ void publish(int64_t t, uint32_t flag, const T& v) const {
+    uint64_t bits;
+    __builtin_memcpy(&bits, &v, 8);
+    const uint64_t hi = static_cast<uint64_t>(flag) << 32;
+    __hip_atomic_store(&words[2 * t], hi | (bits & 0xffffffffu), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&words[2 * t + 1], hi | (bits >> 32), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ uint32_t read(int64_t t, T& v) const {
+    const uint64_t a = __hip_atomic_load(&words[2 * t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t b = __hip_atomic_load(&words[2 * t + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t fa = static_cast<uint32_t>(a >> 32), fb = static_cast<uint32_t>(b >> 32);
+    if (fa != fb) return 0;
+    const uint64_t bits = (a & 0xffffffffu) | (b << 32);
+    __builtin_memcpy(&v, &bits, 8);
+    return fa;
+  }
+};
+
+// Any other value: flag word plus separate aggregate / inclusive slots,
+// ordered with agent-scope release/acquire fences.
+template <class T>
+struct ScanTileState<T, 0> {
+  uint32_t* flags;
+  T* values;  // 2 per tile: [2t] aggregate, [2t+1] inclusive prefix
+  static size_t bytes(int64_t ntiles) { return static_cast<size_t>(ntiles) * 4; }
+  static size_t value_bytes(int64_t ntiles) { return static_cast<size_t>(ntiles) * 2 * sizeof(T); }
+  void bind(void* status, void* vals) {
+    flags  = static_cast<uint32_t*>(status);
+    values = static_cast<T*>(vals);
+  }
+  __device__ void publish(int64_t t, uint32_t flag, const T& v) const {
+    values[2 * t + ((flag & 3) == kScanIncl ? 1 : 0)] = v;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(&flags[t], flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ uint32_t read(int64_t t, T& v) const {
+    const uint32_t f = __hip_atomic_load(&flags[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((f & 3) != 0) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      v = values[2 * t + ((f & 3) == kScanIncl ? 1 : 0)];
+    }
+    return f;
+  }
+};
+
+template <class T>
+struct scan_state_kind {
+  static constexpr int value = (sizeof(T) == 4 ? 1 : (sizeof(T) == 8 ? 2 : 0));
+};
+
+// Exclusive prefix of tile `tile` from its predecessors, computed by the
+// whole workgroup: thread j of a window reads the status of tile
+// hi-(T-1)+j, so one round trip covers T predecessors and the ordered block
+// reduction keeps tile order.  The nearest tile that already holds an
+// inclusive prefix ends the walk.  Returns the same value in every thread.
+template <class State, class V, class Join>
+__device__ V scan_lookback(const State& st, int64_t tile, uint32_t epoch, const V& identity,
+                           const Join& join, V* lds_vals, int* lds_pos) {
+  const int T         = blockDim.x;
+  const int tid       = threadIdx.x;
+  const uint32_t want = (epoch & 0x3fffffffu);
+  V excl              = identity;
+  int64_t hi          = tile - 1;
+  while (true) {
+    const int64_t t = hi - (T - 1) + tid;
+    V v             = identity;
+    uint32_t s      = kScanIncl;
+    if (t >= 0) {
+      uint32_t f = st.read(t, v);
+      int spins  = 0;
+      while ((f >> 2) != want || (f & 3) == 0) {
+        if (++spins > 16) __builtin_amdgcn_s_sleep(1);
+        f = st.read(t, v);
+      }
+      s = f & 3;
+    }
+    if (tid == 0) *lds_pos = -1;
+    __syncthreads();
+    const uint64_t m = wave::ballot(s == kScanIncl);
+    if (wave::lane() == 0 && m) atomicMax(lds_pos, (tid & ~63) + 63 - __builtin_clzll(m));
+    __syncthreads();
+    const int k = *lds_pos;
+    if (k >= 0 && tid < k) v = identity;
+    const V agg = block_reduce(v, identity, join, lds_vals, T);
+    V r         = agg;
+    join(r, excl);
+    excl = r;
+    if (k >= 0) return excl;
+    hi -= T;
+    __syncthreads();  // lds_pos is reset at the top of the next round
+  }
+}
+
+template <class CR, class Index, int IPT, bool WithTotal>
+__global__ __launch_bounds__(1024) void k_range_scan(
+    const CR cr, const int64_t begin, const int64_t end, const int64_t ntiles,
+    ScanTileState<typename CR::value_type> st, unsigned* control,
+    typename CR::value_type* total_out) {
+  using V   = typename CR::value_type;
+  using Tag = typename CR::work_tag;
+  __shared__ LdsSlots16<V> s_wave;  // per-wave totals
+  __shared__ LdsSlots16<V> s_red;   // look-back reduction scratch
+  __shared__ uint64_t s_claim;
+  __shared__ int s_pos;
+  const auto join = [&cr](V& a, const V& b) { cr.join(a, b); };
+  // control[2..3] is one 64-bit word {tile counter (low), epoch (high)}: a
+  // single relaxed fetch_add returns a consistent (epoch, tile) pair, so the
+  // claim needs no fence (an acq_rel claim would write back and invalidate
+  // the L2 at every tile).
+  uint64_t* claim_word = reinterpret_cast<uint64_t*>(control + 2);
+
+  if (threadIdx.x == 0)
+    s_claim = __hip_atomic_fetch_add(claim_word, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const uint64_t claim = s_claim;
+  const int64_t tile   = static_cast<int64_t>(static_cast<uint32_t>(claim));
+  const uint32_t epoch = static_cast<uint32_t>(claim >> 32);
+  const int w          = threadIdx.x >> 6;
+  const int nw         = blockDim.x >> 6;
+  const int l          = wave::lane();
+
+  V identity;
+  cr.init(identity);
+  const int64_t base = begin + tile * (static_cast<int64_t>(blockDim.x) * IPT) +
+                       static_cast<int64_t>(w) * (64 * IPT) + l;
+
+  // Pass 1: all element values first (the loads of every round are in
+  // flight together), then the per-round inclusive wave scans.
+  V incl[IPT];
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    const int64_t i = base + k * 64;
+    incl[k]         = identity;
+    if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), incl[k], false);
+  }
+  V wave_total = identity;
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    incl[k] = wave::inclusive_scan(incl[k], identity, join);
+    join(wave_total, wave::readlane(incl[k], 63));
+  }
+  if (l == 0) s_wave.get()[w] = wave_total;
+  __syncthreads();
+
+  // Every wave scans the wave totals itself (no extra barrier).
+  const V x    = (l < nw) ? s_wave.get()[l] : identity;
+  const V xs   = wave::inclusive_scan(x, identity, join);
+  const V tagg = wave::readlane(xs, nw - 1);
+  const V wpre = (w == 0) ? identity : wave::readlane(xs, w - 1);
+
+  V prefix = identity;
+  if (tile == 0) {
+    if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanIncl), tagg);
+  } else {
+    if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanAgg), tagg);
+    prefix = scan_lookback(st, tile, epoch, identity, join, s_red.get(), &s_pos);
+    if (threadIdx.x == 0) {
+      V incl_t = prefix;
+      join(incl_t, tagg);
+      st.publish(tile, scan_flag(epoch, kScanIncl), incl_t);
+    }
+  }
+  if (threadIdx.x == 0 && tile == ntiles - 1) {
+    if constexpr (WithTotal) {
+      V tot = prefix;
+      join(tot, tagg);
+      *total_out = tot;
+    }
+    // Last tile: every block has claimed; next launch starts at tile 0 with
+    // the next epoch.
+    __hip_atomic_store(claim_word, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+
+  // Pass 2: exclusive prefix of every index first, then the final calls.
+  V running = prefix;
+  join(running, wpre);
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    V upd = running;
+    join(upd, wave::shift_up(incl[k], identity));
+    join(running, wave::readlane(incl[k], 63));
+    incl[k] = upd;
+  }
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    const int64_t i = base + k * 64;
+    if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), incl[k], true);
+  }
+}
+
+// ------------------------------------------------------------------ host helpers
+template <class V>
+struct ResultSink {
+  // Where a reduction/scan total lands: the caller's pointer when the device
+  // can write it, else the instance's pinned staging buffer.
+  V* device_ptr   = nullptr;
+  V* host_target  = nullptr;
+  size_t bytes    = 0;
+  bool needs_copy = false;
+};
+
+template <class V, class ResultMemSpace>
+ResultSink<V> make_sink(MI355XInstance* inst, V* result, size_t count) {
+  ResultSink<V> s;
+  s.bytes = sizeof(V) * count;
+  if constexpr (exec_can_access<MI355X, ResultMemSpace>::value) {
+    s.device_ptr = result;
+  } else {
+    s.device_ptr  = static_cast<V*>(inst->pinned_space(s.bytes));
+    s.host_target = result;
+    s.needs_copy  = (result != nullptr);
+  }
+  return s;
+}
+template <class V>
+void finish_sink(MI355XInstance* inst, const ResultSink<V>& s) {
+  if (s.needs_copy) {
+    KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
+    std::memcpy(static_cast<void*>(s.host_target), s.device_ptr, s.bytes);
+  }
+}
+
+template <class F, class Tag, class Index>
+const void* range_for_kernel(int items) {
+  switch (normalise_items(items)) {
+    case 8: return reinterpret_cast<const void*>(&k_range_for<F, Tag, Index, 8>);
+    case 4: return reinterpret_cast<const void*>(&k_range_for<F, Tag, Index, 4>);
+    case 2: return reinterpret_cast<const void*>(&k_range_for<F, Tag, Index, 2>);
+    default: return reinterpret_cast<const void*>(&k_range_for<F, Tag, Index, 1>);
+  }
+}
+
+template <class CR, class Index>
+const void* range_reduce_kernel(int items) {
+  switch (normalise_items(items)) {
+    case 8: return reinterpret_cast<const void*>(&k_range_reduce<CR, Index, 8>);
+    case 4: return reinterpret_cast<const void*>(&k_range_reduce<CR, Index, 4>);
+    case 2: return reinterpret_cast<const void*>(&k_range_reduce<CR, Index, 2>);
+    default: return reinterpret_cast<const void*>(&k_range_reduce<CR, Index, 1>);
+  }
+}
+
+inline int64_t range_grid(int64_t n, const RangeConfig& c) {
+  const int64_t tile = static_cast<int64_t>(c.block) * c.items;
+  int64_t g          = ceil_div(n, tile);
+  if (c.max_blocks > 0 && g > c.max_blocks) g = c.max_blocks;
+  if (g > 0x7fffffff) g = 0x7fffffff;
+  return g;
+}
+
+// Process-wide launch-shape overrides per pattern (0 for, 1 reduce, 2 scan):
+// set by kkmx_set_range_config for configuration sweeps; a tuning tool's
+// answer for one dispatch takes precedence.
+struct RangeOverrides {
+  RangeConfig cfg[3];
+  bool set[3] = {false, false, false};
+};
+inline RangeOverrides& range_overrides() {
+  static RangeOverrides o;
+  return o;
+}
+
+// Heuristic launch shapes (the learned tuner may replace them through the
+// tuning hooks).
+inline RangeConfig default_for_config(int64_t) { return RangeConfig{256, 1, 0}; }
+inline RangeConfig default_reduce_config(int64_t n) {
+  RangeConfig c{256, 4, 0};
+  const int cus = MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256;
+  c.max_blocks  = cus * 8;  // 32 resident wavefronts per CU
+  if (n < static_cast<int64_t>(c.block) * c.items * c.max_blocks) c.max_blocks = 0;
+  return c;
+}
+
+}  // namespace Impl
+
+// ================================================================== closures
+namespace Impl {
+
+template <class FunctorType, class... Traits>
+class ParallelFor<FunctorType, RangePolicy<Traits...>, MI355X> {
+ public:
+  using Policy = RangePolicy<Traits...>;
+  using Tag    = typename Policy::work_tag;
+  using Index  = typename Policy::member_type;
+
+  ParallelFor(const FunctorType& f, const Policy& p) : m_functor(f), m_policy(p) {}
+  void set_config(const RangeConfig& c) { m_config = c; m_has_config = true; }
+
+  void execute() const {
+    const int64_t b = static_cast<int64_t>(m_policy.begin());
+    const int64_t e = static_cast<int64_t>(m_policy.end());
+    if (e <= b) return;
+    RangeConfig c = m_has_config ? m_config
+                    : range_overrides().set[0] ? range_overrides().cfg[0]
+                                               : default_for_config(e - b);
+    c.block       = normalise_block(c.block);
+    c.items       = normalise_items(c.items);
+    const void* k = range_for_kernel<FunctorType, Tag, Index>(c.items);
+    MI355XInstance* inst = m_policy.space().impl_internal_space_instance();
+    static_assert(sizeof(FunctorType) <= 4000, "functor exceeds the kernel-argument limit");
+    FunctorType f = m_functor;
+    int64_t bb = b, ee = e;
+    mx_launch_kernel(k, static_cast<unsigned>(range_grid(e - b, c)), c.block, 0, inst->stream, f,
+                     bb, ee);
+  }
+
+ private:
+  FunctorType m_functor;
+  Policy m_policy;
+  RangeConfig m_config{};
+  bool m_has_config = false;
+};
+
+// CR = CombinedReducer<F, Tag, Reducer, ValueType>
+template <class CR, class... Traits>
+class ParallelReduce<CR, RangePolicy<Traits...>, MI355X> {
+ public:
+  using Policy = RangePolicy<Traits...>;
+  using Index  = typename Policy::member_type;
+  using V      = typename CR::value_type;
+
+  template <class ResultMem>
+  ParallelReduce(const CR& cr, const Policy& p, V* result, ResultMem)
+      : m_cr(cr), m_policy(p), m_result(result),
+        m_result_on_device(exec_can_access<MI355X, ResultMem>::value) {}
+  void set_config(const RangeConfig& c) { m_config = c; m_has_config = true; }
+
+  void execute() const {
+    MI355XInstance* inst = m_policy.space().impl_internal_space_instance();
+    std::lock_guard<std::mutex> lk(inst->mtx);
+    const int64_t b   = static_cast<int64_t>(m_policy.begin());
+    const int64_t e   = static_cast<int64_t>(m_policy.end());
+    const unsigned cnt = m_cr.value_count();
+    const size_t bytes = sizeof(V) * cnt;
+    V* dst = m_result;
+    bool copy_back = false;
+    if (!m_result_on_device) {
+      dst       = static_cast<V*>(inst->pinned_space(bytes));
+      copy_back = (m_result != nullptr);
+    }
+    if (dst == nullptr) {  // no result requested: still run (final may have effects)
+      dst = static_cast<V*>(inst->pinned_space(bytes));
+    }
+    CR cr     = m_cr;
+    int64_t bb = b, ee = e;
+    unsigned* counter = inst->control;
+    if constexpr (!CR::is_array) {
+      RangeConfig c = m_has_config ? m_config
+                      : range_overrides().set[1] ? range_overrides().cfg[1]
+                                                 : default_reduce_config(e - b);
+      c.block       = normalise_block(c.block);
+      c.items       = normalise_items(c.items);
+      if (e <= b) {
+        c.max_blocks = 1;
+      }
+      int64_t grid = (e <= b) ? 1 : range_grid(e - b, c);
+      V* partials  = static_cast<V*>(inst->partials_space(sizeof(V) * grid));
+      const void* k = range_reduce_kernel<CR, Index>(c.items);
+      mx_launch_kernel(k, static_cast<unsigned>(grid), c.block, 0, inst->stream, cr, bb, ee,
+                       partials, counter, dst);
+    } else {
+      const int block = 256;
+      int64_t grid    = (e <= b) ? 1 : ceil_div(e - b, block);
+      const int cus   = MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256;
+      if (grid > cus * 4) grid = cus * 4;
+      V* partials = static_cast<V*>(inst->partials_space(bytes * grid));
+      const size_t lds = bytes * block;
+      if (lds > 64 * 1024)
+        throw_runtime_exception("Kokkos::MI355X parallel_reduce: value_count too large for LDS");
+      const void* k = reinterpret_cast<const void*>(&k_range_reduce_array<CR, Index>);
+      mx_launch_kernel(k, static_cast<unsigned>(grid), block, lds, inst->stream, cr, bb, ee,
+                       partials, counter, dst);
+    }
+    if (copy_back) {
+      KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
+      std::memcpy(static_cast<void*>(m_result), dst, bytes);
+    }
+  }
+
+ private:
+  CR m_cr;
+  Policy m_policy;
+  V* m_result;
+  bool m_result_on_device;
+  RangeConfig m_config{};
+  bool m_has_config = false;
+};
+
+template <class CR, class Index, bool WithTotal>
+const void* range_scan_kernel(int ipt) {
+  switch (ipt >= 16 ? 16 : ipt >= 8 ? 8 : 4) {
+    case 16: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 16, WithTotal>);
+    case 8: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 8, WithTotal>);
+    default: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 4, WithTotal>);
+  }
+}
+
+// Shared body of ParallelScan and ParallelScanWithTotal.
+template <class CR, class Policy, bool WithTotal>
+void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::value_type* total,
+                        bool total_on_device, const RangeConfig* cfg) {
+  using V     = typename CR::value_type;
+  using Index = typename Policy::member_type;
+  static_assert(!CR::is_array, "parallel_scan does not support runtime-length value types");
+  MI355XInstance* inst = policy.space().impl_internal_space_instance();
+  std::lock_guard<std::mutex> lk(inst->mtx);
+  const int64_t b = static_cast<int64_t>(policy.begin());
+  const int64_t e = static_cast<int64_t>(policy.end());
+  CR cr           = cr_in;
+  if (e <= b) {
+    if (WithTotal && total) {
+      V id;
+      cr.init(id);
+      if (total_on_device) {
+        KKMX_SAFE_CALL(mx_memcpy_async(total, &id, sizeof(V), inst->stream));
+        KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
+      } else {
+        *total = id;
+      }
+    }
+    return;
+  }
+  RangeConfig c = cfg ? *cfg
+                  : range_overrides().set[2] ? range_overrides().cfg[2]
+                                             : RangeConfig{1024, sizeof(V) <= 4 ? 16 : 8, 0};
+  c.block       = normalise_block(c.block);
+  const int ipt = c.items >= 16 ? 16 : c.items >= 8 ? 8 : 4;
+  const int64_t tile   = static_cast<int64_t>(c.block) * ipt;
+  const int64_t ntiles = ceil_div(e - b, tile);
+  ScanTileState<V> st;
+  void* status = inst->scan_status_space(ScanTileState<V>::bytes(ntiles));
+  void* vals   = nullptr;
+  if constexpr (scan_state_kind<V>::value == 0)
+    vals = inst->scan_values_space(ScanTileState<V>::value_bytes(ntiles));
+  st.bind(status, vals);
+  V* tot_dst     = total;
+  bool copy_back = false;
+  if (WithTotal && (!total_on_device || total == nullptr)) {
+    tot_dst   = static_cast<V*>(inst->pinned_space(sizeof(V)));
+    copy_back = (total != nullptr);
+  }
+  int64_t bb = b, ee = e, nt = ntiles;
+  unsigned* control = inst->control;
+  const void* k     = range_scan_kernel<CR, Index, WithTotal>(ipt);
+  mx_launch_kernel(k, static_cast<unsigned>(ntiles), c.block, 0, inst->stream, cr, bb, ee, nt, st,
+                   control, tot_dst);
+  if (copy_back) {
+    KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
+    std::memcpy(static_cast<void*>(total), tot_dst, sizeof(V));
+  }
+}
+
+template <class CR, class... Traits>
+class ParallelScan<CR, RangePolicy<Traits...>, MI355X> {
+ public:
+  using Policy = RangePolicy<Traits...>;
+  ParallelScan(const CR& cr, const Policy& p) : m_cr(cr), m_policy(p) {}
+  void set_config(const RangeConfig& c) { m_config = c; m_has_config = true; }
+  void execute() const {
+    range_scan_execute<CR, Policy, false>(m_cr, m_policy, nullptr, true,
+                                          m_has_config ? &m_config : nullptr);
+  }
+
+ private:
+  CR m_cr;
+  Policy m_policy;
+  RangeConfig m_config{};
+  bool m_has_config = false;
+};
+
+template <class CR, class ReturnType, class... Traits>
+class ParallelScanWithTotal<CR, RangePolicy<Traits...>, ReturnType, MI355X> {
+ public:
+  using Policy = RangePolicy<Traits...>;
+  using V      = typename CR::value_type;
+  ParallelScanWithTotal(const CR& cr, const Policy& p, V* total, bool total_on_device)
+      : m_cr(cr), m_policy(p), m_total(total), m_on_device(total_on_device) {}
+  void set_config(const RangeConfig& c) { m_config = c; m_has_config = true; }
+  void execute() const {
+    range_scan_execute<CR, Policy, true>(m_cr, m_policy, m_total, m_on_device,
+                                         m_has_config ? &m_config : nullptr);
+  }
+
+ private:
+  CR m_cr;
+  Policy m_policy;
+  V* m_total;
+  bool m_on_device;
+  RangeConfig m_config{};
+  bool m_has_config = false;
+};
+
+}  // namespace Impl
+}  // namespace Kokkos
+
+#endif
