@@ -1,0 +1,164 @@
+"""CPU oracle: TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings for oracle/liboracle.so (a plain-C restatement of the
+reference kokkos Serial/OpenMP semantics on the hot path, see kkmx_oracle.c).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline use it, and
+only as the checker.  The product (kokkos_amd/) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+i64, u64, i32, f64, vp = C.c_int64, C.c_uint64, C.c_int32, C.c_double, C.c_void_p
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        sig = {
+            "or_splitmix64": ([u64, u64], u64),
+            "or_fill_uniform_f64": ([vp, i64, u64, f64, f64], None),
+            "or_fill_uniform_i64": ([vp, i64, u64, i64, i64], None),
+            "or_fill_uniform_i32": ([vp, i64, u64, i32, i32], None),
+            "or_xorshift64star": ([vp, i64, u64], None),
+            "or_checksum64": ([vp, i64], u64),
+            "or_sum_f64_serial": ([vp, i64], f64),
+            "or_sum_f64_openmp": ([vp, i64, C.c_int], f64),
+            "or_sum_f64_neumaier": ([vp, i64], f64),
+            "or_sum_f64_exact": ([vp, i64], f64),
+            "or_reduce_i64": ([vp, i64, C.c_int, C.c_int], i64),
+            "or_reduce_i32": ([vp, i64, C.c_int, C.c_int], i64),
+            "or_scan_excl_i32_i64": ([vp, vp, i64, C.c_int, vp], None),
+            "or_scan_incl_i32": ([vp, vp, i64, C.c_int, vp], None),
+            "or_scan_excl_i64": ([vp, vp, i64, C.c_int, vp], None),
+            "or_scan_excl_f64": ([vp, vp, i64, C.c_int, vp], None),
+            "or_stream_copy": ([vp, vp, i64, C.c_int], None),
+            "or_stream_scale": ([vp, vp, i64, f64, C.c_int], None),
+            "or_stream_add": ([vp, vp, vp, i64, C.c_int], None),
+            "or_stream_triad": ([vp, vp, vp, i64, f64, C.c_int, C.c_int], None),
+            "or_max_threads": ([], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------- numpy helpers
+def fill_uniform_f64(n, seed, lo, hi):
+    x = np.empty(n, np.float64)
+    lib().or_fill_uniform_f64(_p(x), n, seed, lo, hi)
+    return x
+
+
+def fill_uniform_i64(n, seed, lo, hi):
+    x = np.empty(n, np.int64)
+    lib().or_fill_uniform_i64(_p(x), n, seed, lo, hi)
+    return x
+
+
+def fill_uniform_i32(n, seed, lo, hi):
+    x = np.empty(n, np.int32)
+    lib().or_fill_uniform_i32(_p(x), n, seed, lo, hi)
+    return x
+
+
+def xorshift64star(n, seed):
+    x = np.empty(n, np.uint64)
+    lib().or_xorshift64star(_p(x), n, seed)
+    return x
+
+
+def checksum(a: np.ndarray) -> int:
+    a = np.ascontiguousarray(a)
+    return int(lib().or_checksum64(_p(a), a.nbytes))
+
+
+def sum_serial(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().or_sum_f64_serial(_p(x), x.size)
+
+
+def sum_openmp(x, nthreads):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().or_sum_f64_openmp(_p(x), x.size, nthreads)
+
+
+def sum_neumaier(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().or_sum_f64_neumaier(_p(x), x.size)
+
+
+def sum_exact(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().or_sum_f64_exact(_p(x), x.size)
+
+
+def reduce_i64(x, op, nthreads=1):
+    x = np.ascontiguousarray(x, np.int64)
+    return int(lib().or_reduce_i64(_p(x), x.size, op, nthreads))
+
+
+def reduce_i32(x, op, nthreads=1):
+    x = np.ascontiguousarray(x, np.int32)
+    return int(lib().or_reduce_i32(_p(x), x.size, op, nthreads))
+
+
+def scan_excl_i32_i64(x, nthreads=1):
+    x = np.ascontiguousarray(x, np.int32)
+    out = np.empty(x.size, np.int64)
+    tot = np.zeros(1, np.int64)
+    lib().or_scan_excl_i32_i64(_p(x), _p(out), x.size, nthreads, _p(tot))
+    return out, int(tot[0])
+
+
+def scan_incl_i32(x, nthreads=1):
+    x = np.ascontiguousarray(x, np.int32)
+    out = np.empty(x.size, np.int32)
+    tot = np.zeros(1, np.int32)
+    lib().or_scan_incl_i32(_p(x), _p(out), x.size, nthreads, _p(tot))
+    return out
+
+
+def scan_excl_i64(x, nthreads=1):
+    x = np.ascontiguousarray(x, np.int64)
+    out = np.empty(x.size, np.int64)
+    tot = np.zeros(1, np.int64)
+    lib().or_scan_excl_i64(_p(x), _p(out), x.size, nthreads, _p(tot))
+    return out, int(tot[0])
+
+
+def scan_excl_f64(x, nthreads=1):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty(x.size, np.float64)
+    tot = np.zeros(1, np.float64)
+    lib().or_scan_excl_f64(_p(x), _p(out), x.size, nthreads, _p(tot))
+    return out, float(tot[0])
+
+
+def stream_triad(b, c, s, fused=True, nthreads=1):
+    a = np.empty_like(b)
+    lib().or_stream_triad(_p(a), _p(b), _p(c), b.size, s, 1 if fused else 0, nthreads)
+    return a
