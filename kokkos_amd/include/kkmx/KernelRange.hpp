@@ -283,6 +283,13 @@ struct ScanTileState<T, 0> {
   }
 };
 
+template <class CR>
+__device__ __forceinline__ typename CR::value_type cr_identity(const CR& cr) {
+  typename CR::value_type v;
+  cr.init(v);
+  return v;
+}
+
 template <class T>
 struct scan_state_kind {
   static constexpr int value = (sizeof(T) == 4 ? 1 : (sizeof(T) == 8 ? 2 : 0));
@@ -331,13 +338,27 @@ __device__ V scan_lookback(const State& st, int64_t tile, uint32_t epoch, const 
   }
 }
 
+// One tile per workgroup, claimed in launch order.  Wavefront w owns the
+// 64*IPT consecutive indices starting at tile_base + w*64*IPT and walks them
+// in rounds of 64 (one index per lane, coalesced), B rounds at a time so B
+// loads per lane are in flight together.
+//   pass 1: f(i, v, false) per index, ordered wave reduction per round ->
+//           wave totals -> tile aggregate (published), nothing kept in
+//           registers, so occupancy stays high;
+//   look-back for the tile's exclusive prefix (whole workgroup);
+//   pass 2: the same rounds again (the tile is still in L2 / the memory-side
+//           cache), per-round inclusive wave scans, then f(i, prefix, true).
+// The functor sees final=false twice and final=true once per index, which
+// the reference allows (core/src/HIP/Kokkos_HIP_ParallelScan_Range.hpp calls
+// it three times per index).
 template <class CR, class Index, int IPT, bool WithTotal>
 __global__ __launch_bounds__(1024) void k_range_scan(
     const CR cr, const int64_t begin, const int64_t end, const int64_t ntiles,
     ScanTileState<typename CR::value_type> st, unsigned* control,
     typename CR::value_type* total_out) {
-  using V   = typename CR::value_type;
-  using Tag = typename CR::work_tag;
+  using V          = typename CR::value_type;
+  using Tag        = typename CR::work_tag;
+  constexpr int B  = IPT < 8 ? IPT : 8;
   __shared__ LdsSlots16<V> s_wave;  // per-wave totals
   __shared__ LdsSlots16<V> s_red;   // look-back reduction scratch
   __shared__ uint64_t s_claim;
@@ -348,7 +369,6 @@ __global__ __launch_bounds__(1024) void k_range_scan(
   // claim needs no fence (an acq_rel claim would write back and invalidate
   // the L2 at every tile).
   uint64_t* claim_word = reinterpret_cast<uint64_t*>(control + 2);
-
   if (threadIdx.x == 0)
     s_claim = __hip_atomic_fetch_add(claim_word, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
@@ -358,31 +378,27 @@ __global__ __launch_bounds__(1024) void k_range_scan(
   const int w          = threadIdx.x >> 6;
   const int nw         = blockDim.x >> 6;
   const int l          = wave::lane();
-
   V identity;
   cr.init(identity);
   const int64_t base = begin + tile * (static_cast<int64_t>(blockDim.x) * IPT) +
                        static_cast<int64_t>(w) * (64 * IPT) + l;
 
-  // Pass 1: all element values first (the loads of every round are in
-  // flight together), then the per-round inclusive wave scans.
-  V incl[IPT];
-#pragma unroll
-  for (int k = 0; k < IPT; ++k) {
-    const int64_t i = base + k * 64;
-    incl[k]         = identity;
-    if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), incl[k], false);
-  }
+  // Pass 1: ordered reduction of the wavefront's indices.
   V wave_total = identity;
 #pragma unroll
-  for (int k = 0; k < IPT; ++k) {
-    incl[k] = wave::inclusive_scan(incl[k], identity, join);
-    join(wave_total, wave::readlane(incl[k], 63));
+  for (int r0 = 0; r0 < IPT; r0 += B) {
+    V v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int64_t i = base + (r0 + b) * 64;
+      v[b]            = identity;
+      if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), v[b], false);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) join(wave_total, wave::reduce(v[b], identity, join));
   }
   if (l == 0) s_wave.get()[w] = wave_total;
   __syncthreads();
-
-  // Every wave scans the wave totals itself (no extra barrier).
   const V x    = (l < nw) ? s_wave.get()[l] : identity;
   const V xs   = wave::inclusive_scan(x, identity, join);
   const V tagg = wave::readlane(xs, nw - 1);
@@ -406,26 +422,37 @@ __global__ __launch_bounds__(1024) void k_range_scan(
       join(tot, tagg);
       *total_out = tot;
     }
-    // Last tile: every block has claimed; next launch starts at tile 0 with
-    // the next epoch.
+    // Last tile: every workgroup has claimed; the next launch starts at tile
+    // 0 with the next epoch.
     __hip_atomic_store(claim_word, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   }
 
-  // Pass 2: exclusive prefix of every index first, then the final calls.
+  // Pass 2: values again, per-round scans, final calls.
   V running = prefix;
   join(running, wpre);
 #pragma unroll
-  for (int k = 0; k < IPT; ++k) {
-    V upd = running;
-    join(upd, wave::shift_up(incl[k], identity));
-    join(running, wave::readlane(incl[k], 63));
-    incl[k] = upd;
-  }
+  for (int r0 = 0; r0 < IPT; r0 += B) {
+    V v[B];
 #pragma unroll
-  for (int k = 0; k < IPT; ++k) {
-    const int64_t i = base + k * 64;
-    if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), incl[k], true);
+    for (int b = 0; b < B; ++b) {
+      const int64_t i = base + (r0 + b) * 64;
+      v[b]            = identity;
+      if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), v[b], false);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const V inc = wave::inclusive_scan(v[b], identity, join);
+      V upd       = running;
+      join(upd, wave::shift_up(inc, identity));
+      join(running, wave::readlane(inc, 63));
+      v[b] = upd;
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int64_t i = base + (r0 + b) * 64;
+      if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), v[b], true);
+    }
   }
 }
 
@@ -632,10 +659,11 @@ class ParallelReduce<CR, RangePolicy<Traits...>, MI355X> {
 
 template <class CR, class Index, bool WithTotal>
 const void* range_scan_kernel(int ipt) {
-  switch (ipt >= 16 ? 16 : ipt >= 8 ? 8 : 4) {
+  switch (ipt >= 64 ? 64 : ipt >= 32 ? 32 : ipt >= 16 ? 16 : 8) {
+    case 64: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 64, WithTotal>);
+    case 32: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 32, WithTotal>);
     case 16: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 16, WithTotal>);
-    case 8: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 8, WithTotal>);
-    default: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 4, WithTotal>);
+    default: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 8, WithTotal>);
   }
 }
 
@@ -666,9 +694,9 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
   }
   RangeConfig c = cfg ? *cfg
                   : range_overrides().set[2] ? range_overrides().cfg[2]
-                                             : RangeConfig{1024, sizeof(V) <= 4 ? 16 : 8, 0};
+                                             : RangeConfig{1024, 32, 0};
   c.block       = normalise_block(c.block);
-  const int ipt = c.items >= 16 ? 16 : c.items >= 8 ? 8 : 4;
+  const int ipt = c.items >= 64 ? 64 : c.items >= 32 ? 32 : c.items >= 16 ? 16 : 8;
   const int64_t tile   = static_cast<int64_t>(c.block) * ipt;
   const int64_t ntiles = ceil_div(e - b, tile);
   ScanTileState<V> st;
