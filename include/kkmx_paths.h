@@ -68,6 +68,44 @@ int kkmx_scan_incl_i32(const int32_t* in_dev, int32_t* out_dev, int64_t n);
 int kkmx_scan_excl_i64(const int64_t* in_dev, int64_t* out_dev, int64_t n, int64_t* total_dev);
 int kkmx_scan_excl_f64(const double* in_dev, double* out_dev, int64_t n, double* total_dev);
 
+/* ---- MDRangePolicy<Rank<3>> 7-point stencil (config 3) ----
+ * u_dev, v_dev: (n+2)^3 doubles, LayoutLeft (first index fastest), 1-cell
+ * halo; v(i,j,k) = c0*u(i,j,k) + c1*(sum of the 6 neighbours) for the
+ * interior.  t0..t2 = tile shape (0 = the space's default).  The sweeps
+ * variant ping-pongs u -> v -> u ...  (reference: core/src/KokkosExp_MDRangePolicy.hpp:158-426) */
+int kkmx_stencil7(const double* u_dev, double* v_dev, int64_t n, double c0, double c1, int t0, int t1, int t2);
+int kkmx_stencil7_sweeps(double* u_dev, double* v_dev, int64_t n, int sweeps, double c0, double c1, int t0, int t1,
+                         int t2);
+
+/* ---- TeamPolicy CSR SpMV (config 3) ----
+ * Seeded CSR: nnz of row r uniform in [nnz_lo, nnz_hi], columns uniform in
+ * [0, ncols), values uniform in [-1, 1).  kkmx_csr_generate fills rowptr
+ * (nrows+1 entries) and, when col/val are given, the nonzeros; *nnz_out gets
+ * the count.  kkmx_spmv_csr: variant 0 = one row per index (RangePolicy),
+ * 1 = TeamPolicy with the x-gather staged in team scratch (LDS).
+ * (reference: core/src/Kokkos_ExecPolicy.hpp:365-980, Kokkos_ScratchSpace.hpp:38-170) */
+int kkmx_csr_generate(int64_t* rowptr_dev, int32_t* col_dev, double* val_dev, int64_t nrows, int64_t ncols,
+                      int64_t nnz_lo, int64_t nnz_hi, uint64_t seed, int64_t* nnz_out);
+int kkmx_csr_fill(int32_t* col_dev, double* val_dev, int64_t nnz, int64_t ncols, uint64_t seed);
+int kkmx_spmv_csr(const int64_t* rowptr_dev, const int32_t* col_dev, const double* val_dev, const double* x_dev,
+                  double* y_dev, int64_t nrows, int64_t nnz, int variant, int team_size, int vector_length);
+
+/* ---- atomics (config 4) ----
+ * kkmx_xorshift64_stream: n outputs of Random_XorShift64::urand64 from seed
+ *   (algorithms/src/Kokkos_Random.hpp:788-808), generated in parallel by
+ *   GF(2) jump-ahead.
+ * kkmx_gups: table[stream[2u] & (table_size-1)] += stream[2u+1] with
+ *   atomic_add (core/src/Kokkos_Atomics_Desul_Wrapper.hpp:120).
+ * kkmx_zipf_keys: Zipf(s) keys over [0, nbins) by inverse CDF.
+ * kkmx_histogram: mode 0 = global atomics, 1 = bins [0, lds_bins) privatised
+ *   in team scratch (LDS atomics), the rest global.
+ * kkmx_scatter_add_f64: bins[idx[i]] += vals[i] with FP64 atomic_add. */
+int kkmx_xorshift64_stream(uint64_t* out_dev, int64_t n, uint64_t seed);
+int kkmx_gups(int64_t* table_dev, int64_t table_size, const uint64_t* stream_dev, int64_t nupdates);
+int kkmx_zipf_keys(int32_t* keys_dev, int64_t n, double s, int32_t nbins, uint64_t seed);
+int kkmx_histogram(const int32_t* keys_dev, int64_t n, int32_t* bins_dev, int32_t nbins, int mode, int lds_bins);
+int kkmx_scatter_add_f64(const double* vals_dev, const int32_t* idx_dev, int64_t n, double* bins_dev, int32_t nbins);
+
 /* ---- launch-shape overrides (what the tuning hooks set), -1 = heuristic ---- */
 int kkmx_set_range_config(int pattern, int block, int items, int max_blocks);
 

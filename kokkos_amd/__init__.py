@@ -97,6 +97,16 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_scan_excl_i64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_scan_excl_f64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_set_range_config", [C.c_int, C.c_int, C.c_int, C.c_int])
+        _sig(lib, "kkmx_stencil7", [vp, vp, i64, f64, f64, C.c_int, C.c_int, C.c_int])
+        _sig(lib, "kkmx_stencil7_sweeps", [vp, vp, i64, C.c_int, f64, f64, C.c_int, C.c_int, C.c_int])
+        _sig(lib, "kkmx_csr_generate", [vp, vp, vp, i64, i64, i64, i64, u64, C.POINTER(i64)])
+        _sig(lib, "kkmx_csr_fill", [vp, vp, i64, i64, u64])
+        _sig(lib, "kkmx_spmv_csr", [vp, vp, vp, vp, vp, i64, i64, C.c_int, C.c_int, C.c_int])
+        _sig(lib, "kkmx_xorshift64_stream", [vp, i64, u64])
+        _sig(lib, "kkmx_gups", [vp, i64, vp, i64])
+        _sig(lib, "kkmx_zipf_keys", [vp, i64, f64, i32, u64])
+        _sig(lib, "kkmx_histogram", [vp, i64, vp, i32, C.c_int, C.c_int])
+        _sig(lib, "kkmx_scatter_add_f64", [vp, vp, i64, vp, i32])
     elif name == "kkmx_peak":
         _sig(lib, "kkmx_peak_copy", [vp, vp, i64, C.c_int, C.c_int, vp])
 

@@ -6,7 +6,9 @@
 // instance.  Nothing here calls HIP directly.
 #include <Kokkos_Core.hpp>
 
+#include <cmath>
 #include <string>
+#include <vector>
 
 #include "kkmx_paths.h"
 
@@ -155,9 +157,240 @@ struct InclusiveScan {
   }
 };
 
+
+// ---------------------------------------------------------------- stencil
+// 7-point Laplacian-type update over the interior of an (n+2)^3 LayoutLeft
+// grid (first index fastest), ping-ponged between two Views.
+using Grid3 = Kokkos::View<double***, Kokkos::LayoutLeft, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+struct Stencil7 {
+  Grid3 u, v;
+  double c0, c1;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, const int64_t j, const int64_t k) const {
+    const double s = u(i - 1, j, k) + u(i + 1, j, k) + u(i, j - 1, k) + u(i, j + 1, k) + u(i, j, k - 1) + u(i, j, k + 1);
+    v(i, j, k)     = c0 * u(i, j, k) + c1 * s;
+  }
+};
+struct Stencil7Residual {  // max |u - v| over the interior, for checks
+  Grid3 u, v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, const int64_t j, const int64_t k, double& m) const {
+    const double d = fabs(u(i, j, k) - v(i, j, k));
+    if (d > m) m = d;
+  }
+};
+
+// ---------------------------------------------------------------- CSR SpMV
+// One team per block of rows; the team first gathers x[col[k]] for every
+// nonzero of its rows into level-0 scratch (LDS) with all lanes of the team
+// (coalesced reads of col, many independent gathers in flight), then each
+// thread forms one row's dot product from LDS with its vector lanes.
+struct SpMVTeam {
+  DevView<const int64_t> rp;
+  DevView<const int32_t> col;
+  DevView<const double> val, x;
+  DevView<double> y;
+  int64_t nrows;
+  int rows_per_team;
+  int max_team_nnz;  // scratch capacity in values
+  using member_type = Kokkos::TeamPolicy<Kokkos::MI355X>::member_type;
+  KOKKOS_INLINE_FUNCTION void operator()(const member_type& m) const {
+    const int64_t r0 = int64_t(m.league_rank()) * rows_per_team;
+    int64_t r1       = r0 + rows_per_team;
+    if (r1 > nrows) r1 = nrows;
+    const int64_t k0 = rp(r0), k1 = rp(r1);
+    Kokkos::View<double*, Kokkos::MI355X::scratch_memory_space, Kokkos::MemoryTraits<Kokkos::Unmanaged>> xs(
+        m.team_scratch(0), max_team_nnz);
+    const bool staged = (k1 - k0) <= max_team_nnz;
+    if (staged) {
+      Kokkos::parallel_for(Kokkos::TeamVectorRange(m, k0, k1), [&](const int64_t k) { xs(k - k0) = x(col(k)); });
+      m.team_barrier();
+    }
+    for (int64_t row = r0 + m.team_rank(); row < r1; row += m.team_size()) {
+      double sum = 0.0;
+      Kokkos::parallel_reduce(
+          Kokkos::ThreadVectorRange(m, rp(row), rp(row + 1)),
+          [&](const int64_t k, double& t) { t += val(k) * (staged ? xs(k - k0) : x(col(k))); }, sum);
+      Kokkos::single(Kokkos::PerThread(m), [&]() { y(row) = sum; });
+    }
+  }
+};
+// Plain row-per-thread SpMV (no scratch), for comparison.
+struct SpMVRow {
+  DevView<const int64_t> rp;
+  DevView<const int32_t> col;
+  DevView<const double> val, x;
+  DevView<double> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t row) const {
+    double s = 0.0;
+    for (int64_t k = rp(row); k < rp(row + 1); ++k) s += val(k) * x(col(k));
+    y(row) = s;
+  }
+};
+// CSR generation: nnz of row r = lo + mulhi(splitmix(seed, r), hi-lo+1);
+// column of nonzero k = mulhi(splitmix(seed ^ C1, k), ncols); value of k from
+// splitmix(seed ^ C2, k) in [-1, 1).
+constexpr uint64_t kColSalt = 0x5bd1e9955bd1e995ull;
+constexpr uint64_t kValSalt = 0xc2b2ae3d27d4eb4full;
+struct CsrRowNnz {
+  DevView<int64_t> cnt;
+  uint64_t seed;
+  int64_t lo;
+  uint64_t range;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t r) const {
+    cnt(r) = lo + static_cast<int64_t>(mulhi64(splitmix64(seed, static_cast<uint64_t>(r)), range));
+  }
+};
+struct CsrRowPtr {  // exclusive scan of counts into rp[0..n], rp[n] = total
+  DevView<const int64_t> cnt;
+  DevView<int64_t> rp;
+  int64_t n;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t r, int64_t& upd, const bool final) const {
+    if (final) rp(r) = upd;
+    upd += cnt(r);
+    if (final && r == n - 1) rp(n) = upd;
+  }
+};
+struct CsrFill {
+  DevView<int32_t> col;
+  DevView<double> val;
+  uint64_t seed;
+  uint64_t ncols;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t k) const {
+    col(k) = static_cast<int32_t>(mulhi64(splitmix64(seed ^ kColSalt, static_cast<uint64_t>(k)), ncols));
+    const double u = static_cast<double>(splitmix64(seed ^ kValSalt, static_cast<uint64_t>(k)) >> 11) * 0x1.0p-53;
+    val(k)         = fma(2.0, u, -1.0);
+  }
+};
+
+// ---------------------------------------------------------------- atomics
+// Random_XorShift64 (algorithms/src/Kokkos_Random.hpp:788-808) outputs,
+// generated in parallel: chunk t starts from the state M^(t*K) s0, where M
+// is the GF(2) matrix of one xorshift step (the host computes the starts).
+struct XorShiftChunks {
+  DevView<uint64_t> out;
+  DevView<const uint64_t> starts;
+  int64_t n, chunk;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t t) const {
+    uint64_t s      = starts(t);
+    const int64_t b = t * chunk;
+    int64_t e       = b + chunk;
+    if (e > n) e = n;
+    for (int64_t i = b; i < e; ++i) {
+      s ^= s >> 12;
+      s ^= s << 25;
+      s ^= s >> 27;
+      out(i) = s * 2685821657736338717ull - 1;
+    }
+  }
+};
+struct Gups {
+  DevView<int64_t> table;
+  DevView<const uint64_t> stream;
+  uint64_t mask;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t u) const {
+    const uint64_t idx = stream(2 * u) & mask;
+    Kokkos::atomic_add(&table(idx), static_cast<int64_t>(stream(2 * u + 1)));
+  }
+};
+// Zipf(s) keys over [0, nbins) by inverse CDF: key = first k with u < cdf[k].
+struct ZipfKeys {
+  DevView<int32_t> keys;
+  DevView<const double> cdf;
+  uint64_t seed;
+  int32_t nbins;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const {
+    const double u = static_cast<double>(splitmix64(seed, static_cast<uint64_t>(i)) >> 11) * 0x1.0p-53;
+    int32_t lo = 0, hi = nbins - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (u < cdf(mid)) hi = mid;
+      else lo = mid + 1;
+    }
+    keys(i) = lo;
+  }
+};
+struct HistGlobal {
+  DevView<const int32_t> keys;
+  DevView<int32_t> bins;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { Kokkos::atomic_add(&bins(keys(i)), 1); }
+};
+// LDS-privatised histogram: every team keeps bins [0, lds_bins) in level-0
+// scratch, counts its share of the keys there with LDS atomics, and keys at
+// or above lds_bins go straight to global atomics; the team then adds its
+// nonzero LDS bins to the global histogram.
+struct HistPrivatised {
+  DevView<const int32_t> keys;
+  DevView<int32_t> bins;
+  int64_t n;
+  int32_t lds_bins;
+  using member_type = Kokkos::TeamPolicy<Kokkos::MI355X>::member_type;
+  KOKKOS_INLINE_FUNCTION void operator()(const member_type& m) const {
+    Kokkos::View<int32_t*, Kokkos::MI355X::scratch_memory_space, Kokkos::MemoryTraits<Kokkos::Unmanaged>> h(
+        m.team_scratch(0), lds_bins);
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, lds_bins), [&](const int b) { h(b) = 0; });
+    m.team_barrier();
+    const int64_t nteams = m.league_size();
+    const int64_t chunk  = (n + nteams - 1) / nteams;
+    const int64_t b0     = chunk * m.league_rank();
+    int64_t b1           = b0 + chunk;
+    if (b1 > n) b1 = n;
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, b0, b1), [&](const int64_t i) {
+      const int32_t k = keys(i);
+      if (k < lds_bins) Kokkos::atomic_add(&h(k), 1);
+      else Kokkos::atomic_add(&bins(k), 1);
+    });
+    m.team_barrier();
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, lds_bins), [&](const int b) {
+      const int32_t c = h(b);
+      if (c) Kokkos::atomic_add(&bins(b), c);
+    });
+  }
+};
+struct ScatterAdd {
+  DevView<const double> vals;
+  DevView<const int32_t> idx;
+  DevView<double> bins;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { Kokkos::atomic_add(&bins(idx(i)), vals(i)); }
+};
+
 using Range64 = Kokkos::RangePolicy<Kokkos::IndexType<int64_t>>;
 
 }  // namespace
+
+
+// GF(2) 64x64 matrices as 64 column words: M * s = xor of columns where s has 1s.
+struct Gf2Mat {
+  uint64_t col[64];
+};
+static uint64_t gf2_apply(const Gf2Mat& m, uint64_t s) {
+  uint64_t r = 0;
+  for (int b = 0; b < 64; ++b)
+    if ((s >> b) & 1) r ^= m.col[b];
+  return r;
+}
+static Gf2Mat gf2_mul(const Gf2Mat& a, const Gf2Mat& b) {  // a * b
+  Gf2Mat r;
+  for (int c = 0; c < 64; ++c) r.col[c] = gf2_apply(a, b.col[c]);
+  return r;
+}
+static uint64_t xorshift_step(uint64_t s) {
+  s ^= s >> 12;
+  s ^= s << 25;
+  s ^= s >> 27;
+  return s;
+}
+static Gf2Mat xorshift_matrix_pow(uint64_t k) {
+  Gf2Mat base, acc;
+  for (int c = 0; c < 64; ++c) {
+    base.col[c] = xorshift_step(uint64_t(1) << c);
+    acc.col[c]  = uint64_t(1) << c;
+  }
+  while (k) {
+    if (k & 1) acc = gf2_mul(base, acc);
+    base = gf2_mul(base, base);
+    k >>= 1;
+  }
+  return acc;
+}
 
 namespace {
 template <class T>
@@ -329,6 +562,146 @@ int kkmx_scan_excl_f64(const double* in, double* out, int64_t n, double* total) 
     } else {
       Kokkos::parallel_scan("kkmx::scan_excl_f64", Range64(0, n), f);
     }
+  });
+}
+
+
+// ---------------------------------------------------------------- config 3
+int kkmx_stencil7(const double* u, double* v, int64_t n, double c0, double c1, int t0, int t1, int t2) {
+  if (n < 1 || !u || !v) return reject("kkmx_stencil7: bad arguments");
+  return guarded([&] {
+    Grid3 U(const_cast<double*>(u), n + 2, n + 2, n + 2), V(v, n + 2, n + 2, n + 2);
+    using P = Kokkos::MDRangePolicy<Kokkos::Rank<3>, Kokkos::IndexType<int64_t>>;
+    if (t0 > 0)
+      Kokkos::parallel_for("kkmx::stencil7", P({1, 1, 1}, {n + 1, n + 1, n + 1}, {t0, t1, t2}), Stencil7{U, V, c0, c1});
+    else
+      Kokkos::parallel_for("kkmx::stencil7", P({1, 1, 1}, {n + 1, n + 1, n + 1}), Stencil7{U, V, c0, c1});
+  });
+}
+int kkmx_stencil7_sweeps(double* u, double* v, int64_t n, int sweeps, double c0, double c1, int t0, int t1, int t2) {
+  for (int s = 0; s < sweeps; ++s) {
+    int rc = (s % 2 == 0) ? kkmx_stencil7(u, v, n, c0, c1, t0, t1, t2) : kkmx_stencil7(v, u, n, c0, c1, t0, t1, t2);
+    if (rc) return rc;
+  }
+  return 0;
+}
+int kkmx_csr_generate(int64_t* rowptr, int32_t* col, double* val, int64_t nrows, int64_t ncols, int64_t nnz_lo,
+                      int64_t nnz_hi, uint64_t seed, int64_t* nnz_out) {
+  if (nrows < 1 || !rowptr || nnz_hi < nnz_lo) return reject("kkmx_csr_generate: bad arguments");
+  return guarded([&] {
+    Kokkos::View<int64_t*, Kokkos::MI355XSpace> cnt("csr_counts", nrows);
+    DevView<int64_t> RP(rowptr, nrows + 1);
+    Kokkos::parallel_for("kkmx::csr_counts", Range64(0, nrows),
+                         CsrRowNnz{DevView<int64_t>(cnt.data(), nrows), seed, nnz_lo, uint64_t(nnz_hi - nnz_lo + 1)});
+    Kokkos::parallel_scan("kkmx::csr_rowptr", Range64(0, nrows), CsrRowPtr{DevView<const int64_t>(cnt.data(), nrows), RP, nrows});
+    int64_t nnz = 0;
+    KKMX_SAFE_CALL(mx_stream_sync(Kokkos::MI355X().stream()));
+    KKMX_SAFE_CALL(mx_memcpy(&nnz, rowptr + nrows, sizeof(int64_t)));
+    *nnz_out = nnz;
+    if (col && val)
+      Kokkos::parallel_for("kkmx::csr_fill", Range64(0, nnz),
+                           CsrFill{DevView<int32_t>(col, nnz), DevView<double>(val, nnz), seed, uint64_t(ncols)});
+  });
+}
+int kkmx_csr_fill(int32_t* col, double* val, int64_t nnz, int64_t ncols, uint64_t seed) {
+  if (nnz < 0 || (nnz && (!col || !val))) return reject("kkmx_csr_fill: bad arguments");
+  return guarded([&] {
+    Kokkos::parallel_for("kkmx::csr_fill", Range64(0, nnz),
+                         CsrFill{DevView<int32_t>(col, nnz), DevView<double>(val, nnz), seed, uint64_t(ncols)});
+  });
+}
+int kkmx_spmv_csr(const int64_t* rowptr, const int32_t* col, const double* val, const double* x, double* y,
+                  int64_t nrows, int64_t nnz, int variant, int team_size, int vector_length) {
+  if (nrows < 1 || !rowptr || !x || !y) return reject("kkmx_spmv_csr: bad arguments");
+  return guarded([&] {
+    DevView<const int64_t> RP(rowptr, nrows + 1);
+    DevView<const int32_t> C(col, nnz);
+    DevView<const double> V(val, nnz), X(x, nrows);
+    DevView<double> Y(y, nrows);
+    if (variant == 0) {
+      Kokkos::parallel_for("kkmx::spmv_row", Range64(0, nrows), SpMVRow{RP, C, V, X, Y});
+      return;
+    }
+    const int ts = team_size > 0 ? team_size : 64;
+    const int vl = vector_length > 0 ? vector_length : 8;
+    const int rows_per_team = ts;
+    const int cap  = rows_per_team * 72;  // scratch values per team (mean row length 34)
+    const int64_t league = (nrows + rows_per_team - 1) / rows_per_team;
+    auto pol = Kokkos::TeamPolicy<Kokkos::MI355X>(static_cast<int>(league), ts, vl)
+                   .set_scratch_size(0, Kokkos::PerTeam(size_t(cap) * sizeof(double)));
+    Kokkos::parallel_for("kkmx::spmv_team", pol, SpMVTeam{RP, C, V, X, Y, nrows, rows_per_team, cap});
+  });
+}
+
+// ---------------------------------------------------------------- config 4
+int kkmx_xorshift64_stream(uint64_t* out, int64_t n, uint64_t seed) {
+  if (n < 0 || (n && !out)) return reject("kkmx_xorshift64_stream: bad arguments");
+  return guarded([&] {
+    const int64_t chunk = 4096;
+    const int64_t nchunks = (n + chunk - 1) / chunk;
+    std::vector<uint64_t> starts(static_cast<size_t>(nchunks));
+    const Gf2Mat jump = xorshift_matrix_pow(static_cast<uint64_t>(chunk));
+    uint64_t s = seed == 0 ? 1318319ull : seed;
+    for (int64_t t = 0; t < nchunks; ++t) {
+      starts[t] = s;
+      s         = gf2_apply(jump, s);
+    }
+    Kokkos::View<uint64_t*, Kokkos::MI355XSpace> d_starts(Kokkos::view_alloc(Kokkos::WithoutInitializing, std::string("xs_starts")), nchunks);
+    KKMX_SAFE_CALL(mx_memcpy(d_starts.data(), starts.data(), starts.size() * sizeof(uint64_t)));
+    Kokkos::parallel_for("kkmx::xorshift64", Range64(0, nchunks),
+                         XorShiftChunks{DevView<uint64_t>(out, n), DevView<const uint64_t>(d_starts.data(), nchunks), n, chunk});
+    Kokkos::fence();
+  });
+}
+int kkmx_gups(int64_t* table, int64_t table_size, const uint64_t* stream, int64_t nupdates) {
+  if (table_size < 1 || (table_size & (table_size - 1)) || !table || nupdates < 0 || (nupdates && !stream))
+    return reject("kkmx_gups: table_size must be a power of two");
+  return guarded([&] {
+    Kokkos::parallel_for("kkmx::gups", Range64(0, nupdates),
+                         Gups{DevView<int64_t>(table, table_size), DevView<const uint64_t>(stream, 2 * nupdates),
+                              uint64_t(table_size - 1)});
+  });
+}
+int kkmx_zipf_keys(int32_t* keys, int64_t n, double s, int32_t nbins, uint64_t seed) {
+  if (n < 0 || nbins < 1 || (n && !keys)) return reject("kkmx_zipf_keys: bad arguments");
+  return guarded([&] {
+    // cdf[k] = sum_{j<=k} (j+1)^-s / H, accumulated left to right in double
+    std::vector<double> cdf(static_cast<size_t>(nbins));
+    double acc = 0.0;
+    for (int32_t k = 0; k < nbins; ++k) {
+      acc += std::pow(double(k + 1), -s);
+      cdf[k] = acc;
+    }
+    for (int32_t k = 0; k < nbins; ++k) cdf[k] /= acc;
+    cdf[nbins - 1] = 1.0;
+    Kokkos::View<double*, Kokkos::MI355XSpace> d_cdf(Kokkos::view_alloc(Kokkos::WithoutInitializing, std::string("zipf_cdf")), nbins);
+    KKMX_SAFE_CALL(mx_memcpy(d_cdf.data(), cdf.data(), cdf.size() * sizeof(double)));
+    Kokkos::parallel_for("kkmx::zipf_keys", Range64(0, n),
+                         ZipfKeys{DevView<int32_t>(keys, n), DevView<const double>(d_cdf.data(), nbins), seed, nbins});
+    Kokkos::fence();
+  });
+}
+int kkmx_histogram(const int32_t* keys, int64_t n, int32_t* bins, int32_t nbins, int mode, int lds_bins) {
+  if (n < 0 || nbins < 1 || !bins || (n && !keys)) return reject("kkmx_histogram: bad arguments");
+  return guarded([&] {
+    DevView<const int32_t> K(keys, n);
+    DevView<int32_t> B(bins, nbins);
+    if (mode == 0) {
+      Kokkos::parallel_for("kkmx::hist_global", Range64(0, n), HistGlobal{K, B});
+      return;
+    }
+    int lb = lds_bins > 0 ? lds_bins : 32768;
+    if (lb > nbins) lb = nbins;
+    const int cus = Kokkos::MI355X::impl_cu_count();
+    auto pol = Kokkos::TeamPolicy<Kokkos::MI355X>(cus, 1024).set_scratch_size(0, Kokkos::PerTeam(size_t(lb) * 4));
+    Kokkos::parallel_for("kkmx::hist_lds", pol, HistPrivatised{K, B, n, lb});
+  });
+}
+int kkmx_scatter_add_f64(const double* vals, const int32_t* idx, int64_t n, double* bins, int32_t nbins) {
+  if (n < 0 || nbins < 1 || !bins || (n && (!vals || !idx))) return reject("kkmx_scatter_add_f64: bad arguments");
+  return guarded([&] {
+    Kokkos::parallel_for("kkmx::scatter_f64", Range64(0, n),
+                         ScatterAdd{DevView<const double>(vals, n), DevView<const int32_t>(idx, n), DevView<double>(bins, nbins)});
   });
 }
 

@@ -1,4 +1,894 @@
-#ifndef KKMX_KERNELTEAM_HPP
-#define KKMX_KERNELTEAM_HPP
-#include <kkmx/Parallel.hpp>
+// TeamPolicy on MI355X: one team per workgroup, vector lanes contiguous
+// within a wavefront, level-0 scratch in LDS, level-1 scratch in HBM.
+//
+// Reference behaviour:
+//  * member API, team_barrier / team_broadcast / team_reduce / team_scan:
+//    core/src/HIP/Kokkos_HIP_Team.hpp:67-345;
+//  * nested TeamThreadRange / ThreadVectorRange / TeamVectorRange for, reduce
+//    and scan, single(PerTeam/PerThread): core/src/HIP/Kokkos_HIP_Team.hpp:357-1006
+//    and core/src/Kokkos_ExecPolicy.hpp:779-980;
+//  * scratch bump allocation with per-thread offsets:
+//    core/src/Kokkos_ScratchSpace.hpp:83-140;
+//  * a team reduction counts one value per team thread (vector lane 0).
+//
+// Layout: flat lane id = team_rank * vector_length + vector_lane.  The
+// workgroup is the team rounded up to a multiple of 64 lanes; lanes beyond
+// the team never call the functor but take part in barriers and in the
+// workgroup-wide combine (with the identity), so every DPP and barrier sees
+// whole wavefronts.  Collectives inside the functor use ds_bpermute shuffles
+// within a wavefront and a 17-slot LDS area across wavefronts, always
+// combining in thread order.
+#ifndef KKMX_KERNEL_TEAM_HPP
+#define KKMX_KERNEL_TEAM_HPP
+
+#include <kkmx/KernelRange.hpp>
+
+namespace Kokkos {
+
+struct ParallelForTag {};
+struct ParallelReduceTag {};
+struct ParallelScanTag {};
+
+// ------------------------------------------------------------------ scratch
+template <class ExecSpace>
+class ScratchMemorySpace {
+ public:
+  static constexpr bool kkmx_is_scratch = true;
+  using execution_space = ExecSpace;
+  using memory_space    = ScratchMemorySpace<ExecSpace>;
+  using device_type     = Device<execution_space, memory_space>;
+  using array_layout    = typename ExecSpace::array_layout;
+  using size_type       = typename ExecSpace::size_type;
+
+  KOKKOS_INLINE_FUNCTION ScratchMemorySpace() = default;
+  KOKKOS_INLINE_FUNCTION ScratchMemorySpace(void* l0, size_t l0_size, void* l1 = nullptr, size_t l1_size = 0)
+      : m_iter_L0(static_cast<char*>(l0)),
+        m_end_L0(static_cast<char*>(l0) + l0_size),
+        m_iter_L1(static_cast<char*>(l1)),
+        m_end_L1(static_cast<char*>(l1) + l1_size) {}
+
+  template <class IntType>
+  KOKKOS_INLINE_FUNCTION void* get_shmem(const IntType& size, int level = -1) const {
+    return common(static_cast<size_t>(size), 1, level);
+  }
+  template <class IntType>
+  KOKKOS_INLINE_FUNCTION void* get_shmem_aligned(const IntType& size, const ptrdiff_t align,
+                                                 int level = -1) const {
+    return common(static_cast<size_t>(size), static_cast<size_t>(align), level);
+  }
+  KOKKOS_INLINE_FUNCTION const ScratchMemorySpace& set_team_thread_mode(int level, int multiplier,
+                                                                        int offset) const {
+    m_default_level = level;
+    m_multiplier    = multiplier;
+    m_offset        = offset;
+    return *this;
+  }
+
+ private:
+  KOKKOS_INLINE_FUNCTION void* common(size_t size, size_t align, int level) const {
+    if (level == -1) level = m_default_level;
+    char*& it      = (level == 0) ? m_iter_L0 : m_iter_L1;
+    char* const e  = (level == 0) ? m_end_L0 : m_end_L1;
+    char* old      = it;
+    if (align > 1) {
+      const size_t mis = reinterpret_cast<uintptr_t>(it) % align;
+      if (mis) it += align - mis;
+    }
+    void* p        = it + static_cast<size_t>(m_offset) * size;
+    const size_t inc = size * static_cast<size_t>(m_multiplier);
+    if (inc > static_cast<size_t>(e - it)) {
+      it = old;
+      return nullptr;
+    }
+    it += inc;
+    return p;
+  }
+  mutable char* m_iter_L0   = nullptr;
+  char* m_end_L0            = nullptr;
+  mutable char* m_iter_L1   = nullptr;
+  char* m_end_L1            = nullptr;
+  mutable int m_multiplier  = 1;
+  mutable int m_offset      = 0;
+  mutable int m_default_level = 0;
+};
+
+namespace Impl {
+
+// Bytes of LDS the runtime keeps per team for collectives (17 slots of up to
+// 64-byte values: 16 wavefront partials + 1 broadcast slot).
+constexpr int kTeamCollectiveBytes = 17 * 64;
+
+// ------------------------------------------------------------------ MI355X member
+template <>
+class TeamMember<MI355X> {
+ public:
+  using execution_space      = MI355X;
+  using scratch_memory_space = ScratchMemorySpace<MI355X>;
+  using team_handle          = TeamMember<MI355X>;
+
+  KOKKOS_INLINE_FUNCTION TeamMember(char* collective, void* l0, size_t l0_size, void* l1, size_t l1_size,
+                                    int league_rank, int league_size, int team_size, int vector_length)
+      : m_coll(collective),
+        m_scratch(l0, l0_size, l1, l1_size),
+        m_league_rank(league_rank),
+        m_league_size(league_size),
+        m_team_size(team_size),
+        m_vl(vector_length) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    m_team_rank   = static_cast<int>(threadIdx.x) / vector_length;
+    m_vector_lane = static_cast<int>(threadIdx.x) % vector_length;
+#endif
+  }
+
+  KOKKOS_INLINE_FUNCTION int league_rank() const { return m_league_rank; }
+  KOKKOS_INLINE_FUNCTION int league_size() const { return m_league_size; }
+  KOKKOS_INLINE_FUNCTION int team_rank() const { return m_team_rank; }
+  KOKKOS_INLINE_FUNCTION int team_size() const { return m_team_size; }
+  KOKKOS_INLINE_FUNCTION int impl_vector_lane() const { return m_vector_lane; }
+  KOKKOS_INLINE_FUNCTION int impl_vector_length() const { return m_vl; }
+
+  KOKKOS_INLINE_FUNCTION const scratch_memory_space& team_shmem() const {
+    return m_scratch.set_team_thread_mode(0, 1, 0);
+  }
+  KOKKOS_INLINE_FUNCTION const scratch_memory_space& team_scratch(int level) const {
+    return m_scratch.set_team_thread_mode(level, 1, 0);
+  }
+  KOKKOS_INLINE_FUNCTION const scratch_memory_space& thread_scratch(int level) const {
+    return m_scratch.set_team_thread_mode(level, m_team_size, m_team_rank);
+  }
+
+  KOKKOS_INLINE_FUNCTION void team_barrier() const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __syncthreads();
+#endif
+  }
+
+  template <class T>
+  KOKKOS_INLINE_FUNCTION void team_broadcast(T& val, const int root) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    static_assert(sizeof(T) <= 64, "team_broadcast: value larger than 64 bytes");
+    T* slot = reinterpret_cast<T*>(m_coll + 16 * 64);
+    __syncthreads();
+    if (m_team_rank == root && m_vector_lane == 0) *slot = val;
+    __syncthreads();
+    val = *slot;
+#else
+    (void)val;
+    (void)root;
+#endif
+  }
+  template <class Closure, class T>
+  KOKKOS_INLINE_FUNCTION void team_broadcast(const Closure& f, T& val, const int root) const {
+    f(val);
+    team_broadcast(val, root);
+  }
+
+  // ---- ordered collectives (device only)
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Inclusive scan across lanes [seg_start, seg_start + seg_len) of this
+  // wavefront, lower lanes on the left; returns the inclusive value.
+  template <class T, class Join>
+  __device__ static T seg_inclusive(T v, int pos_in_seg, int seg_len, const Join& join) {
+    const int l = wave::lane();
+    for (int d = 1; d < seg_len; d <<= 1) {
+      T t = wave::shuffle(v, l - d >= 0 ? l - d : l);
+      if (pos_in_seg >= d) {
+        join(t, v);
+        v = t;
+      }
+    }
+    return v;
+  }
+  // Reduce over the team threads (vector lane 0 values) in thread order;
+  // every lane of the team receives the result.
+  template <class T, class Join>
+  __device__ void reduce_threads(T& v, const T& identity, const Join& join) const {
+    static_assert(sizeof(T) <= 64, "team reduction: value larger than 64 bytes");
+    const int flat   = threadIdx.x;
+    const int nthr   = m_team_size * m_vl;
+    const int w      = flat >> 6;
+    const int l      = wave::lane();
+    const int wstart = w * 64;
+    int nact         = nthr - wstart;
+    if (nact > 64) nact = 64;
+    T x  = (m_vector_lane == 0) ? v : identity;
+    T in = seg_inclusive(x, l, nact, join);
+    T* slots = reinterpret_cast<T*>(m_coll);
+    __syncthreads();
+    if (l == nact - 1) slots[w] = in;
+    __syncthreads();
+    const int nw = (nthr + 63) >> 6;
+    T r          = identity;
+    for (int k = 0; k < nw; ++k) join(r, slots[k]);
+    __syncthreads();
+    v = r;
+  }
+  // Exclusive scan over team threads in thread order; returns this thread's
+  // exclusive prefix and writes the team total.
+  template <class T, class Join>
+  __device__ T scan_threads(const T& v, const T& identity, const Join& join, T* total) const {
+    const int flat   = threadIdx.x;
+    const int nthr   = m_team_size * m_vl;
+    const int w      = flat >> 6;
+    const int l      = wave::lane();
+    const int wstart = w * 64;
+    int nact         = nthr - wstart;
+    if (nact > 64) nact = 64;
+    T x  = (m_vector_lane == 0) ? v : identity;
+    T in = seg_inclusive(x, l, nact, join);
+    // exclusive within the wavefront: inclusive value of the previous lane of
+    // this thread's leader
+    const int leader = (m_team_rank * m_vl) - wstart;
+    T ex             = wave::shuffle(in, leader > 0 ? leader - 1 : 0);
+    if (leader == 0) ex = identity;
+    T* slots = reinterpret_cast<T*>(m_coll);
+    __syncthreads();
+    if (l == nact - 1) slots[w] = in;
+    __syncthreads();
+    const int nw = (nthr + 63) >> 6;
+    T off        = identity;
+    T tot        = identity;
+    for (int k = 0; k < nw; ++k) {
+      if (k < w) join(off, slots[k]);
+      join(tot, slots[k]);
+    }
+    __syncthreads();
+    join(off, ex);
+    if (total) *total = tot;
+    return off;
+  }
+  // Reduce over the vector lanes of this thread, result in every lane.
+  template <class T, class Join>
+  __device__ void reduce_vector(T& v, const Join& join) const {
+    const int seg0 = wave::lane() - m_vector_lane;
+    T in           = seg_inclusive(v, m_vector_lane, m_vl, join);
+    v              = wave::shuffle(in, seg0 + m_vl - 1);
+  }
+  template <class T, class Join>
+  __device__ T scan_vector(const T& v, const T& identity, const Join& join, T* total) const {
+    const int seg0 = wave::lane() - m_vector_lane;
+    T in           = seg_inclusive(v, m_vector_lane, m_vl, join);
+    T ex           = wave::shuffle(in, m_vector_lane > 0 ? wave::lane() - 1 : wave::lane());
+    if (m_vector_lane == 0) ex = identity;
+    if (total) *total = wave::shuffle(in, seg0 + m_vl - 1);
+    return ex;
+  }
+  // Reduce over every lane of the team (TeamVectorRange), lane order.
+  template <class T, class Join>
+  __device__ void reduce_lanes(T& v, const T& identity, const Join& join) const {
+    const int flat   = threadIdx.x;
+    const int nthr   = m_team_size * m_vl;
+    const int w      = flat >> 6;
+    const int l      = wave::lane();
+    int nact         = nthr - w * 64;
+    if (nact > 64) nact = 64;
+    T in     = seg_inclusive(v, l, nact, join);
+    T* slots = reinterpret_cast<T*>(m_coll);
+    __syncthreads();
+    if (l == nact - 1) slots[w] = in;
+    __syncthreads();
+    const int nw = (nthr + 63) >> 6;
+    T r          = identity;
+    for (int k = 0; k < nw; ++k) join(r, slots[k]);
+    __syncthreads();
+    v = r;
+  }
+#endif
+
+  template <class ReducerType>
+  KOKKOS_INLINE_FUNCTION std::enable_if_t<is_reducer<ReducerType>::value> team_reduce(
+      const ReducerType& reducer) const {
+    team_reduce(reducer, reducer.reference());
+  }
+  template <class ReducerType>
+  KOKKOS_INLINE_FUNCTION std::enable_if_t<is_reducer<ReducerType>::value> team_reduce(
+      const ReducerType& reducer, typename ReducerType::value_type& value) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    using T = typename ReducerType::value_type;
+    T id;
+    reducer.init(id);
+    reduce_threads(value, id, [&reducer](T& a, const T& b) { reducer.join(a, b); });
+    reducer.reference() = value;
+#else
+    (void)reducer;
+    (void)value;
+#endif
+  }
+  template <class T>
+  KOKKOS_INLINE_FUNCTION T team_scan(const T& value, T* const global_accum = nullptr) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    T total;
+    T ex = scan_threads(value, T(0), [](T& a, const T& b) { a += b; }, &total);
+    if (global_accum) {
+      T* slot = reinterpret_cast<T*>(m_coll + 16 * 64);
+      if (m_team_rank == 0 && m_vector_lane == 0) *slot = atomic_fetch_add(global_accum, total);
+      __syncthreads();
+      ex += *slot;
+      __syncthreads();
+    }
+    return ex;
+#else
+    (void)global_accum;
+    return value;
+#endif
+  }
+
+ private:
+  char* m_coll;
+  scratch_memory_space m_scratch;
+  int m_league_rank, m_league_size, m_team_size, m_vl;
+  int m_team_rank   = 0;
+  int m_vector_lane = 0;
+};
+
+// ------------------------------------------------------------------ Serial member
+template <>
+class TeamMember<Serial> {
+ public:
+  using execution_space      = Serial;
+  using scratch_memory_space = ScratchMemorySpace<Serial>;
+  using team_handle          = TeamMember<Serial>;
+  TeamMember(void* l0, size_t l0_size, void* l1, size_t l1_size, int league_rank, int league_size)
+      : m_scratch(l0, l0_size, l1, l1_size), m_league_rank(league_rank), m_league_size(league_size) {}
+  KOKKOS_INLINE_FUNCTION int league_rank() const { return m_league_rank; }
+  KOKKOS_INLINE_FUNCTION int league_size() const { return m_league_size; }
+  KOKKOS_INLINE_FUNCTION int team_rank() const { return 0; }
+  KOKKOS_INLINE_FUNCTION int team_size() const { return 1; }
+  KOKKOS_INLINE_FUNCTION int impl_vector_lane() const { return 0; }
+  KOKKOS_INLINE_FUNCTION int impl_vector_length() const { return 1; }
+  const scratch_memory_space& team_shmem() const { return m_scratch.set_team_thread_mode(0, 1, 0); }
+  const scratch_memory_space& team_scratch(int l) const { return m_scratch.set_team_thread_mode(l, 1, 0); }
+  const scratch_memory_space& thread_scratch(int l) const { return m_scratch.set_team_thread_mode(l, 1, 0); }
+  void team_barrier() const {}
+  template <class T>
+  void team_broadcast(T&, const int) const {}
+  template <class Closure, class T>
+  void team_broadcast(const Closure& f, T& val, const int) const {
+    f(val);
+  }
+  template <class ReducerType>
+  std::enable_if_t<is_reducer<ReducerType>::value> team_reduce(const ReducerType&) const {}
+  template <class ReducerType>
+  std::enable_if_t<is_reducer<ReducerType>::value> team_reduce(const ReducerType& r,
+                                                               typename ReducerType::value_type& v) const {
+    r.reference() = v;
+  }
+  template <class T>
+  T team_scan(const T& value, T* const global_accum = nullptr) const {
+    if (global_accum) {
+      T old = *global_accum;
+      *global_accum += value;
+      return old;
+    }
+    return T(0);
+  }
+
+ private:
+  scratch_memory_space m_scratch;
+  int m_league_rank, m_league_size;
+};
+
+// ------------------------------------------------------------------ nested range structs
+template <class iType, class Member>
+struct TeamThreadRangeBoundariesStruct {
+  using index_type = iType;
+  const Member& member;
+  iType start, end;
+  KOKKOS_INLINE_FUNCTION TeamThreadRangeBoundariesStruct(const Member& m, iType b, iType e)
+      : member(m), start(b), end(e) {}
+};
+template <class iType, class Member>
+struct ThreadVectorRangeBoundariesStruct {
+  using index_type = iType;
+  const Member& member;
+  iType start, end;
+  KOKKOS_INLINE_FUNCTION ThreadVectorRangeBoundariesStruct(const Member& m, iType b, iType e)
+      : member(m), start(b), end(e) {}
+};
+template <class iType, class Member>
+struct TeamVectorRangeBoundariesStruct {
+  using index_type = iType;
+  const Member& member;
+  iType start, end;
+  KOKKOS_INLINE_FUNCTION TeamVectorRangeBoundariesStruct(const Member& m, iType b, iType e)
+      : member(m), start(b), end(e) {}
+};
+template <class iType, class Member>
+struct ThreadVectorMDRangeDummy {};
+
+template <class T>
+struct value_of_result {
+  using type = std::remove_cv_t<std::remove_reference_t<T>>;
+};
+
+// Join/identity helpers for nested reductions: reducer or scalar sum.
+template <class R, class = void>
+struct NestedOps {
+  using value_type = R;
+  KOKKOS_INLINE_FUNCTION static void init(const R&, value_type& v) { v = value_type(); }
+  KOKKOS_INLINE_FUNCTION static void join(const R&, value_type& a, const value_type& b) { a += b; }
+};
+template <class R>
+struct NestedOps<R, std::enable_if_t<is_reducer<R>::value>> {
+  using value_type = typename R::value_type;
+  KOKKOS_INLINE_FUNCTION static void init(const R& r, value_type& v) { r.init(v); }
+  KOKKOS_INLINE_FUNCTION static void join(const R& r, value_type& a, const value_type& b) { r.join(a, b); }
+};
+template <class R>
+KOKKOS_INLINE_FUNCTION void nested_store(const R& r, const typename NestedOps<R>::value_type& v) {
+  if constexpr (is_reducer<R>::value) r.reference() = v;
+}
+
+}  // namespace Impl
+
+template <class Member, class iType>
+KOKKOS_INLINE_FUNCTION Impl::TeamThreadRangeBoundariesStruct<iType, Member> TeamThreadRange(const Member& m,
+                                                                                            const iType& count) {
+  return {m, iType(0), count};
+}
+template <class Member, class iType1, class iType2>
+KOKKOS_INLINE_FUNCTION Impl::TeamThreadRangeBoundariesStruct<std::common_type_t<iType1, iType2>, Member>
+TeamThreadRange(const Member& m, const iType1& b, const iType2& e) {
+  using I = std::common_type_t<iType1, iType2>;
+  return {m, I(b), I(e)};
+}
+template <class Member, class iType>
+KOKKOS_INLINE_FUNCTION Impl::ThreadVectorRangeBoundariesStruct<iType, Member> ThreadVectorRange(const Member& m,
+                                                                                                const iType& count) {
+  return {m, iType(0), count};
+}
+template <class Member, class iType1, class iType2>
+KOKKOS_INLINE_FUNCTION Impl::ThreadVectorRangeBoundariesStruct<std::common_type_t<iType1, iType2>, Member>
+ThreadVectorRange(const Member& m, const iType1& b, const iType2& e) {
+  using I = std::common_type_t<iType1, iType2>;
+  return {m, I(b), I(e)};
+}
+template <class Member, class iType>
+KOKKOS_INLINE_FUNCTION Impl::TeamVectorRangeBoundariesStruct<iType, Member> TeamVectorRange(const Member& m,
+                                                                                            const iType& count) {
+  return {m, iType(0), count};
+}
+template <class Member, class iType1, class iType2>
+KOKKOS_INLINE_FUNCTION Impl::TeamVectorRangeBoundariesStruct<std::common_type_t<iType1, iType2>, Member>
+TeamVectorRange(const Member& m, const iType1& b, const iType2& e) {
+  using I = std::common_type_t<iType1, iType2>;
+  return {m, I(b), I(e)};
+}
+
+template <class Member, class = decltype(std::declval<const Member&>().team_rank())>
+KOKKOS_INLINE_FUNCTION Impl::SinglePerTeam<Member> PerTeam(const Member& m) {
+  return {m};
+}
+template <class Member, class = decltype(std::declval<const Member&>().team_rank())>
+KOKKOS_INLINE_FUNCTION Impl::SinglePerThread<Member> PerThread(const Member& m) {
+  return {m};
+}
+
+// ------------------------------------------------------------------ nested parallel_for
+template <class iType, class Member, class F>
+KOKKOS_INLINE_FUNCTION void parallel_for(const Impl::TeamThreadRangeBoundariesStruct<iType, Member>& r, const F& f) {
+  const int ts = r.member.team_size();
+  for (iType i = r.start + r.member.team_rank(); i < r.end; i += ts) f(i);
+}
+template <class iType, class Member, class F>
+KOKKOS_INLINE_FUNCTION void parallel_for(const Impl::ThreadVectorRangeBoundariesStruct<iType, Member>& r,
+                                         const F& f) {
+  const int vl = r.member.impl_vector_length();
+  for (iType i = r.start + r.member.impl_vector_lane(); i < r.end; i += vl) f(i);
+}
+template <class iType, class Member, class F>
+KOKKOS_INLINE_FUNCTION void parallel_for(const Impl::TeamVectorRangeBoundariesStruct<iType, Member>& r, const F& f) {
+  const int n  = r.member.team_size() * r.member.impl_vector_length();
+  const int id = r.member.team_rank() * r.member.impl_vector_length() + r.member.impl_vector_lane();
+  for (iType i = r.start + id; i < r.end; i += n) f(i);
+}
+
+// ------------------------------------------------------------------ nested parallel_reduce
+// Result: a scalar (sum) or a reducer; every thread of the team (or of the
+// vector group) receives the result.
+template <class iType, class Member, class F, class R>
+KOKKOS_INLINE_FUNCTION void parallel_reduce(const Impl::TeamThreadRangeBoundariesStruct<iType, Member>& r,
+                                            const F& f, R&& result) {
+  using RR  = Impl::remove_cvref_t<R>;
+  using Ops = Impl::NestedOps<RR>;
+  using T   = typename Ops::value_type;
+  const RR& red = result;
+  T v;
+  Ops::init(red, v);
+  const int ts = r.member.team_size();
+  for (iType i = r.start + r.member.team_rank(); i < r.end; i += ts) f(i, v);
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
+    T id;
+    Ops::init(red, id);
+    r.member.reduce_threads(v, id, [&red](T& a, const T& b) { Ops::join(red, a, b); });
+  }
+#endif
+  if constexpr (is_reducer<RR>::value) Impl::nested_store(red, v);
+  else result = v;
+}
+template <class iType, class Member, class F, class R>
+KOKKOS_INLINE_FUNCTION void parallel_reduce(const Impl::ThreadVectorRangeBoundariesStruct<iType, Member>& r,
+                                            const F& f, R&& result) {
+  using RR  = Impl::remove_cvref_t<R>;
+  using Ops = Impl::NestedOps<RR>;
+  using T   = typename Ops::value_type;
+  const RR& red = result;
+  T v;
+  Ops::init(red, v);
+  const int vl = r.member.impl_vector_length();
+  for (iType i = r.start + r.member.impl_vector_lane(); i < r.end; i += vl) f(i, v);
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
+    if (vl > 1) r.member.reduce_vector(v, [&red](T& a, const T& b) { Ops::join(red, a, b); });
+  }
+#endif
+  if constexpr (is_reducer<RR>::value) Impl::nested_store(red, v);
+  else result = v;
+}
+template <class iType, class Member, class F, class R>
+KOKKOS_INLINE_FUNCTION void parallel_reduce(const Impl::TeamVectorRangeBoundariesStruct<iType, Member>& r,
+                                            const F& f, R&& result) {
+  using RR  = Impl::remove_cvref_t<R>;
+  using Ops = Impl::NestedOps<RR>;
+  using T   = typename Ops::value_type;
+  const RR& red = result;
+  T v;
+  Ops::init(red, v);
+  const int n  = r.member.team_size() * r.member.impl_vector_length();
+  const int id = r.member.team_rank() * r.member.impl_vector_length() + r.member.impl_vector_lane();
+  for (iType i = r.start + id; i < r.end; i += n) f(i, v);
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
+    T idv;
+    Ops::init(red, idv);
+    r.member.reduce_lanes(v, idv, [&red](T& a, const T& b) { Ops::join(red, a, b); });
+  }
+#endif
+  if constexpr (is_reducer<RR>::value) Impl::nested_store(red, v);
+  else result = v;
+}
+
+// ------------------------------------------------------------------ nested parallel_scan
+namespace Impl {
+template <class F, class iType, class T>
+KOKKOS_INLINE_FUNCTION void nested_scan_value(const F& f, iType i, T& v) {
+  f(i, v, false);
+}
+}  // namespace Impl
+
+template <class iType, class Member, class F, class T>
+KOKKOS_INLINE_FUNCTION void parallel_scan(const Impl::TeamThreadRangeBoundariesStruct<iType, Member>& r,
+                                          const F& f, T& return_value) {
+  const int ts = r.member.team_size();
+  T running    = T();
+  for (iType base = r.start; base < r.end; base += ts) {
+    const iType i = base + r.member.team_rank();
+    T v           = T();
+    if (i < r.end) f(i, v, false);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
+      T total;
+      T ex  = r.member.scan_threads(v, T(), [](T& a, const T& b) { a += b; }, &total);
+      T upd = running + ex;
+      if (i < r.end) f(i, upd, true);
+      running += total;
+    } else
+#endif
+    {
+      T upd = running;
+      if (i < r.end) f(i, upd, true);
+      running = upd;
+    }
+  }
+  return_value = running;
+}
+template <class iType, class Member, class F>
+KOKKOS_INLINE_FUNCTION void parallel_scan(const Impl::TeamThreadRangeBoundariesStruct<iType, Member>& r,
+                                          const F& f) {
+  using T = typename Impl::arg_from_back<typename Impl::op_args<F>::type, 1>::type;
+  T dummy;
+  parallel_scan(r, f, dummy);
+}
+template <class iType, class Member, class F, class T>
+KOKKOS_INLINE_FUNCTION void parallel_scan(const Impl::ThreadVectorRangeBoundariesStruct<iType, Member>& r,
+                                          const F& f, T& return_value) {
+  const int vl = r.member.impl_vector_length();
+  T running    = T();
+  for (iType base = r.start; base < r.end; base += vl) {
+    const iType i = base + r.member.impl_vector_lane();
+    T v           = T();
+    if (i < r.end) f(i, v, false);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
+      T total;
+      T ex  = r.member.scan_vector(v, T(), [](T& a, const T& b) { a += b; }, &total);
+      T upd = running + ex;
+      if (i < r.end) f(i, upd, true);
+      running += total;
+    } else
+#endif
+    {
+      T upd = running;
+      if (i < r.end) f(i, upd, true);
+      running = upd;
+    }
+  }
+  return_value = running;
+}
+template <class iType, class Member, class F>
+KOKKOS_INLINE_FUNCTION void parallel_scan(const Impl::ThreadVectorRangeBoundariesStruct<iType, Member>& r,
+                                          const F& f) {
+  using T = typename Impl::arg_from_back<typename Impl::op_args<F>::type, 1>::type;
+  T dummy;
+  parallel_scan(r, f, dummy);
+}
+
+// ------------------------------------------------------------------ single
+template <class Member, class F>
+KOKKOS_INLINE_FUNCTION void single(const Impl::SinglePerTeam<Member>& s, const F& f) {
+  if (s.team.team_rank() == 0 && s.team.impl_vector_lane() == 0) f();
+}
+template <class Member, class F>
+KOKKOS_INLINE_FUNCTION void single(const Impl::SinglePerThread<Member>& s, const F& f) {
+  if (s.team.impl_vector_lane() == 0) f();
+}
+template <class Member, class F, class T>
+KOKKOS_INLINE_FUNCTION void single(const Impl::SinglePerTeam<Member>& s, const F& f, T& val) {
+  if (s.team.team_rank() == 0 && s.team.impl_vector_lane() == 0) f(val);
+  s.team.team_broadcast(val, 0);
+}
+template <class Member, class F, class T>
+KOKKOS_INLINE_FUNCTION void single(const Impl::SinglePerThread<Member>& s, const F& f, T& val) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
+    if (s.team.impl_vector_lane() == 0) f(val);
+    val = Impl::wave::shuffle(val, Impl::wave::lane() - s.team.impl_vector_lane());
+    return;
+  }
+#endif
+  f(val);
+}
+
+// ================================================================== closures
+namespace Impl {
+
+template <class F, class = void>
+struct functor_team_shmem {
+  static size_t get(const F&, int) { return 0; }
+};
+template <class F>
+struct functor_team_shmem<F, std::void_t<decltype(std::declval<const F&>().team_shmem_size(0))>> {
+  static size_t get(const F& f, int ts) { return static_cast<size_t>(f.team_shmem_size(ts)); }
+};
+template <class F, class = void>
+struct functor_shmem_size {
+  static size_t get(const F&, int) { return 0; }
+};
+template <class F>
+struct functor_shmem_size<F, std::void_t<decltype(std::declval<const F&>().shmem_size(0))>> {
+  static size_t get(const F& f, int ts) { return static_cast<size_t>(f.shmem_size(ts)); }
+};
+
+struct TeamLaunch {
+  int league_size;
+  int team_size;
+  int vector_length;
+  int l0_offset;      // start of level-0 scratch within dynamic LDS
+  int l0_bytes;       // level-0 scratch bytes per team
+  size_t l1_bytes;    // level-1 scratch bytes per team
+  char* l1_pool;      // gridDim.x slices of l1_bytes
+};
+
+template <class F, class Tag>
+__global__ __launch_bounds__(1024) void k_team_for(const F f, const TeamLaunch p) {
+  extern __shared__ __align__(16) char kkmx_team_lds[];
+  const bool active = static_cast<int>(threadIdx.x) < p.team_size * p.vector_length;
+  char* l1          = p.l1_pool ? p.l1_pool + static_cast<size_t>(blockIdx.x) * p.l1_bytes : nullptr;
+  for (int lr = blockIdx.x; lr < p.league_size; lr += gridDim.x) {
+    TeamMember<MI355X> m(kkmx_team_lds, kkmx_team_lds + p.l0_offset, p.l0_bytes, l1, p.l1_bytes, lr,
+                         p.league_size, p.team_size, p.vector_length);
+    if (active) call_functor<Tag>(f, m);
+    __syncthreads();
+  }
+}
+
+template <class CR>
+__global__ __launch_bounds__(1024) void k_team_reduce(const CR cr, const TeamLaunch p,
+                                                      typename CR::value_type* partials, unsigned* counter,
+                                                      typename CR::value_type* result) {
+  using V   = typename CR::value_type;
+  using Tag = typename CR::work_tag;
+  extern __shared__ __align__(16) char kkmx_team_lds[];
+  __shared__ LdsSlots16<V> slots;
+  __shared__ int s_last;
+  const auto join   = [&cr](V& a, const V& b) { cr.join(a, b); };
+  const bool active = static_cast<int>(threadIdx.x) < p.team_size * p.vector_length;
+  char* l1          = p.l1_pool ? p.l1_pool + static_cast<size_t>(blockIdx.x) * p.l1_bytes : nullptr;
+  V identity;
+  cr.init(identity);
+  V v = identity;
+  for (int lr = blockIdx.x; lr < p.league_size; lr += gridDim.x) {
+    TeamMember<MI355X> m(kkmx_team_lds, kkmx_team_lds + p.l0_offset, p.l0_bytes, l1, p.l1_bytes, lr,
+                         p.league_size, p.team_size, p.vector_length);
+    if (active) call_functor<Tag>(cr.functor(), m, v);
+    __syncthreads();
+  }
+  // one value per team thread: vector lane 0 only
+  const bool counts = active && (static_cast<int>(threadIdx.x) % p.vector_length) == 0;
+  V x               = counts ? v : identity;
+  x                 = block_reduce(x, identity, join, slots.get(), blockDim.x);
+  if (gridDim.x == 1) {
+    if (threadIdx.x == 0) {
+      cr.final(x);
+      *result = x;
+    }
+    return;
+  }
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = x;
+    const unsigned prev  = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    s_last               = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  V w = identity;
+  for (int j = threadIdx.x; j < static_cast<int>(gridDim.x); j += blockDim.x) join(w, partials[j]);
+  w = block_reduce(w, identity, join, slots.get(), blockDim.x);
+  if (threadIdx.x == 0) {
+    cr.final(w);
+    *result = w;
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Launch shape of a team dispatch.
+template <class Policy, class F>
+TeamLaunch team_launch_shape(const Policy& policy, const F& f, MI355XInstance* inst, int& block, int& grid,
+                             size_t& lds) {
+  TeamLaunch p{};
+  p.vector_length = policy.impl_vector_length() > 0 ? policy.impl_vector_length() : 1;
+  int ts          = policy.impl_team_size_raw();
+  if (ts <= 0) ts = policy.impl_default_team_size();
+  if (ts * p.vector_length > 1024)
+    throw_runtime_exception("Kokkos::MI355X TeamPolicy: team_size * vector_length exceeds 1024");
+  p.team_size   = ts;
+  p.league_size = policy.league_size();
+  block         = ((ts * p.vector_length + 63) / 64) * 64;
+  const size_t l0 = policy.team_scratch_size(0) + static_cast<size_t>(ts) * policy.thread_scratch_size(0) +
+                    functor_team_shmem<F>::get(f, ts) + functor_shmem_size<F>::get(f, ts);
+  p.l0_offset = kTeamCollectiveBytes;
+  p.l0_bytes  = static_cast<int>(l0);
+  lds         = kTeamCollectiveBytes + ((l0 + 15) / 16) * 16;
+  if (lds > 160 * 1024)
+    throw_runtime_exception("Kokkos::MI355X TeamPolicy: requested level-0 scratch exceeds the 160 KiB of LDS");
+  const int cus = MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256;
+  int per_cu    = (2048 / block) > 0 ? 2048 / block : 1;
+  if (lds > 0) {
+    int by_lds = static_cast<int>((160 * 1024) / lds);
+    if (by_lds < per_cu) per_cu = by_lds > 0 ? by_lds : 1;
+  }
+  int64_t g = std::min<int64_t>(p.league_size, static_cast<int64_t>(cus) * per_cu * 8);
+  grid      = static_cast<int>(g > 0 ? g : 0);
+  p.l1_bytes = policy.team_scratch_size(1) + static_cast<size_t>(ts) * policy.thread_scratch_size(1);
+  p.l1_pool  = nullptr;
+  if (p.l1_bytes > 0 && grid > 0)
+    p.l1_pool = static_cast<char*>(inst->team_l1_space(p.l1_bytes * static_cast<size_t>(grid)));
+  return p;
+}
+
+template <class F, class... Traits>
+class ParallelFor<F, TeamPolicy<Traits...>, MI355X> {
+ public:
+  using Policy = TeamPolicy<Traits...>;
+  ParallelFor(const F& f, const Policy& p) : m_f(f), m_p(p) {}
+  void execute() const {
+    MI355XInstance* inst = m_p.space().impl_internal_space_instance();
+    std::lock_guard<std::mutex> lk(inst->mtx);
+    int block = 0, grid = 0;
+    size_t lds = 0;
+    TeamLaunch tl = team_launch_shape(m_p, m_f, inst, block, grid, lds);
+    if (grid == 0) return;
+    F f = m_f;
+    const void* k = reinterpret_cast<const void*>(&k_team_for<F, typename Policy::work_tag>);
+    mx_launch_kernel(k, static_cast<unsigned>(grid), block, lds, inst->stream, f, tl);
+  }
+
+ private:
+  F m_f;
+  Policy m_p;
+};
+
+template <class CR, class... Traits>
+class ParallelReduce<CR, TeamPolicy<Traits...>, MI355X> {
+ public:
+  using Policy = TeamPolicy<Traits...>;
+  using V      = typename CR::value_type;
+  template <class Mem>
+  ParallelReduce(const CR& cr, const Policy& p, V* result, Mem)
+      : m_cr(cr), m_p(p), m_result(result), m_on_device(exec_can_access<MI355X, Mem>::value) {}
+  void execute() const {
+    static_assert(!CR::is_array, "TeamPolicy reductions with runtime-length value types are not supported");
+    MI355XInstance* inst = m_p.space().impl_internal_space_instance();
+    std::lock_guard<std::mutex> lk(inst->mtx);
+    int block = 0, grid = 0;
+    size_t lds = 0;
+    TeamLaunch tl = team_launch_shape(m_p, m_cr.functor(), inst, block, grid, lds);
+    if (grid == 0) grid = 1;
+    V* dst         = m_result;
+    bool copy_back = false;
+    if (!m_on_device || !dst) {
+      dst       = static_cast<V*>(inst->pinned_space(sizeof(V)));
+      copy_back = (m_result != nullptr);
+    }
+    V* partials = static_cast<V*>(inst->partials_space(sizeof(V) * grid));
+    CR cr       = m_cr;
+    unsigned* counter = inst->control;
+    const void* k = reinterpret_cast<const void*>(&k_team_reduce<CR>);
+    mx_launch_kernel(k, static_cast<unsigned>(grid), block, lds, inst->stream, cr, tl, partials, counter, dst);
+    if (copy_back) {
+      KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
+      std::memcpy(static_cast<void*>(m_result), dst, sizeof(V));
+    }
+  }
+
+ private:
+  CR m_cr;
+  Policy m_p;
+  V* m_result;
+  bool m_on_device;
+};
+
+// ------------------------------------------------------------------ Serial team closures
+template <class F, class... Traits>
+class ParallelFor<F, TeamPolicy<Traits...>, Serial> {
+ public:
+  using Policy = TeamPolicy<Traits...>;
+  ParallelFor(const F& f, const Policy& p) : m_f(f), m_p(p) {}
+  void execute() const {
+    const size_t l0 = m_p.team_scratch_size(0) + m_p.thread_scratch_size(0) + functor_team_shmem<F>::get(m_f, 1) +
+                      functor_shmem_size<F>::get(m_f, 1);
+    const size_t l1 = m_p.team_scratch_size(1) + m_p.thread_scratch_size(1);
+    std::vector<char> b0(l0 + 16), b1(l1 + 16);
+    for (int lr = 0; lr < m_p.league_size(); ++lr) {
+      TeamMember<Serial> m(b0.data(), l0, b1.data(), l1, lr, m_p.league_size());
+      call_functor<typename Policy::work_tag>(m_f, m);
+    }
+  }
+
+ private:
+  F m_f;
+  Policy m_p;
+};
+template <class CR, class... Traits>
+class ParallelReduce<CR, TeamPolicy<Traits...>, Serial> {
+ public:
+  using Policy = TeamPolicy<Traits...>;
+  using V      = typename CR::value_type;
+  template <class Mem>
+  ParallelReduce(const CR& cr, const Policy& p, V* result, Mem) : m_cr(cr), m_p(p), m_result(result) {}
+  void execute() const {
+    const size_t l0 = m_p.team_scratch_size(0) + m_p.thread_scratch_size(0);
+    const size_t l1 = m_p.team_scratch_size(1) + m_p.thread_scratch_size(1);
+    std::vector<char> b0(l0 + 16), b1(l1 + 16);
+    V v;
+    m_cr.init(v);
+    for (int lr = 0; lr < m_p.league_size(); ++lr) {
+      TeamMember<Serial> m(b0.data(), l0, b1.data(), l1, lr, m_p.league_size());
+      call_functor<typename CR::work_tag>(m_cr.functor(), m, v);
+    }
+    m_cr.final(v);
+    if (m_result) *m_result = v;
+  }
+
+ private:
+  CR m_cr;
+  Policy m_p;
+  V* m_result;
+};
+
+}  // namespace Impl
+}  // namespace Kokkos
+
 #endif

@@ -197,6 +197,8 @@ struct MI355XInstance {
   size_t pinned_bytes     = 0;
   void* kernarg_stage     = nullptr;  // device copy of large closures
   size_t kernarg_stage_bytes = 0;
+  void* team_l1           = nullptr;  // level-1 team scratch pool
+  size_t team_l1_bytes    = 0;
 
   explicit MI355XInstance(mx_stream_t s, bool owns, uint32_t ident)
       : stream(s), owns_stream(owns), id(ident) {
@@ -214,6 +216,7 @@ struct MI355XInstance {
     (void)mx_free(scan_status, nullptr);
     (void)mx_free(scan_values, nullptr);
     (void)mx_free(kernarg_stage, nullptr);
+    (void)mx_free(team_l1, nullptr);
     (void)mx_host_free(pinned);
     if (owns_stream) (void)mx_stream_destroy(stream);
   }
@@ -238,6 +241,7 @@ struct MI355XInstance {
   void* scan_status_space(size_t need) { return grow(scan_status, scan_status_bytes, need, true); }
   void* scan_values_space(size_t need) { return grow(scan_values, scan_values_bytes, need, false); }
   void* kernarg_space(size_t need) { return grow(kernarg_stage, kernarg_stage_bytes, need, false); }
+  void* team_l1_space(size_t need) { return grow(team_l1, team_l1_bytes, need, false); }
   void* pinned_space(size_t need) {
     if (need <= pinned_bytes) return pinned;
     size_t n = pinned_bytes ? pinned_bytes : 4096;

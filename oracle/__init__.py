@@ -53,6 +53,15 @@ def lib() -> C.CDLL:
             "or_stream_add": ([vp, vp, vp, i64, C.c_int], None),
             "or_stream_triad": ([vp, vp, vp, i64, f64, C.c_int, C.c_int], None),
             "or_max_threads": ([], C.c_int),
+            "or_stencil7": ([vp, vp, i64, f64, f64, C.c_int], None),
+            "or_csr_rowptr": ([vp, i64, i64, i64, u64], i64),
+            "or_csr_fill": ([vp, vp, i64, i64, u64], None),
+            "or_spmv_csr": ([vp, vp, vp, vp, vp, vp, i64, C.c_int], None),
+            "or_gups": ([vp, i64, vp, i64, C.c_int], None),
+            "or_zipf_cdf": ([vp, f64, i32], None),
+            "or_zipf_keys": ([vp, i64, vp, i32, u64], None),
+            "or_histogram": ([vp, i64, vp, i32], None),
+            "or_scatter_f64": ([vp, vp, i64, vp, vp, i32], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -162,3 +171,56 @@ def stream_triad(b, c, s, fused=True, nthreads=1):
     a = np.empty_like(b)
     lib().or_stream_triad(_p(a), _p(b), _p(c), b.size, s, 1 if fused else 0, nthreads)
     return a
+
+
+def stencil7(u, n, c0, c1, nthreads=8):
+    v = u.copy()
+    lib().or_stencil7(_p(u), _p(v), n, c0, c1, nthreads)
+    return v
+
+
+def csr(nrows, ncols, lo, hi, seed):
+    rp = np.empty(nrows + 1, np.int64)
+    nnz = lib().or_csr_rowptr(_p(rp), nrows, lo, hi, seed)
+    col = np.empty(nnz, np.int32)
+    val = np.empty(nnz, np.float64)
+    lib().or_csr_fill(_p(col), _p(val), nnz, ncols, seed)
+    return rp, col, val
+
+
+def spmv(rp, col, val, x, nthreads=8):
+    n = rp.size - 1
+    y = np.empty(n)
+    a = np.empty(n)
+    lib().or_spmv_csr(_p(rp), _p(col), _p(val), _p(x), _p(y), _p(a), n, nthreads)
+    return y, a
+
+
+def gups(table_size, stream, nupdates, nthreads=8):
+    t = np.zeros(table_size, np.int64)
+    lib().or_gups(_p(t), table_size, _p(stream), nupdates, nthreads)
+    return t
+
+
+def zipf_keys(n, s, nbins, seed):
+    cdf = np.empty(nbins)
+    lib().or_zipf_cdf(_p(cdf), s, nbins)
+    k = np.empty(n, np.int32)
+    lib().or_zipf_keys(_p(k), n, _p(cdf), nbins, seed)
+    return k
+
+
+def histogram(keys, nbins):
+    keys = np.ascontiguousarray(keys, np.int32)
+    b = np.empty(nbins, np.int32)
+    lib().or_histogram(_p(keys), keys.size, _p(b), nbins)
+    return b
+
+
+def scatter_f64(vals, idx, nbins):
+    vals = np.ascontiguousarray(vals, np.float64)
+    idx = np.ascontiguousarray(idx, np.int32)
+    b = np.empty(nbins)
+    a = np.empty(nbins)
+    lib().or_scatter_f64(_p(vals), _p(idx), vals.size, _p(b), _p(a), nbins)
+    return b, a

@@ -317,3 +317,112 @@ EXPORT void or_stream_triad(double* a, const double* b, const double* c, int64_t
 }
 
 EXPORT int or_max_threads(void) { return omp_get_max_threads(); }
+
+/* ------------------------------------------------------------ config 3 */
+/* 7-point update over the interior of an (n+2)^3 grid, first index fastest
+ * (LayoutLeft); neighbour sum in the order the product functor writes it:
+ * (((((u[i-1]+u[i+1])+u[j-1])+u[j+1])+u[k-1])+u[k+1]).  out = c0*u + c1*s,
+ * unfused (the device may fuse one product; the comparison is toleranced). */
+EXPORT void or_stencil7(const double* u, double* v, int64_t n, double c0, double c1, int nthreads) {
+  const int64_t m = n + 2, sj = m, sk = m * m;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t k = 1; k <= n; ++k)
+    for (int64_t j = 1; j <= n; ++j)
+      for (int64_t i = 1; i <= n; ++i) {
+        const int64_t o = i + j * sj + k * sk;
+        const double s = u[o - 1] + u[o + 1] + u[o - sj] + u[o + sj] + u[o - sk] + u[o + sk];
+        v[o] = c0 * u[o] + c1 * s;
+      }
+}
+
+/* Seeded CSR, the same formulas as the device generator. */
+#define OR_COL_SALT 0x5bd1e9955bd1e995ull
+#define OR_VAL_SALT 0xc2b2ae3d27d4eb4full
+EXPORT int64_t or_csr_rowptr(int64_t* rp, int64_t nrows, int64_t lo, int64_t hi, uint64_t seed) {
+  const uint64_t range = (uint64_t)(hi - lo + 1);
+  int64_t acc = 0;
+  for (int64_t r = 0; r < nrows; ++r) {
+    rp[r] = acc;
+    acc += lo + (int64_t)mulhi64(or_splitmix64(seed, (uint64_t)r), range);
+  }
+  rp[nrows] = acc;
+  return acc;
+}
+EXPORT void or_csr_fill(int32_t* col, double* val, int64_t nnz, int64_t ncols, uint64_t seed) {
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < nnz; ++k) {
+    col[k] = (int32_t)mulhi64(or_splitmix64(seed ^ OR_COL_SALT, (uint64_t)k), (uint64_t)ncols);
+    const double u = (double)(or_splitmix64(seed ^ OR_VAL_SALT, (uint64_t)k) >> 11) * 0x1.0p-53;
+    val[k] = fma(2.0, u, -1.0);
+  }
+}
+/* y = A x, each row summed left to right; also the per-row normwise scale
+ * sum |a_ij x_j| used as the tolerance denominator. */
+EXPORT void or_spmv_csr(const int64_t* rp, const int32_t* col, const double* val, const double* x, double* y,
+                        double* absrow, int64_t nrows, int nthreads) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t r = 0; r < nrows; ++r) {
+    double s = 0.0, a = 0.0;
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+      s += val[k] * x[col[k]];
+      a += fabs(val[k] * x[col[k]]);
+    }
+    y[r] = s;
+    if (absrow) absrow[r] = a;
+  }
+}
+
+/* ------------------------------------------------------------ config 4 */
+/* GUPS: table[stream[2u] & mask] += stream[2u+1] (wrapping). */
+EXPORT void or_gups(int64_t* table, int64_t table_size, const uint64_t* stream, int64_t nupdates, int nthreads) {
+  const uint64_t mask = (uint64_t)table_size - 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t u = 0; u < nupdates; ++u)
+    __atomic_fetch_add((uint64_t*)&table[stream[2 * u] & mask], stream[2 * u + 1], __ATOMIC_RELAXED);
+}
+/* Zipf(s) over [0, nbins): cdf accumulated left to right, normalised, last
+ * entry forced to 1; key = first k with u < cdf[k]. */
+EXPORT void or_zipf_cdf(double* cdf, double s, int32_t nbins) {
+  double acc = 0.0;
+  for (int32_t k = 0; k < nbins; ++k) {
+    acc += pow((double)(k + 1), -s);
+    cdf[k] = acc;
+  }
+  for (int32_t k = 0; k < nbins; ++k) cdf[k] /= acc;
+  cdf[nbins - 1] = 1.0;
+}
+EXPORT void or_zipf_keys(int32_t* keys, int64_t n, const double* cdf, int32_t nbins, uint64_t seed) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const double u = (double)(or_splitmix64(seed, (uint64_t)i) >> 11) * 0x1.0p-53;
+    int32_t lo = 0, hi = nbins - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (u < cdf[mid]) hi = mid;
+      else lo = mid + 1;
+    }
+    keys[i] = lo;
+  }
+}
+EXPORT void or_histogram(const int32_t* keys, int64_t n, int32_t* bins, int32_t nbins) {
+  memset(bins, 0, sizeof(int32_t) * (size_t)nbins);
+  for (int64_t i = 0; i < n; ++i) bins[keys[i]]++;
+}
+/* Per-bin compensated (Neumaier) sums of the scattered values, in index
+ * order, and per-bin sum of |v| for relative errors. */
+EXPORT void or_scatter_f64(const double* vals, const int32_t* idx, int64_t n, double* bins, double* babs,
+                           int32_t nbins) {
+  double* c = (double*)calloc((size_t)nbins, sizeof(double));
+  memset(bins, 0, sizeof(double) * (size_t)nbins);
+  if (babs) memset(babs, 0, sizeof(double) * (size_t)nbins);
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t b = idx[i];
+    const double x = vals[i], s = bins[b], t = s + x;
+    if (fabs(s) >= fabs(x)) c[b] += (s - t) + x;
+    else c[b] += (x - t) + s;
+    bins[b] = t;
+    if (babs) babs[b] += fabs(x);
+  }
+  for (int32_t b = 0; b < nbins; ++b) bins[b] += c[b];
+  free(c);
+}
