@@ -106,6 +106,29 @@ int kkmx_zipf_keys(int32_t* keys_dev, int64_t n, double s, int32_t nbins, uint64
 int kkmx_histogram(const int32_t* keys_dev, int64_t n, int32_t* bins_dev, int32_t nbins, int mode, int lds_bins);
 int kkmx_scatter_add_f64(const double* vals_dev, const int32_t* idx_dev, int64_t n, double* bins_dev, int32_t nbins);
 
+/* ---- batched uneven dispatch (config 5) ----
+ * A fixed set of independent ops over device arrays, planned once and run
+ * many times.  op: 0 = Sum, 1 = Min, 2 = Max (into *result_dev),
+ * 3 = exclusive scan, 4 = inclusive scan (into out_dev, total into
+ * result_dev when not NULL), 5 = out = alpha*in + beta.  type of the input:
+ * 0 = int32, 1 = int64, 2 = float64; the accumulator and outputs are int64
+ * for both integer types and double for float64.
+ * kkmx_batch_run modes: 0 = one parallel_reduce / parallel_scan /
+ * parallel_for per op (plain per-call launches), 1 = one persistent fused
+ * kernel per step, 2 = that kernel captured once into a hipGraph and
+ * replayed, 3 = the per-op dispatches as an Experimental::Graph, replayed.
+ * Runs are asynchronous on kkmx_stream(); kkmx_fence() waits.
+ * (reference: the same ops through core/src/Kokkos_Parallel_Reduce.hpp:1679-1833
+ *  and core/src/Kokkos_Parallel.hpp:130-173,348-454; graph replay through
+ *  core/src/Kokkos_Graph.hpp:43-185 and HIP/Kokkos_HIP_Graph_Impl.hpp:82-221) */
+typedef struct kkmx_batch kkmx_batch;
+int kkmx_batch_create(kkmx_batch** out);
+int kkmx_batch_destroy(kkmx_batch* b);
+int kkmx_batch_add(kkmx_batch* b, int op, int type, const void* in_dev, int64_t n, void* out_dev, void* result_dev,
+                   double alpha, double beta);
+int kkmx_batch_plan(kkmx_batch* b, int64_t* tiles_out);
+int kkmx_batch_run(kkmx_batch* b, int mode, int steps);
+
 /* ---- launch-shape overrides (what the tuning hooks set), -1 = heuristic ---- */
 int kkmx_set_range_config(int pattern, int block, int items, int max_blocks);
 

@@ -107,8 +107,17 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_zipf_keys", [vp, i64, f64, i32, u64])
         _sig(lib, "kkmx_histogram", [vp, i64, vp, i32, C.c_int, C.c_int])
         _sig(lib, "kkmx_scatter_add_f64", [vp, vp, i64, vp, i32])
+        _sig(lib, "kkmx_batch_create", [C.POINTER(vp)])
+        _sig(lib, "kkmx_batch_destroy", [vp])
+        _sig(lib, "kkmx_batch_add", [vp, C.c_int, C.c_int, vp, i64, vp, vp, f64, f64])
+        _sig(lib, "kkmx_batch_plan", [vp, C.POINTER(i64)])
+        _sig(lib, "kkmx_batch_run", [vp, C.c_int, C.c_int])
     elif name == "kkmx_peak":
         _sig(lib, "kkmx_peak_copy", [vp, vp, i64, C.c_int, C.c_int, vp])
+    elif name == "kkmx_selftest":
+        _sig(lib, "kkmx_selftest_count", [])
+        _sig(lib, "kkmx_selftest_name", [C.c_int], C.c_char_p)
+        _sig(lib, "kkmx_selftest_run", [C.c_int, C.c_char_p, C.c_int])
 
 
 def check(rc: int, lib: C.CDLL | None = None) -> None:
@@ -203,6 +212,21 @@ def init(device: int = 0) -> C.CDLL:
     p = load("kkmx_paths")
     check(p.kkmx_init(device), p)
     return p
+
+
+def selftests() -> list[str]:
+    """Names of the restated reference unit tests in libkkmx_selftest.so."""
+    L = load("kkmx_selftest")
+    return [L.kkmx_selftest_name(i).decode() for i in range(L.kkmx_selftest_count())]
+
+
+def run_selftest(name: str) -> tuple[int, str]:
+    """Run one restated unit test on the GPU: (status, message); 0 = pass."""
+    L = load("kkmx_selftest")
+    idx = selftests().index(name)
+    buf = C.create_string_buffer(4096)
+    rc = L.kkmx_selftest_run(idx, buf, len(buf))
+    return rc, buf.value.decode(errors="replace")
 
 
 def fence() -> None:

@@ -39,6 +39,7 @@ LIBS = {
     "kkmx_rt": (["csrc/runtime.cpp"], [], False),
     "kkmx_paths": (["csrc/paths.hip"], [], True),
     "kkmx_peak": (["csrc/peak.hip"], [], True),
+    "kkmx_selftest": (["csrc/selftest.hip"], [], True),
 }
 
 

@@ -7,6 +7,7 @@
 #include <Kokkos_Core.hpp>
 
 #include <cmath>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -715,6 +716,78 @@ int kkmx_set_range_config(int pattern, int block, int items, int max_blocks) {
   o.cfg[pattern] = Kokkos::Impl::RangeConfig{block, items < 1 ? 1 : items, max_blocks < 0 ? 0 : max_blocks};
   o.set[pattern] = true;
   return 0;
+}
+
+// ---------------------------------------------------------------- batched dispatch
+struct kkmx_batch {
+  Kokkos::Experimental::BatchDispatch batch;
+  std::unique_ptr<Kokkos::Experimental::Graph<Kokkos::MI355X>> plain_graph;
+  mx_graph_t fused_graph     = nullptr;
+  mx_graph_exec_t fused_exec = nullptr;
+  ~kkmx_batch() {
+    (void)mx_stream_sync(Kokkos::MI355X().stream());
+    if (fused_exec) (void)mx_graph_exec_destroy(fused_exec);
+    if (fused_graph) (void)mx_graph_destroy(fused_graph);
+  }
+};
+
+int kkmx_batch_create(kkmx_batch** out) {
+  if (!out) return reject("kkmx_batch_create: null handle");
+  return guarded([&] { *out = new kkmx_batch(); });
+}
+int kkmx_batch_destroy(kkmx_batch* b) {
+  return guarded([&] { delete b; });
+}
+int kkmx_batch_add(kkmx_batch* b, int op, int type, const void* in, int64_t n, void* out, void* result, double alpha,
+                   double beta) {
+  if (!b) return reject("kkmx_batch_add: null handle");
+  if (op < 0 || op > 5) return reject("kkmx_batch_add: op must be 0..5");
+  if (type < 0 || type > 2) return reject("kkmx_batch_add: type must be 0, 1 or 2");
+  return guarded([&] {
+    b->batch.add(Kokkos::Experimental::BatchOp(op), Kokkos::Experimental::BatchType(type), in, n, out, result, alpha,
+                 beta);
+  });
+}
+int kkmx_batch_plan(kkmx_batch* b, int64_t* tiles_out) {
+  if (!b) return reject("kkmx_batch_plan: null handle");
+  return guarded([&] {
+    b->batch.plan();
+    if (tiles_out) *tiles_out = b->batch.tiles();
+  });
+}
+int kkmx_batch_run(kkmx_batch* b, int mode, int steps) {
+  if (!b) return reject("kkmx_batch_run: null handle");
+  if (mode < 0 || mode > 3) return reject("kkmx_batch_run: mode must be 0..3");
+  if (steps < 0) return reject("kkmx_batch_run: negative step count");
+  return guarded([&] {
+    Kokkos::MI355X ex;
+    b->batch.plan();
+    switch (mode) {
+      case 0:
+        for (int s = 0; s < steps; ++s) b->batch.submit_plain(ex);
+        break;
+      case 1:
+        for (int s = 0; s < steps; ++s) b->batch.submit(ex);
+        break;
+      case 2:
+        if (!b->fused_exec) {
+          KKMX_SAFE_CALL(mx_stream_sync(ex.stream()));
+          KKMX_SAFE_CALL(mx_graph_capture_begin(ex.stream()));
+          b->batch.submit(ex);
+          KKMX_SAFE_CALL(mx_graph_capture_end(ex.stream(), &b->fused_graph));
+          KKMX_SAFE_CALL(mx_graph_instantiate(&b->fused_exec, b->fused_graph));
+        }
+        for (int s = 0; s < steps; ++s) KKMX_SAFE_CALL(mx_graph_launch(b->fused_exec, ex.stream()));
+        break;
+      case 3:
+        if (!b->plain_graph) {
+          b->plain_graph.reset(new Kokkos::Experimental::Graph<Kokkos::MI355X>(b->batch.plain_graph(ex)));
+          b->plain_graph->instantiate();
+        }
+        for (int s = 0; s < steps; ++s) b->plain_graph->submit(ex);
+        break;
+    }
+  });
 }
 
 }  // extern "C"

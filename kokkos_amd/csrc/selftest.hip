@@ -1,0 +1,1540 @@
+// Restated unit tests of the reference (core/unit_test/*.hpp), compiled
+// against the MI355X execution space as ordinary Kokkos user code.
+//
+// Each test is a host function that dispatches kernels and checks closed-form
+// answers; a failed check throws with the file:line and the values.  The
+// tests are exposed through a small C-ABI (kkmx_selftest_*) so pytest can
+// run them one by one on the GPU.  Closed forms and invariants are the ones
+// the reference's tests assert; each test names the reference test it follows.
+#include <Kokkos_Core.hpp>
+
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Failure : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+struct Entry {
+  const char* name;
+  void (*fn)();
+};
+std::vector<Entry>& registry() {
+  static std::vector<Entry> r;
+  return r;
+}
+struct Registrar {
+  Registrar(const char* n, void (*f)()) { registry().push_back({n, f}); }
+};
+
+#define KKMX_TEST(NAME)                                 \
+  static void NAME();                                   \
+  static Registrar KOKKOS_IMPL_CONCAT(reg_, NAME)(#NAME, NAME); \
+  static void NAME()
+
+template <class A, class B>
+void check_eq(const A& a, const B& b, const char* ea, const char* eb, const char* file, int line) {
+  if (!(a == b)) {
+    std::ostringstream os;
+    os << file << ":" << line << ": " << ea << " == " << eb << " failed: " << a << " vs " << b;
+    throw Failure(os.str());
+  }
+}
+#define CHECK_EQ(a, b) check_eq((a), (b), #a, #b, __FILE__, __LINE__)
+#define CHECK(c)                                                                                  \
+  do {                                                                                            \
+    if (!(c)) throw Failure(std::string(__FILE__) + ":" + std::to_string(__LINE__) + ": " #c); \
+  } while (0)
+
+using Exec = Kokkos::MI355X;
+
+template <class V>
+typename V::non_const_value_type read0(const Exec& ex, const V& v) {
+  typename V::non_const_value_type x;
+  Kokkos::deep_copy(ex, x, v);
+  ex.fence();
+  return x;
+}
+
+// =========================================================================
+// Graph (core/unit_test/TestGraph.hpp:27-420)
+// =========================================================================
+template <class T>
+using AtomicScalar = Kokkos::View<T, Exec, Kokkos::MemoryTraits<Kokkos::Atomic>>;
+
+struct CountTestFunctor {
+  using value_type = int;
+  AtomicScalar<int> count;
+  AtomicScalar<int> bugs;
+  int expected_count_min;
+  int expected_count_max;
+  template <class... Ts>
+  KOKKOS_FUNCTION void operator()(Ts&&...) const noexcept {
+    bugs() += int(count() > expected_count_max || count() < expected_count_min);
+    count()++;
+  }
+};
+struct SetViewToValueFunctor {
+  using value_type = int;
+  AtomicScalar<int> v;
+  int value;
+  template <class... Ts>
+  KOKKOS_FUNCTION void operator()(Ts&&...) const noexcept {
+    v() = value;
+  }
+};
+struct SetResultToViewFunctor {
+  using value_type = int;
+  AtomicScalar<int> v;
+  template <class U>
+  KOKKOS_FUNCTION void operator()(U&&, value_type& val) const noexcept {
+    val += v();
+  }
+};
+// The reference aborts if a zero-work reduction calls the functor; here a
+// call is counted instead so the check fails without trapping the GPU.
+struct NoOpReduceFunctor {
+  AtomicScalar<int> bugs;
+  KOKKOS_FUNCTION void operator()(int, int&) const { bugs() += 1; }
+  KOKKOS_FUNCTION void operator()(int, int, int&) const { bugs() += 1; }
+  KOKKOS_FUNCTION void operator()(const Kokkos::TeamPolicy<Exec>::member_type&, int&) const { bugs() += 1; }
+};
+
+struct GraphFixture {
+  Exec ex{};
+  AtomicScalar<int> count{Kokkos::view_alloc("count", ex)};
+  AtomicScalar<int> bugs{Kokkos::view_alloc("bugs", ex)};
+  GraphFixture() {
+    Kokkos::deep_copy(ex, count, 0);
+    Kokkos::deep_copy(ex, bugs, 0);
+    ex.fence();
+  }
+};
+
+KKMX_TEST(graph_submit_once) {
+  GraphFixture t;
+  auto graph = Kokkos::Experimental::create_graph<Exec>([&](auto root) {
+    root.then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 0, 0});
+  });
+  graph.submit();
+  CHECK_EQ(read0(graph.get_execution_space(), t.count), 1);
+  CHECK_EQ(read0(graph.get_execution_space(), t.bugs), 0);
+}
+
+KKMX_TEST(graph_submit_once_rvalue) {
+  GraphFixture t;
+  Kokkos::Experimental::create_graph(t.ex, [&](auto root) {
+    root.then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 0, 0});
+  }).submit();
+  CHECK_EQ(read0(t.ex, t.count), 1);
+  CHECK_EQ(read0(t.ex, t.bugs), 0);
+}
+
+KKMX_TEST(graph_instantiate_and_submit_once) {
+  GraphFixture t;
+  auto graph = Kokkos::Experimental::create_graph(t.ex, [&](auto root) {
+    root.then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 0, 0});
+  });
+  graph.instantiate();
+  graph.submit();
+  CHECK_EQ(read0(t.ex, t.count), 1);
+  CHECK_EQ(read0(t.ex, t.bugs), 0);
+}
+
+// Submitting on another instance than the one the graph was built with
+// (TestGraph.hpp:203-230).
+KKMX_TEST(graph_submit_onto_another_instance) {
+  GraphFixture t;
+  mx_stream_t s;
+  KKMX_SAFE_CALL(mx_stream_create(&s));
+  {
+    Exec other(s, true);
+    auto graph = Kokkos::Experimental::create_graph(t.ex, [&](auto root) {
+      root.then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 0, 0});
+    });
+    graph.instantiate();
+    t.ex.fence();
+    graph.submit(other);
+    CHECK_EQ(read0(other, t.count), 1);
+    CHECK_EQ(read0(other, t.bugs), 0);
+  }
+}
+
+KKMX_TEST(graph_create_graph_no_closure_with_exec) {
+  GraphFixture t;
+  Kokkos::Experimental::Graph<Exec> graph{t.ex};
+  auto root = graph.root_node();
+  root.then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 0, 0});
+  graph.submit(t.ex);
+  CHECK_EQ(read0(t.ex, t.count), 1);
+  CHECK_EQ(read0(t.ex, t.bugs), 0);
+}
+
+KKMX_TEST(graph_submit_six) {
+  GraphFixture t;
+  auto graph = Kokkos::Experimental::create_graph(t.ex, [&](auto root) {
+    auto f_setup_count = root.then_parallel_for(1, SetViewToValueFunctor{t.count, 0});
+    auto f_setup_bugs  = root.then_parallel_for(1, SetViewToValueFunctor{t.bugs, 0});
+    auto ready         = Kokkos::Experimental::when_all(f_setup_count, f_setup_bugs);
+    ready.then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 0, 6});
+    ready.then_parallel_for(Kokkos::RangePolicy<Exec>{0, 1}, CountTestFunctor{t.count, t.bugs, 0, 6});
+    ready.then_parallel_for(Kokkos::MDRangePolicy<Exec, Kokkos::Rank<2>>{{0, 0}, {1, 1}},
+                            CountTestFunctor{t.count, t.bugs, 0, 6});
+    ready.then_parallel_for(Kokkos::TeamPolicy<Exec>{1, 1}, CountTestFunctor{t.count, t.bugs, 0, 6});
+    ready.then_parallel_for(2, CountTestFunctor{t.count, t.bugs, 0, 6});
+  });
+  graph.submit();
+  CHECK_EQ(read0(t.ex, t.count), 6);
+  CHECK_EQ(read0(t.ex, t.bugs), 0);
+}
+
+KKMX_TEST(graph_when_all_cycle) {
+  GraphFixture t;
+  Kokkos::View<int, Exec> reduction_out{"reduction_out"};
+  Kokkos::Experimental::create_graph(t.ex, [&](auto root) {
+    auto f1 = root.then_parallel_for(1, SetViewToValueFunctor{t.count, 0});
+    auto f2 = f1.then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 0, 0});
+    auto f3 = f2.then_parallel_for(5, CountTestFunctor{t.count, t.bugs, 1, 5});
+    auto f4 = Kokkos::Experimental::when_all(f2, f3).then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 6, 6});
+    Kokkos::Experimental::when_all(f1, f4, f3).then_parallel_reduce(6, SetResultToViewFunctor{t.count}, reduction_out);
+  }).submit();
+  CHECK_EQ(read0(t.ex, t.bugs), 0);
+  CHECK_EQ(read0(t.ex, t.count), 7);
+  CHECK_EQ(read0(t.ex, reduction_out), 42);
+}
+
+// repeat_chain (TestGraph.hpp:333-372) with device-resident results: the
+// MI355X cannot write HostSpace, so the results are copied back after each
+// replay instead.
+KKMX_TEST(graph_repeat_chain) {
+  GraphFixture t;
+  Kokkos::View<int, Exec> count_out{"count_out"}, bugs_out{"bugs_out"};
+  auto graph = Kokkos::Experimental::create_graph(t.ex, [&](auto root) {
+    root.then_parallel_for(1, SetViewToValueFunctor{t.count, 0})
+        .then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 0, 0})
+        .then_parallel_for(1, CountTestFunctor{t.count, t.bugs, 1, 1})
+        .then_parallel_reduce(1, SetResultToViewFunctor{t.count}, count_out)
+        .then_parallel_reduce(1, SetResultToViewFunctor{t.bugs}, Kokkos::Sum<int, Kokkos::MI355XSpace>{bugs_out});
+  });
+  for (int i = 0; i < 10; ++i) {
+    graph.submit();
+    t.ex.fence();
+    CHECK_EQ(read0(t.ex, count_out), 2);
+    CHECK_EQ(read0(t.ex, bugs_out), 0);
+  }
+}
+
+KKMX_TEST(graph_zero_work_reduce) {
+  GraphFixture t;
+  auto graph = Kokkos::Experimental::create_graph(t.ex, [&](Kokkos::Experimental::GraphNodeRef<Exec> root) {
+    NoOpReduceFunctor no_op{t.bugs};
+    root.then_parallel_reduce(Kokkos::RangePolicy<Exec>(0, 0), no_op, t.count)
+        .then_parallel_reduce(Kokkos::MDRangePolicy<Exec, Kokkos::Rank<2>>{{0, 0}, {0, 0}}, no_op, t.count)
+        .then_parallel_reduce(Kokkos::TeamPolicy<Exec>{0, Kokkos::AUTO}, no_op, t.count);
+  });
+  graph.submit();
+  Kokkos::deep_copy(t.ex, t.count, 1);
+  graph.submit();
+  CHECK_EQ(read0(t.ex, t.count), 0);
+  CHECK_EQ(read0(t.ex, t.bugs), 0);
+}
+
+KKMX_TEST(graph_empty_graph) {
+  Exec ex;
+  auto graph = Kokkos::Experimental::create_graph(ex, [](auto) {});
+  graph.instantiate();
+  graph.submit(ex);
+  ex.fence();
+}
+
+// A DAG of a fill, a scan, a reduce of the scan and a dependent for, replayed
+// many times: every replay must equal the eager result (scan state and
+// reduction counters are self-cleaning, so replays need no resets).
+struct FillIota {
+  Kokkos::View<int64_t*, Exec> x;
+  int64_t mult;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { x(i) = (i % 13) * mult - 5; }
+};
+struct ExclScan {
+  Kokkos::View<int64_t*, Exec> x, y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int64_t& upd, const bool final) const {
+    if (final) y(i) = upd;
+    upd += x(i);
+  }
+};
+struct SumOf {
+  Kokkos::View<int64_t*, Exec> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int64_t& s) const { s += y(i) ^ i; }
+};
+struct MaxOf {
+  Kokkos::View<int64_t*, Exec> x;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int64_t& m) const {
+    if (x(i) * 3 + i > m) m = x(i) * 3 + i;
+  }
+};
+struct CombineInto {
+  Kokkos::View<int64_t, Exec> s, m;
+  Kokkos::View<int64_t[2], Exec> out;
+  KOKKOS_INLINE_FUNCTION void operator()() const {
+    out(0) = s();
+    out(1) = m() - s();
+  }
+};
+
+KKMX_TEST(graph_dag_replay_matches_eager) {
+  Exec ex;
+  const int64_t n = 3000017;
+  Kokkos::View<int64_t*, Exec> x("x", n), y("y", n);
+  Kokkos::View<int64_t, Exec> s("s"), m("m");
+  Kokkos::View<int64_t[2], Exec> out("out");
+  // eager
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(ex, 0, n), FillIota{x, 7});
+  Kokkos::parallel_scan(Kokkos::RangePolicy<Exec>(ex, 0, n), ExclScan{x, y});
+  Kokkos::parallel_reduce(Kokkos::RangePolicy<Exec>(ex, 0, n), SumOf{y}, s);
+  Kokkos::parallel_reduce(Kokkos::RangePolicy<Exec>(ex, 0, n), MaxOf{x}, Kokkos::Max<int64_t, Kokkos::MI355XSpace>(m));
+  int64_t s_eager = read0(ex, s), m_eager = read0(ex, m);
+  auto y_eager = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), y);
+  Kokkos::deep_copy(ex, y, 0);
+  Kokkos::deep_copy(ex, s, 0);
+  ex.fence();
+  auto graph = Kokkos::Experimental::create_graph(ex, [&](auto root) {
+    auto f = root.then_parallel_for("fill", Kokkos::RangePolicy<Exec>(0, n), FillIota{x, 7});
+    auto sc = f.then_parallel_scan("scan", Kokkos::RangePolicy<Exec>(0, n), ExclScan{x, y});
+    auto r1 = sc.then_parallel_reduce("sum", Kokkos::RangePolicy<Exec>(0, n), SumOf{y}, s);
+    auto r2 = f.then_parallel_reduce("max", Kokkos::RangePolicy<Exec>(0, n), MaxOf{x},
+                                     Kokkos::Max<int64_t, Kokkos::MI355XSpace>(m));
+    Kokkos::Experimental::when_all(r1, r2).then("combine", CombineInto{s, m, out});
+  });
+  for (int rep = 0; rep < 50; ++rep) {
+    graph.submit();
+    ex.fence();
+    CHECK_EQ(read0(ex, s), s_eager);
+    CHECK_EQ(read0(ex, m), m_eager);
+  }
+  auto y_graph = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), y);
+  for (int64_t i = 0; i < n; ++i)
+    if (y_graph(i) != y_eager(i)) CHECK_EQ(y_graph(i), y_eager(i));
+  auto o = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+  CHECK_EQ(o(0), s_eager);
+  CHECK_EQ(o(1), m_eager - s_eager);
+  size_t nodes = 0;
+  KKMX_SAFE_CALL(mx_graph_node_count(graph.native_graph(), &nodes));
+  CHECK(nodes >= 5);
+}
+
+// The same DAG on the Serial space runs nodes in order at submit.
+KKMX_TEST(graph_serial_backend) {
+  Kokkos::Serial sx;
+  Kokkos::View<int, Kokkos::HostSpace> v("v"), r("r");
+  auto graph = Kokkos::Experimental::create_graph(sx, [&](auto root) {
+    auto a = root.then_parallel_for(Kokkos::RangePolicy<Kokkos::Serial>(0, 10), [=](int i) { v() += i; });
+    a.then_parallel_reduce(Kokkos::RangePolicy<Kokkos::Serial>(0, 4), [=](int i, int& s) { s += v() + i; }, r);
+  });
+  graph.submit();
+  graph.submit();
+  CHECK_EQ(v(), 90);
+  CHECK_EQ(r(), 4 * 90 + 6);
+}
+
+// =========================================================================
+// Reduce (core/unit_test/TestReduce.hpp:27-260,520-700)
+// =========================================================================
+template <class ScalarType>
+struct ReduceFunctor {
+  using execution_space = Exec;
+  using size_type       = Exec::size_type;
+  struct value_type {
+    ScalarType value[3];
+  };
+  const size_type nwork;
+  ReduceFunctor(size_type n) : nwork(n) {}
+  KOKKOS_INLINE_FUNCTION void join(value_type& dst, const value_type& src) const {
+    dst.value[0] += src.value[0];
+    dst.value[1] += src.value[1];
+    dst.value[2] += src.value[2];
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(size_type iwork, value_type& dst) const {
+    dst.value[0] += 1;
+    dst.value[1] += iwork + 1;
+    dst.value[2] += nwork - iwork;
+  }
+};
+struct ReduceFunctorFinal : ReduceFunctor<int64_t> {
+  using value_type = ReduceFunctor<int64_t>::value_type;
+  ReduceFunctorFinal(size_t n) : ReduceFunctor<int64_t>(n) {}
+  KOKKOS_INLINE_FUNCTION void final(value_type& dst) const {
+    dst.value[0] = -dst.value[0];
+    dst.value[1] = -dst.value[1];
+    dst.value[2] = -dst.value[2];
+  }
+};
+struct ReducerTag {};
+struct ReduceFunctorFinalTag {
+  using execution_space = Exec;
+  using size_type       = Exec::size_type;
+  struct value_type {
+    int64_t value[3];
+  };
+  const size_type nwork;
+  ReduceFunctorFinalTag(size_type n) : nwork(n) {}
+  KOKKOS_INLINE_FUNCTION void join(ReducerTag, value_type& dst, const value_type& src) const {
+    dst.value[0] += src.value[0];
+    dst.value[1] += src.value[1];
+    dst.value[2] += src.value[2];
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(ReducerTag, size_type iwork, value_type& dst) const {
+    dst.value[0] -= 1;
+    dst.value[1] -= iwork + 1;
+    dst.value[2] -= nwork - iwork;
+  }
+  KOKKOS_INLINE_FUNCTION void final(ReducerTag, value_type& dst) const {
+    ++dst.value[0];
+    ++dst.value[1];
+    ++dst.value[2];
+  }
+};
+
+template <class ScalarType>
+void reduce_struct_value(uint64_t nw) {
+  const uint64_t nsum = nw % 2 ? nw * ((nw + 1) / 2) : (nw / 2) * (nw + 1);
+  for (int rep = 0; rep < 4; ++rep) {
+    typename ReduceFunctor<ScalarType>::value_type r;
+    Kokkos::parallel_reduce(nw, ReduceFunctor<ScalarType>(nw), r);
+    for (int j = 0; j < 3; ++j) CHECK_EQ(r.value[j], (ScalarType)(j == 0 ? nw : nsum));
+    ReduceFunctorFinal::value_type rf;
+    if (rep % 2) Kokkos::parallel_reduce(nw, ReduceFunctorFinal(nw), rf);
+    else Kokkos::parallel_reduce("Reduce", nw, ReduceFunctorFinal(nw), rf);
+    for (int j = 0; j < 3; ++j) CHECK_EQ(-rf.value[j], (int64_t)(j == 0 ? nw : nsum));
+    ReduceFunctorFinalTag::value_type rt;
+    Kokkos::parallel_reduce(Kokkos::RangePolicy<Exec, ReducerTag>(0, nw), ReduceFunctorFinalTag(nw), rt);
+    for (int j = 0; j < 3; ++j) CHECK_EQ(rt.value[j], 1 - (int64_t)(j == 0 ? nw : nsum));
+  }
+}
+KKMX_TEST(reduce_int64_struct_value) {
+  reduce_struct_value<int64_t>(0);
+  reduce_struct_value<int64_t>(1000000);
+}
+KKMX_TEST(reduce_double_struct_value) {
+  reduce_struct_value<double>(0);
+  reduce_struct_value<double>(1000000);
+}
+
+template <class ScalarType>
+struct RuntimeReduceFunctor {
+  using execution_space = Exec;
+  using value_type      = ScalarType[];
+  using size_type       = Exec::size_type;
+  const unsigned value_count;
+  const size_type nwork;
+  RuntimeReduceFunctor(size_type n, unsigned c) : value_count(c), nwork(n) {}
+  KOKKOS_INLINE_FUNCTION void init(ScalarType dst[]) const {
+    for (unsigned i = 0; i < value_count; ++i) dst[i] = 0;
+  }
+  KOKKOS_INLINE_FUNCTION void join(ScalarType dst[], const ScalarType src[]) const {
+    for (unsigned i = 0; i < value_count; ++i) dst[i] += src[i];
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(size_type iwork, ScalarType dst[]) const {
+    const size_type tmp[3] = {1, iwork + 1, nwork - iwork};
+    for (size_type i = 0; i < static_cast<size_type>(value_count); ++i) dst[i] += tmp[i % 3];
+  }
+};
+template <class ScalarType>
+struct RuntimeReduceMinMax {
+  using execution_space = Exec;
+  using value_type      = ScalarType[];
+  using size_type       = Exec::size_type;
+  const unsigned value_count;
+  const size_type nwork;
+  const ScalarType amin, amax;
+  RuntimeReduceMinMax(size_type n, unsigned c)
+      : value_count(c), nwork(n), amin(std::numeric_limits<ScalarType>::min()),
+        amax(std::numeric_limits<ScalarType>::max()) {}
+  KOKKOS_INLINE_FUNCTION void init(ScalarType dst[]) const {
+    for (unsigned i = 0; i < value_count; ++i) dst[i] = i % 2 ? amax : amin;
+  }
+  KOKKOS_INLINE_FUNCTION void join(ScalarType dst[], const ScalarType src[]) const {
+    for (unsigned i = 0; i < value_count; ++i)
+      dst[i] = i % 2 ? (dst[i] < src[i] ? dst[i] : src[i]) : (dst[i] > src[i] ? dst[i] : src[i]);
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(size_type iwork, ScalarType dst[]) const {
+    const ScalarType tmp[2] = {ScalarType(iwork + 1), ScalarType(nwork - iwork)};
+    for (size_type i = 0; i < static_cast<size_type>(value_count); ++i)
+      dst[i] = i % 2 ? (dst[i] < tmp[i % 2] ? dst[i] : tmp[i % 2]) : (dst[i] > tmp[i % 2] ? dst[i] : tmp[i % 2]);
+  }
+};
+struct RuntimeReduceFunctorFinal : RuntimeReduceFunctor<int64_t> {
+  using value_type = int64_t[];
+  RuntimeReduceFunctorFinal(size_t n, unsigned c) : RuntimeReduceFunctor<int64_t>(n, c) {}
+  KOKKOS_INLINE_FUNCTION void final(int64_t dst[]) const {
+    for (unsigned i = 0; i < value_count; ++i) dst[i] = -dst[i];
+  }
+};
+
+// TestReduceDynamic (TestReduce.hpp:381-470).
+template <class ScalarType>
+void reduce_dynamic(uint64_t nw) {
+  const uint64_t nsum = nw % 2 ? nw * ((nw + 1) / 2) : (nw / 2) * (nw + 1);
+  const ScalarType amin = nw ? ScalarType(1) : std::numeric_limits<ScalarType>::max();
+  const ScalarType amax = nw ? ScalarType(nw) : std::numeric_limits<ScalarType>::min();
+  for (unsigned count = 1; count <= 9; count += 2) {
+    std::vector<ScalarType> r(count), rmm(count);
+    std::vector<int64_t> rf(count);
+    Kokkos::parallel_reduce(nw, RuntimeReduceFunctor<ScalarType>(nw, count), r.data());
+    Kokkos::parallel_reduce(nw, RuntimeReduceMinMax<ScalarType>(nw, count), rmm.data());
+    Kokkos::parallel_reduce(nw, RuntimeReduceFunctorFinal(nw, count), rf.data());
+    for (unsigned j = 0; j < count; ++j) {
+      const uint64_t correct = 0 == j % 3 ? nw : nsum;
+      CHECK_EQ(r[j], (ScalarType)correct);
+      CHECK_EQ(rf[j], -(int64_t)correct);
+      CHECK_EQ(rmm[j], (j % 2) ? amin : amax);
+    }
+  }
+}
+KKMX_TEST(reduce_dynamic_int64) {
+  reduce_dynamic<int64_t>(0);
+  reduce_dynamic<int64_t>(1000000);
+}
+KKMX_TEST(reduce_dynamic_double) {
+  reduce_dynamic<double>(0);
+  reduce_dynamic<double>(1000000);
+}
+// TestReduceDynamicView (TestReduce.hpp:472-518): the result is a View.
+KKMX_TEST(reduce_dynamic_view) {
+  for (uint64_t nw : {uint64_t(0), uint64_t(1000000)}) {
+    const uint64_t nsum = nw % 2 ? nw * ((nw + 1) / 2) : (nw / 2) * (nw + 1);
+    const unsigned count = 7;
+    Kokkos::View<int64_t*, Kokkos::HostSpace> host("host", count);
+    Kokkos::View<int64_t*, Exec> dev("dev", count);
+    Kokkos::parallel_reduce(nw, RuntimeReduceFunctor<int64_t>(nw, count), host);
+    Kokkos::parallel_reduce(nw, RuntimeReduceFunctor<int64_t>(nw, count), dev);
+    Kokkos::fence();
+    auto hd = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), dev);
+    for (unsigned j = 0; j < count; ++j) {
+      const uint64_t correct = 0 == j % 3 ? nw : nsum;
+      CHECK_EQ(host(j), (int64_t)correct);
+      CHECK_EQ(hd(j), (int64_t)correct);
+    }
+  }
+}
+
+struct CombinedReduceFunctorSameType {
+  using execution_space = Exec;
+  using size_type       = Exec::size_type;
+  const size_type nwork;
+  explicit CombinedReduceFunctorSameType(size_type n) : nwork(n) {}
+  KOKKOS_INLINE_FUNCTION void operator()(size_type iwork, int64_t& d1, int64_t& d2, int64_t& d3) const {
+    d1 += 1;
+    d2 += iwork + 1;
+    d3 += nwork - iwork;
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(size_type iwork, size_type z1, size_type z2, int64_t& d1, int64_t& d2,
+                                         int64_t& d3) const {
+    d1 += 1 + z1;
+    d2 += iwork + 1 + z2;
+    d3 += nwork - iwork;
+  }
+};
+KKMX_TEST(reduce_int_combined) {
+  constexpr uint64_t nw = 1000, nsum = (nw / 2) * (nw + 1);
+  int64_t r1 = 0, r2 = 0, r3 = 0;
+  Kokkos::parallel_reduce("int_combined_reduce", Kokkos::RangePolicy<Exec>(0, nw),
+                          CombinedReduceFunctorSameType(nw), r1, r2, r3);
+  CHECK_EQ(uint64_t(r1), nw);
+  CHECK_EQ(uint64_t(r2), nsum);
+  CHECK_EQ(uint64_t(r3), nsum);
+}
+KKMX_TEST(reduce_mdrange_combined) {
+  constexpr uint64_t nw = 1000, nsum = (nw / 2) * (nw + 1);
+  int64_t r1 = 0, r2 = 0, r3 = 0;
+  Kokkos::parallel_reduce("int_combined_reduce_mdrange",
+                          Kokkos::MDRangePolicy<Exec, Kokkos::Rank<3>>({{0, 0, 0}}, {{int64_t(nw), 1, 1}}),
+                          CombinedReduceFunctorSameType(nw), r1, r2, r3);
+  CHECK_EQ(uint64_t(r1), nw);
+  CHECK_EQ(uint64_t(r2), nsum);
+  CHECK_EQ(uint64_t(r3), nsum);
+  r1 = r2 = r3 = 0;
+  Kokkos::parallel_reduce(
+      "int_combined_reduce_mdrange",
+      Kokkos::MDRangePolicy<Exec, Kokkos::Rank<3>, Kokkos::Schedule<Kokkos::Dynamic>>({{0, 0, 0}}, {{int64_t(nw), 1, 1}}),
+      CombinedReduceFunctorSameType(nw), r1, r2, r3);
+  CHECK_EQ(uint64_t(r1), nw);
+  CHECK_EQ(uint64_t(r2), nsum);
+  CHECK_EQ(uint64_t(r3), nsum);
+}
+KKMX_TEST(reduce_int_combined_mixed) {
+  constexpr uint64_t nw = 1000, nsum = (nw / 2) * (nw + 1);
+  {
+    Kokkos::View<int64_t, Kokkos::HostSpace> r1("r1"), r3("r3");
+    int64_t r2 = 0;
+    Kokkos::parallel_reduce("mixed", Kokkos::RangePolicy<Exec>(0, nw), CombinedReduceFunctorSameType(nw), r1, r2,
+                            Kokkos::Sum<int64_t, Kokkos::HostSpace>{r3});
+    CHECK_EQ(r1(), int64_t(nw));
+    CHECK_EQ(r2, int64_t(nsum));
+    CHECK_EQ(r3(), int64_t(nsum));
+  }
+  {
+    Kokkos::View<int64_t, Kokkos::MI355XSpace> r1("r1"), r3("r3");
+    int64_t r2 = 0;
+    Kokkos::parallel_reduce("mixed", Kokkos::RangePolicy<Exec>(0, nw), CombinedReduceFunctorSameType(nw), r1, r2,
+                            Kokkos::Sum<int64_t, Kokkos::MI355XSpace>{r3});
+    int64_t h1, h3;
+    Kokkos::deep_copy(h1, r1);
+    Kokkos::deep_copy(h3, r3);
+    CHECK_EQ(h1, int64_t(nw));
+    CHECK_EQ(r2, int64_t(nsum));
+    CHECK_EQ(h3, int64_t(nsum));
+  }
+}
+
+// reduction_with_large_iteration_count (TestReduce.hpp:656-680): 2^39-2^8+1
+// ones summed in double are exact.
+struct OnesF64 {
+  KOKKOS_FUNCTION void operator()(const int64_t, double& u) const { u += 1.0; }
+};
+KKMX_TEST(reduce_large_iteration_count) {
+  const int64_t N = (int64_t(1) << 39) - (int64_t(1) << 8) + 1;
+  double nu = 0;
+  Kokkos::parallel_reduce("sample reduction", Kokkos::RangePolicy<Exec, Kokkos::IndexType<int64_t>>(0, N), OnesF64(),
+                          nu);
+  CHECK_EQ(nu, double(N));
+}
+
+// =========================================================================
+// Reducers (core/unit_test/TestReducers.hpp): each reducer over a known array
+// against a host fold; Loc reducers keep the first index among ties.
+// =========================================================================
+template <class T>
+struct ValuesOf {
+  Kokkos::View<T*, Exec> v;
+};
+template <class T, class R>
+struct ReduceWith {
+  Kokkos::View<T*, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, typename R::value_type& u) const;
+};
+template <class T>
+Kokkos::View<T*, Exec> upload(const std::vector<T>& h) {
+  Kokkos::View<T*, Exec> d("d", h.size());
+  Kokkos::View<const T*, Kokkos::HostSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>> hv(h.data(), h.size());
+  Kokkos::deep_copy(d, hv);
+  return d;
+}
+template <class T>
+struct SumF {
+  Kokkos::View<T*, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, T& u) const { u += v(i); }
+};
+template <class T>
+struct ProdF {
+  Kokkos::View<T*, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, T& u) const { u *= v(i); }
+};
+template <class T>
+struct MinF {
+  Kokkos::View<T*, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, T& u) const {
+    if (v(i) < u) u = v(i);
+  }
+};
+template <class T>
+struct MaxF {
+  Kokkos::View<T*, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, T& u) const {
+    if (v(i) > u) u = v(i);
+  }
+};
+template <class T>
+struct LAndF {
+  Kokkos::View<T*, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, T& u) const { u = u && v(i); }
+};
+template <class T>
+struct LOrF {
+  Kokkos::View<T*, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, T& u) const { u = u || v(i); }
+};
+template <class T>
+struct BAndF {
+  Kokkos::View<T*, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, T& u) const { u = u & v(i); }
+};
+template <class T>
+struct BOrF {
+  Kokkos::View<T*, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, T& u) const { u = u | v(i); }
+};
+template <class T>
+struct MinLocF {
+  Kokkos::View<T*, Exec> v;
+  using VT = Kokkos::ValLocScalar<T, int>;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, VT& u) const {
+    if (v(i) < u.val) {
+      u.val = v(i);
+      u.loc = i;
+    }
+  }
+};
+template <class T>
+struct MaxLocF {
+  Kokkos::View<T*, Exec> v;
+  using VT = Kokkos::ValLocScalar<T, int>;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, VT& u) const {
+    if (v(i) > u.val) {
+      u.val = v(i);
+      u.loc = i;
+    }
+  }
+};
+template <class T>
+struct MinMaxF {
+  Kokkos::View<T*, Exec> v;
+  using VT = Kokkos::MinMaxScalar<T>;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, VT& u) const {
+    if (v(i) < u.min_val) u.min_val = v(i);
+    if (v(i) > u.max_val) u.max_val = v(i);
+  }
+};
+template <class T>
+struct MinMaxLocF {
+  Kokkos::View<T*, Exec> v;
+  using VT = Kokkos::MinMaxLocScalar<T, int>;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, VT& u) const {
+    if (v(i) < u.min_val) {
+      u.min_val = v(i);
+      u.min_loc = i;
+    }
+    if (v(i) > u.max_val) {
+      u.max_val = v(i);
+      u.max_loc = i;
+    }
+  }
+};
+
+template <class T>
+void reducers_for(const std::vector<T>& h) {
+  const int n = static_cast<int>(h.size());
+  auto d      = upload(h);
+  using P     = Kokkos::RangePolicy<Exec>;
+  T sum = 0, prod = 1, mn = std::numeric_limits<T>::max(), mx = std::numeric_limits<T>::lowest();
+  int mnl = -1, mxl = -1;
+  for (int i = 0; i < n; ++i) {
+    sum += h[i];
+    prod *= (i < 20 ? h[i] : T(1));
+    if (h[i] < mn) { mn = h[i]; mnl = i; }
+    if (h[i] > mx) { mx = h[i]; mxl = i; }
+  }
+  {
+    T r = 0;
+    Kokkos::parallel_reduce(P(0, n), SumF<T>{d}, Kokkos::Sum<T>(r));
+    if (std::is_integral<T>::value) CHECK_EQ(r, sum);
+    else CHECK(std::abs(double(r - sum)) <= 1e-12 * std::abs(double(sum)) + 1e-12);
+  }
+  {
+    T r = 0;
+    Kokkos::parallel_reduce(P(0, std::min(n, 20)), ProdF<T>{d}, Kokkos::Prod<T>(r));
+    if (std::is_integral<T>::value) CHECK_EQ(r, prod);
+    else CHECK(std::abs(double(r - prod)) <= 1e-12 * std::abs(double(prod)));
+  }
+  {
+    T r = 0;
+    Kokkos::parallel_reduce(P(0, n), MinF<T>{d}, Kokkos::Min<T>(r));
+    CHECK_EQ(r, mn);
+    Kokkos::parallel_reduce(P(0, n), MaxF<T>{d}, Kokkos::Max<T>(r));
+    CHECK_EQ(r, mx);
+  }
+  {
+    typename MinLocF<T>::VT r;
+    Kokkos::parallel_reduce(P(0, n), MinLocF<T>{d}, Kokkos::MinLoc<T, int>(r));
+    CHECK_EQ(r.val, mn);
+    CHECK_EQ(r.loc, mnl);
+    Kokkos::parallel_reduce(P(0, n), MaxLocF<T>{d}, Kokkos::MaxLoc<T, int>(r));
+    CHECK_EQ(r.val, mx);
+    CHECK_EQ(r.loc, mxl);
+  }
+  {
+    typename MinMaxF<T>::VT r;
+    Kokkos::parallel_reduce(P(0, n), MinMaxF<T>{d}, Kokkos::MinMax<T>(r));
+    CHECK_EQ(r.min_val, mn);
+    CHECK_EQ(r.max_val, mx);
+    typename MinMaxLocF<T>::VT rl;
+    Kokkos::parallel_reduce(P(0, n), MinMaxLocF<T>{d}, Kokkos::MinMaxLoc<T, int>(rl));
+    CHECK_EQ(rl.min_val, mn);
+    CHECK_EQ(rl.max_val, mx);
+    CHECK_EQ(rl.min_loc, mnl);
+    CHECK_EQ(rl.max_loc, mxl);
+  }
+  // Empty range: every reducer returns its identity.
+  {
+    T r = 7;
+    Kokkos::parallel_reduce(P(0, 0), MinF<T>{d}, Kokkos::Min<T>(r));
+    CHECK_EQ(r, Kokkos::reduction_identity<T>::min());
+    Kokkos::parallel_reduce(P(0, 0), MaxF<T>{d}, Kokkos::Max<T>(r));
+    CHECK_EQ(r, Kokkos::reduction_identity<T>::max());
+    Kokkos::parallel_reduce(P(0, 0), ProdF<T>{d}, Kokkos::Prod<T>(r));
+    CHECK_EQ(r, T(1));
+  }
+}
+
+KKMX_TEST(reducers_int) {
+  std::vector<int> h(100003);
+  uint64_t s = 12345;
+  for (auto& x : h) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    x = int((s >> 33) % 100001) - 50000;
+  }
+  // ties for min and max at several places: the first index wins
+  h[5] = -60000; h[777] = -60000; h[90000] = -60000;
+  h[11] = 60000; h[12] = 60000; h[99999] = 60000;
+  for (int i = 0; i < 20; ++i) h[i] = (i % 3) + 1;
+  h[5] = -60000; h[11] = 60000; h[12] = 60000;
+  reducers_for<int>(h);
+}
+KKMX_TEST(reducers_int64) {
+  std::vector<int64_t> h(300007);
+  uint64_t s = 777;
+  for (auto& x : h) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    x = int64_t(s >> 20) - (int64_t(1) << 43);
+  }
+  for (int i = 0; i < 20; ++i) h[i] = (i % 2) ? 3 : -2;
+  h[1000] = std::numeric_limits<int64_t>::min() / 4;
+  h[200000] = std::numeric_limits<int64_t>::min() / 4;
+  h[3000] = std::numeric_limits<int64_t>::max() / 4;
+  reducers_for<int64_t>(h);
+}
+KKMX_TEST(reducers_double) {
+  std::vector<double> h(65537);
+  uint64_t s = 99;
+  for (auto& x : h) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    x = double(s >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+  }
+  for (int i = 0; i < 20; ++i) h[i] = 0.5 + 0.25 * (i % 4);
+  h[4096] = -3.0; h[60000] = -3.0;
+  h[7] = 3.0; h[65536] = 3.0;
+  reducers_for<double>(h);
+}
+KKMX_TEST(reducers_logical_bitwise) {
+  const int n = 50000;
+  std::vector<int> h(n, 1);
+  auto d = upload(h);
+  using P = Kokkos::RangePolicy<Exec>;
+  int r = 0;
+  Kokkos::parallel_reduce(P(0, n), LAndF<int>{d}, Kokkos::LAnd<int>(r));
+  CHECK_EQ(r, 1);
+  h[31337] = 0;
+  d = upload(h);
+  Kokkos::parallel_reduce(P(0, n), LAndF<int>{d}, Kokkos::LAnd<int>(r));
+  CHECK_EQ(r, 0);
+  Kokkos::parallel_reduce(P(0, n), LOrF<int>{d}, Kokkos::LOr<int>(r));
+  CHECK_EQ(r, 1);
+  std::vector<int> z(n, 0);
+  z[49999] = 1;
+  auto dz = upload(z);
+  Kokkos::parallel_reduce(P(0, n - 1), LOrF<int>{dz}, Kokkos::LOr<int>(r));
+  CHECK_EQ(r, 0);
+  Kokkos::parallel_reduce(P(0, n), LOrF<int>{dz}, Kokkos::LOr<int>(r));
+  CHECK_EQ(r, 1);
+  std::vector<unsigned> b(n);
+  unsigned band = ~0u, bor = 0;
+  for (int i = 0; i < n; ++i) {
+    b[i] = ~(1u << (i % 29)) & (0xF0F0F0F0u | (1u << (i % 31)));
+    band &= b[i];
+    bor |= b[i];
+  }
+  auto db = upload(b);
+  unsigned rb = 0;
+  Kokkos::parallel_reduce(P(0, n), BAndF<unsigned>{db}, Kokkos::BAnd<unsigned>(rb));
+  CHECK_EQ(rb, band);
+  Kokkos::parallel_reduce(P(0, n), BOrF<unsigned>{db}, Kokkos::BOr<unsigned>(rb));
+  CHECK_EQ(rb, bor);
+}
+
+// =========================================================================
+// Scan (core/unit_test/TestParallelScanRangePolicy.hpp:24-270)
+// =========================================================================
+template <class ValueType>
+struct ScanRangeFunctor {
+  using execution_space = Exec;
+  using ViewType        = Kokkos::View<ValueType*, Exec>;
+  ViewType prefix_results, postfix_results;
+  KOKKOS_INLINE_FUNCTION void operator()(const size_t i, ValueType& update, bool final_pass) const {
+    if (final_pass) prefix_results(i) = update;
+    update += i;
+    if (final_pass) postfix_results(i) = update;
+  }
+  KOKKOS_INLINE_FUNCTION void init(ValueType& update) const { update = 0; }
+  KOKKOS_INLINE_FUNCTION void join(ValueType& update, const ValueType& input) const { update += input; }
+
+  void check(size_t work_size) const {
+    auto pre  = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), prefix_results);
+    auto post = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), postfix_results);
+    for (size_t i = 0; i < work_size; ++i) {
+      CHECK_EQ(pre(i), static_cast<ValueType>((static_cast<ValueType>(i) * (i - 1)) / 2));
+      CHECK_EQ(post(i), static_cast<ValueType>((static_cast<ValueType>(i) * (i + 1)) / 2));
+    }
+    Kokkos::deep_copy(prefix_results, 0);
+    Kokkos::deep_copy(postfix_results, 0);
+  }
+  ValueType total(size_t work_size) const {
+    return static_cast<ValueType>(static_cast<ValueType>(work_size) * (work_size - 1) / 2);
+  }
+
+  template <class... Args>
+  void test_scan(const size_t work_size) {
+    prefix_results  = ViewType("prefix_results", work_size);
+    postfix_results = ViewType("postfix_results", work_size);
+    if constexpr (sizeof...(Args) == 0) {
+      Kokkos::parallel_scan("TestWithStrArg1", work_size, *this);
+      check(work_size);
+      Kokkos::parallel_scan(work_size, *this);
+      check(work_size);
+      {
+        ValueType rv = 0;
+        Kokkos::parallel_scan("TestWithStrArg2", work_size, *this, rv);
+        check(work_size);
+        CHECK_EQ(rv, total(work_size));
+      }
+      {
+        ValueType rv = 0;
+        Kokkos::parallel_scan(work_size, *this, rv);
+        check(work_size);
+        CHECK_EQ(rv, total(work_size));
+      }
+      {
+        Kokkos::View<ValueType, Kokkos::HostSpace> rview("return_view");
+        Kokkos::parallel_scan(work_size, *this, rview);
+        check(work_size);
+        CHECK_EQ(rview(), total(work_size));
+      }
+    } else {
+      Kokkos::RangePolicy<Exec, Args...> policy(0, work_size);
+      Kokkos::parallel_scan("TestWithStrArg3", policy, *this);
+      check(work_size);
+      Kokkos::parallel_scan(policy, *this);
+      check(work_size);
+      {
+        ValueType rv = 0;
+        Kokkos::parallel_scan("TestWithStrArg4", policy, *this, rv);
+        check(work_size);
+        CHECK_EQ(rv, total(work_size));
+      }
+      {
+        Kokkos::View<ValueType, Exec> rview("return_view");
+        Kokkos::parallel_scan(policy, *this, rview);
+        check(work_size);
+        ValueType t;
+        Kokkos::deep_copy(t, rview);
+        CHECK_EQ(t, total(work_size));
+      }
+      {
+        const auto pr = Kokkos::Experimental::require(policy, Kokkos::Experimental::WorkItemProperty::HintLightWeight);
+        ValueType rv = 0;
+        Kokkos::parallel_scan(pr, *this, rv);
+        check(work_size);
+        CHECK_EQ(rv, total(work_size));
+      }
+    }
+  }
+  template <class... Args>
+  void test_scan(const std::vector<size_t>& sizes) {
+    for (size_t s : sizes) test_scan<Args...>(s);
+  }
+  void all(const std::vector<size_t>& sizes) {
+    test_scan<>(sizes);
+    test_scan<Kokkos::Schedule<Kokkos::Static>>(sizes);
+    test_scan<Kokkos::Schedule<Kokkos::Dynamic>>(sizes);
+  }
+};
+KKMX_TEST(scan_range_policy_char) { ScanRangeFunctor<char>{}.all({5, 10}); }
+KKMX_TEST(scan_range_policy_short) { ScanRangeFunctor<short>{}.all({50, 100}); }
+KKMX_TEST(scan_range_policy_int) { ScanRangeFunctor<int>{}.all({0, 1, 2, 1000, 1001}); }
+KKMX_TEST(scan_range_policy_long) { ScanRangeFunctor<long>{}.all({1000, 10000}); }
+KKMX_TEST(scan_range_policy_float) { ScanRangeFunctor<float>{}.all({13, 34}); }
+KKMX_TEST(scan_range_policy_double) { ScanRangeFunctor<double>{}.all({17, 59}); }
+KKMX_TEST(scan_range_policy_large) { ScanRangeFunctor<int64_t>{}.all({(size_t(1) << 22) + 12345, 10000019}); }
+
+// A scan whose value is a struct (generic tile state) and a scan with a tag
+// and a non-zero begin.
+struct Pair64 {
+  int64_t a, b;
+  KOKKOS_INLINE_FUNCTION Pair64& operator+=(const Pair64& o) {
+    a += o.a;
+    b += o.b;
+    return *this;
+  }
+};
+struct ScanTag {};
+struct StructScan {
+  Kokkos::View<int64_t*, Exec> outa, outb;
+  int64_t begin;
+  KOKKOS_INLINE_FUNCTION void operator()(ScanTag, const int64_t i, Pair64& u, const bool final) const {
+    u.a += 1;
+    u.b += i;
+    if (final) {
+      outa(i - begin) = u.a;
+      outb(i - begin) = u.b;
+    }
+  }
+};
+KKMX_TEST(scan_struct_value_with_tag_and_offset) {
+  for (int64_t n : {int64_t(1), int64_t(777), int64_t(1000003)}) {
+    const int64_t b = 1000;
+    Kokkos::View<int64_t*, Exec> a("a", n), c("c", n);
+    Pair64 total{0, 0};
+    Kokkos::parallel_scan(Kokkos::RangePolicy<Exec, ScanTag>(b, b + n), StructScan{a, c, b}, total);
+    auto ha = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), a);
+    auto hc = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), c);
+    for (int64_t k = 0; k < n; ++k) {
+      const int64_t i = b + k;
+      if (ha(k) != k + 1) CHECK_EQ(ha(k), k + 1);
+      const int64_t want = (i * (i + 1)) / 2 - (b * (b - 1)) / 2;
+      if (hc(k) != want) CHECK_EQ(hc(k), want);
+    }
+    CHECK_EQ(total.a, n);
+    CHECK_EQ(total.b, ((b + n - 1) * (b + n)) / 2 - (b * (b - 1)) / 2);
+  }
+}
+
+// =========================================================================
+// MDRange (core/unit_test/TestMDRange.hpp): every point visited once, for and
+// reduce, ranks 2..6, both iteration orders, explicit tiles, negative begins.
+// =========================================================================
+template <int R>
+struct MDLinear {
+  int64_t lo[6], ext[6];
+  KOKKOS_INLINE_FUNCTION int64_t lin(const int64_t* idx) const {
+    int64_t l = 0;
+    for (int r = 0; r < R; ++r) l = l * ext[r] + (idx[r] - lo[r]);
+    return l;
+  }
+};
+template <int R>
+struct MDCount : MDLinear<R> {
+  Kokkos::View<int*, Exec> hits;
+  template <class... I>
+  KOKKOS_INLINE_FUNCTION void operator()(const I... i) const {
+    const int64_t idx[] = {int64_t(i)...};
+    Kokkos::atomic_add(&hits(this->lin(idx)), 1);
+  }
+};
+// Sum over the box of (linear index + 1) = P(P+1)/2 for P points.
+template <int R>
+struct MDLinSum : MDLinear<R> {
+  using value_type = int64_t;
+  KOKKOS_INLINE_FUNCTION void add(const int64_t* x, int64_t& u) const { u += this->lin(x) + 1; }
+  KOKKOS_INLINE_FUNCTION void operator()(int64_t a, int64_t b, int64_t& u) const {
+    const int64_t x[] = {a, b};
+    add(x, u);
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(int64_t a, int64_t b, int64_t c, int64_t& u) const {
+    const int64_t x[] = {a, b, c};
+    add(x, u);
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(int64_t a, int64_t b, int64_t c, int64_t d, int64_t& u) const {
+    const int64_t x[] = {a, b, c, d};
+    add(x, u);
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(int64_t a, int64_t b, int64_t c, int64_t d, int64_t e, int64_t& u) const {
+    const int64_t x[] = {a, b, c, d, e};
+    add(x, u);
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(int64_t a, int64_t b, int64_t c, int64_t d, int64_t e, int64_t f,
+                                         int64_t& u) const {
+    const int64_t x[] = {a, b, c, d, e, f};
+    add(x, u);
+  }
+};
+
+template <int R, Kokkos::Iterate It>
+void md_case(const std::vector<int64_t>& lo, const std::vector<int64_t>& hi, const std::vector<int64_t>& tile) {
+  using Pol = Kokkos::MDRangePolicy<Exec, Kokkos::Rank<R, It, It>, Kokkos::IndexType<int64_t>>;
+  typename Pol::point_type b, e;
+  typename Pol::tile_type t;
+  MDCount<R> f;
+  MDLinSum<R> fs;
+  int64_t total = 1;
+  for (int r = 0; r < R; ++r) {
+    b[r]      = lo[r];
+    e[r]      = hi[r];
+    t[r]      = tile.empty() ? 0 : tile[r];
+    f.lo[r]   = fs.lo[r] = lo[r];
+    f.ext[r]  = fs.ext[r] = hi[r] - lo[r];
+    total *= f.ext[r];
+  }
+  f.hits = Kokkos::View<int*, Exec>("hits", total > 0 ? total : 1);
+  Kokkos::parallel_for(Pol(b, e, t), f);
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), f.hits);
+  for (int64_t k = 0; k < total; ++k)
+    if (h(k) != 1) CHECK_EQ(h(k), 1);
+  int64_t sum = -1;
+  Kokkos::parallel_reduce(Pol(b, e, t), fs, sum);
+  CHECK_EQ(sum, total * (total + 1) / 2);
+  Kokkos::View<int64_t, Exec> dsum("dsum");
+  Kokkos::parallel_reduce(Pol(b, e, t), fs, dsum);
+  int64_t hs;
+  Kokkos::deep_copy(hs, dsum);
+  CHECK_EQ(hs, total * (total + 1) / 2);
+}
+template <int R>
+void md_both(const std::vector<int64_t>& lo, const std::vector<int64_t>& hi, const std::vector<int64_t>& tile) {
+  md_case<R, Kokkos::Iterate::Left>(lo, hi, tile);
+  md_case<R, Kokkos::Iterate::Right>(lo, hi, tile);
+}
+KKMX_TEST(mdrange_2d) {
+  md_both<2>({0, 0}, {100, 100}, {});
+  md_both<2>({0, 0}, {1000, 3}, {});
+  md_both<2>({3, 7}, {517, 130}, {32, 8});
+  md_both<2>({0, 0}, {33, 1025}, {1, 256});
+  md_both<2>({0, 0}, {0, 10}, {});
+}
+KKMX_TEST(mdrange_3d) {
+  md_both<3>({0, 0, 0}, {64, 64, 64}, {});
+  md_both<3>({1, 2, 3}, {101, 52, 33}, {4, 4, 4});
+  md_both<3>({0, 0, 0}, {130, 7, 9}, {64, 2, 2});
+}
+KKMX_TEST(mdrange_neg_idx) {
+  md_both<2>({-1, -1}, {12, 7}, {});
+  md_both<3>({-5, -2, -3}, {20, 9, 4}, {8, 2, 2});
+}
+KKMX_TEST(mdrange_4d) {
+  md_both<4>({0, 0, 0, 0}, {20, 10, 9, 7}, {});
+  md_both<4>({2, 1, 0, 3}, {19, 14, 6, 9}, {4, 4, 2, 2});
+}
+KKMX_TEST(mdrange_5d) {
+  md_both<5>({0, 0, 0, 0, 0}, {8, 7, 6, 5, 9}, {});
+  md_both<5>({1, 0, 2, 0, 1}, {9, 6, 7, 4, 5}, {2, 2, 2, 2, 2});
+}
+KKMX_TEST(mdrange_6d) {
+  md_both<6>({0, 0, 0, 0, 0, 0}, {6, 5, 4, 7, 3, 5}, {});
+  md_both<6>({1, 1, 0, 0, 2, 0}, {5, 6, 4, 3, 5, 4}, {2, 2, 2, 1, 2, 2});
+}
+
+// mdrange_array_reduce (TestMDRange_c.hpp:30): a runtime-array value over an
+// MDRange is not supported here; the scalar and struct forms are.
+struct MDScalarArray {
+  struct value_type {
+    int64_t v[4];
+  };
+  KOKKOS_INLINE_FUNCTION void init(value_type& u) const {
+    for (int k = 0; k < 4; ++k) u.v[k] = 0;
+  }
+  KOKKOS_INLINE_FUNCTION void join(value_type& a, const value_type& b) const {
+    for (int k = 0; k < 4; ++k) a.v[k] += b.v[k];
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, const int j, value_type& u) const {
+    u.v[0] += 1;
+    u.v[1] += i;
+    u.v[2] += j;
+    u.v[3] += int64_t(i) * j;
+  }
+};
+KKMX_TEST(mdrange_struct_reduce) {
+  const int N0 = 513, N1 = 77;
+  MDScalarArray::value_type r;
+  Kokkos::parallel_reduce(Kokkos::MDRangePolicy<Exec, Kokkos::Rank<2>>({0, 0}, {N0, N1}), MDScalarArray(), r);
+  const int64_t s0 = int64_t(N0) * (N0 - 1) / 2, s1 = int64_t(N1) * (N1 - 1) / 2;
+  CHECK_EQ(r.v[0], int64_t(N0) * N1);
+  CHECK_EQ(r.v[1], s0 * N1);
+  CHECK_EQ(r.v[2], s1 * N0);
+  CHECK_EQ(r.v[3], s0 * s1);
+}
+
+// =========================================================================
+// Team (core/unit_test/TestTeamBasic.hpp, TestTeamScan.hpp, TestTeamVector.hpp)
+// =========================================================================
+using TeamPol = Kokkos::TeamPolicy<Exec>;
+using Member  = TeamPol::member_type;
+
+// team_for: every (league_rank, team_rank) runs once.
+struct TeamForF {
+  Kokkos::View<int**, Exec> hits;
+  KOKKOS_INLINE_FUNCTION void operator()(const Member& m) const {
+    if (m.league_size() != int(hits.extent(0)) || m.team_size() != int(hits.extent(1))) {
+      Kokkos::atomic_add(&hits(0, 0), 1000);
+      return;
+    }
+    Kokkos::atomic_add(&hits(m.league_rank(), m.team_rank()), 1);
+  }
+};
+KKMX_TEST(team_for) {
+  for (int league : {1, 7, 1000, 10007})
+    for (int ts : {1, 3, 64, 127, 256, 1024}) {
+      Kokkos::View<int**, Exec> hits("hits", league, ts);
+      Kokkos::parallel_for(TeamPol(league, ts), TeamForF{hits});
+      auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), hits);
+      for (int l = 0; l < league; ++l)
+        for (int t = 0; t < ts; ++t)
+          if (h(l, t) != 1) CHECK_EQ(h(l, t), 1);
+    }
+}
+// team_reduce: sum of (league_rank * team_size + team_rank + 1) over all.
+struct TeamReduceF {
+  using value_type = int64_t;
+  KOKKOS_INLINE_FUNCTION void operator()(const Member& m, int64_t& u) const {
+    u += int64_t(m.league_rank()) * m.team_size() + m.team_rank() + 1;
+  }
+};
+KKMX_TEST(team_reduce) {
+  for (int league : {0, 1, 13, 4096})
+    for (int ts : {1, 64, 100, 512}) {
+      int64_t r = -5;
+      Kokkos::parallel_reduce(TeamPol(league, ts), TeamReduceF(), r);
+      const int64_t p = int64_t(league) * ts;
+      CHECK_EQ(r, p * (p + 1) / 2);
+    }
+}
+// Nested TeamThreadRange reduce + team_broadcast + single.
+struct TeamNestedF {
+  Kokkos::View<int64_t*, Exec> out;
+  int n;
+  KOKKOS_INLINE_FUNCTION void operator()(const Member& m) const {
+    const int lr = m.league_rank();
+    int64_t s    = 0;
+    Kokkos::parallel_reduce(
+        Kokkos::TeamThreadRange(m, n), [&](const int i, int64_t& u) { u += int64_t(lr) * n + i; }, s);
+    int64_t mx = 0;
+    Kokkos::parallel_reduce(
+        Kokkos::TeamThreadRange(m, n), [&](const int i, int64_t& u) { if (i * 7 % n > u) u = i * 7 % n; },
+        Kokkos::Max<int64_t>(mx));
+    int64_t b = (m.team_rank() == m.team_size() - 1) ? s + lr : -1;
+    m.team_broadcast(b, m.team_size() - 1);
+    Kokkos::single(Kokkos::PerTeam(m), [&]() { out(lr) = s * 3 + (b - s - lr) + mx * 0; });
+    if (b != s + lr || s != int64_t(lr) * n * n + int64_t(n) * (n - 1) / 2) Kokkos::atomic_add(&out(lr), int64_t(1) << 50);
+  }
+};
+KKMX_TEST(team_nested_reduce_broadcast_single) {
+  for (int ts : {32, 64, 96, 256})
+    for (int n : {1, 5, 64, 1000}) {
+      const int league = 301;
+      Kokkos::View<int64_t*, Exec> out("out", league);
+      Kokkos::parallel_for(TeamPol(league, ts), TeamNestedF{out, n});
+      auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+      for (int l = 0; l < league; ++l) {
+        const int64_t s = int64_t(l) * n * n + int64_t(n) * (n - 1) / 2;
+        if (h(l) != 3 * s) CHECK_EQ(h(l), 3 * s);
+      }
+    }
+}
+// team_scan (TestTeamScan.hpp:24-114).
+template <class DataType>
+struct TeamScanF {
+  Kokkos::View<DataType**, Exec> a_d, a_r;
+  int32_t N;
+  KOKKOS_FUNCTION void operator()(const Member& team) const {
+    const int lr = team.league_rank();
+    Kokkos::parallel_for(Kokkos::TeamThreadRange(team, 0, N), [&](const int i) { a_d(lr, i) = lr * N + i; });
+    Kokkos::parallel_scan(Kokkos::TeamThreadRange(team, 0, N), [&](int i, DataType& val, const bool final) {
+      val += a_d(lr, i);
+      if (final) a_r(lr, i) = val;
+    });
+  }
+};
+template <class DataType>
+void team_scan_case(int M, int N) {
+  TeamScanF<DataType> f{Kokkos::View<DataType**, Exec>("a_d", M, N), Kokkos::View<DataType**, Exec>("a_r", M, N), N};
+  Kokkos::parallel_for(TeamPol(M, 127), f);
+  auto ai = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), f.a_d);
+  auto ao = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), f.a_r);
+  for (int i = 0; i < M; ++i) {
+    DataType ref = 0;
+    for (int j = 0; j < N; ++j) {
+      ref += ai(i, j);
+      if (ao(i, j) != ref) CHECK_EQ(ao(i, j), ref);
+    }
+  }
+}
+KKMX_TEST(team_scan) {
+  for (int M : {0, 1, 10, 100})
+    for (int N : {0, 1, 2, 127, 128, 1000}) {
+      team_scan_case<int32_t>(M, N);
+      team_scan_case<int64_t>(M, N);
+    }
+}
+// ThreadVectorRange reduce and scan inside TeamThreadRange.
+struct TeamVectorF {
+  Kokkos::View<int64_t**, Exec> out;
+  int n;
+  KOKKOS_INLINE_FUNCTION void operator()(const Member& m) const {
+    const int lr = m.league_rank();
+    Kokkos::parallel_for(Kokkos::TeamThreadRange(m, n), [&](const int i) {
+      int64_t s = 0;
+      Kokkos::parallel_reduce(
+          Kokkos::ThreadVectorRange(m, i + 1), [&](const int k, int64_t& u) { u += k + lr; }, s);
+      // The scan total comes back to every vector lane; a variable written
+      // by the lane that owns k == i would only be visible in that lane.
+      int64_t last = 0;
+      Kokkos::parallel_scan(
+          Kokkos::ThreadVectorRange(m, i + 1), [&](const int k, int64_t& u, const bool) { u += k; }, last);
+      Kokkos::single(Kokkos::PerThread(m), [&]() { out(lr, i) = s * 1000003 + last; });
+    });
+  }
+};
+KKMX_TEST(team_vector_reduce_scan) {
+  for (int vl : {1, 4, 16, 64})
+    for (int ts : {1, 4, 16}) {
+      if (ts * vl > 1024) continue;
+      const int league = 37, n = 70;
+      Kokkos::View<int64_t**, Exec> out("out", league, n);
+      Kokkos::parallel_for(TeamPol(league, ts, vl), TeamVectorF{out, n});
+      auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+      for (int l = 0; l < league; ++l)
+        for (int i = 0; i < n; ++i) {
+          const int64_t tri = int64_t(i) * (i + 1) / 2;
+          const int64_t want = (tri + int64_t(i + 1) * l) * 1000003 + tri;
+          if (h(l, i) != want) CHECK_EQ(h(l, i), want);
+        }
+    }
+}
+// Team scratch: PerTeam and PerThread level-0 buffers (TestTeamBasic.hpp:182-215).
+struct TeamScratchF {
+  using shmem = Kokkos::View<int64_t*, Exec::scratch_memory_space, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+  Kokkos::View<int64_t*, Exec> out;
+  int team_n, thread_n;
+  KOKKOS_INLINE_FUNCTION void operator()(const Member& m) const {
+    shmem team_buf(m.team_scratch(0), team_n);
+    shmem mine(m.thread_scratch(0), thread_n);
+    Kokkos::parallel_for(Kokkos::TeamThreadRange(m, team_n), [&](const int i) { team_buf(i) = i + m.league_rank(); });
+    for (int k = 0; k < thread_n; ++k) mine(k) = m.team_rank() * 10 + k;
+    m.team_barrier();
+    int64_t s = 0;
+    Kokkos::parallel_reduce(
+        Kokkos::TeamThreadRange(m, team_n), [&](const int i, int64_t& u) { u += team_buf(team_n - 1 - i); }, s);
+    int64_t t = 0;
+    for (int k = 0; k < thread_n; ++k) t += mine(k) - m.team_rank() * 10;
+    if (m.team_rank() == 0) out(m.league_rank()) = s;
+    if (t != int64_t(thread_n) * (thread_n - 1) / 2) Kokkos::atomic_add(&out(m.league_rank()), int64_t(1) << 40);
+  }
+};
+KKMX_TEST(team_scratch_per_team_per_thread) {
+  for (int team_n : {1, 100, 4000})
+    for (int thread_n : {1, 8, 32}) {
+      const int league = 513, ts = 64;
+      Kokkos::View<int64_t*, Exec> out("out", league);
+      const size_t bytes_team   = TeamScratchF::shmem::shmem_size(team_n);
+      const size_t bytes_thread = TeamScratchF::shmem::shmem_size(thread_n);
+      Kokkos::parallel_for(
+          TeamPol(league, ts).set_scratch_size(0, Kokkos::PerTeam(bytes_team), Kokkos::PerThread(bytes_thread)),
+          TeamScratchF{out, team_n, thread_n});
+      auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+      for (int l = 0; l < league; ++l) {
+        const int64_t want = int64_t(team_n) * (team_n - 1) / 2 + int64_t(team_n) * l;
+        if (h(l) != want) CHECK_EQ(h(l), want);
+      }
+    }
+}
+
+// =========================================================================
+// Atomics (core/unit_test/TestAtomics.hpp:26-455): add, CAS and exchange loops
+// against the serial answer; min/max; sub-word and wide types.
+// =========================================================================
+template <class T>
+struct AddLoop {
+  Kokkos::View<T, Exec> data;
+  KOKKOS_INLINE_FUNCTION void operator()(const int) const { Kokkos::atomic_fetch_add(&data(), (T)1); }
+};
+template <class T>
+struct CASLoop {
+  Kokkos::View<T, Exec> data;
+  KOKKOS_INLINE_FUNCTION void operator()(const int) const {
+    T* p = &data();
+    T old = *p, assumed;
+    do {
+      assumed = old;
+      old     = Kokkos::atomic_compare_exchange(p, assumed, T(assumed + (T)1));
+    } while (old != assumed);
+  }
+};
+template <class T>
+struct ExchLoop {
+  Kokkos::View<T, Exec> data, data2;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
+    T old = Kokkos::atomic_exchange(&data(), (T)i);
+    Kokkos::atomic_fetch_add(&data2(), old);
+  }
+};
+template <class T>
+void atomic_loops(int loop) {
+  Kokkos::View<T, Exec> d("d"), d2("d2");
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, loop), AddLoop<T>{d});
+  T r;
+  Kokkos::deep_copy(r, d);
+  CHECK_EQ(r, (T)loop);
+  Kokkos::deep_copy(d, T(0));
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, loop), CASLoop<T>{d});
+  Kokkos::deep_copy(r, d);
+  CHECK_EQ(r, (T)loop);
+  // exchange: the values 0..loop-1 each leave once, either through an
+  // exchange's old value (summed into d2) or as the final content of d.
+  Kokkos::deep_copy(d, T(0));
+  Kokkos::deep_copy(d2, T(0));
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, loop), ExchLoop<T>{d, d2});
+  T r2;
+  Kokkos::deep_copy(r, d);
+  Kokkos::deep_copy(r2, d2);
+  CHECK_EQ(T(r + r2), T((T)loop * (T)(loop - 1) / 2));
+}
+KKMX_TEST(atomics_loops) {
+  const int loop = 10000;
+  atomic_loops<int>(loop);
+  atomic_loops<unsigned int>(loop);
+  atomic_loops<long>(loop);
+  atomic_loops<unsigned long long>(loop);
+  atomic_loops<float>(1000);
+  atomic_loops<double>(loop);
+}
+
+template <class T>
+struct MinMaxOps {
+  Kokkos::View<T*, Exec> v;  // 0: min, 1: max, 2: fetch_add sum, 3: sub
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
+    const T x = T((i * 7919) % 10007) - T(5000);
+    Kokkos::atomic_min(&v(0), x);
+    Kokkos::atomic_max(&v(1), x);
+    Kokkos::atomic_add(&v(2), x);
+    Kokkos::atomic_sub(&v(3), x);
+  }
+};
+template <class T>
+void atomic_minmax_case(int n) {
+  Kokkos::View<T*, Exec> v("v", 4);
+  T init[4] = {std::numeric_limits<T>::max(), std::numeric_limits<T>::lowest(), T(0), T(0)};
+  Kokkos::View<T*, Kokkos::HostSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>> hi(init, 4);
+  Kokkos::deep_copy(v, hi);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), MinMaxOps<T>{v});
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), v);
+  T mn = std::numeric_limits<T>::max(), mx = std::numeric_limits<T>::lowest(), sm = 0;
+  for (int i = 0; i < n; ++i) {
+    const T x = T((i * 7919) % 10007) - T(5000);
+    mn = x < mn ? x : mn;
+    mx = x > mx ? x : mx;
+    sm += x;
+  }
+  CHECK_EQ(h(0), mn);
+  CHECK_EQ(h(1), mx);
+  CHECK_EQ(h(2), sm);
+  CHECK_EQ(h(3), T(-sm));
+}
+KKMX_TEST(atomics_min_max_add_sub) {
+  atomic_minmax_case<int>(100000);
+  atomic_minmax_case<long long>(100000);
+  atomic_minmax_case<double>(100000);  // integer-valued: exact in any order
+  atomic_minmax_case<float>(1000);
+}
+
+// Sub-word (char, short) and wide (16-byte) types.
+struct Wide {
+  double a, b;
+  KOKKOS_INLINE_FUNCTION Wide operator+(const Wide& o) const { return Wide{a + o.a, b + o.b}; }
+  KOKKOS_INLINE_FUNCTION bool operator==(const Wide& o) const { return a == o.a && b == o.b; }
+};
+struct SubWordWideF {
+  Kokkos::View<int8_t*, Exec> c;
+  Kokkos::View<int16_t*, Exec> s;
+  Kokkos::View<Wide*, Exec> w;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
+    Kokkos::atomic_add(&c(i % 4), int8_t(1));
+    Kokkos::atomic_add(&s(i % 3), int16_t(2));
+    Kokkos::atomic_add(&w(i % 5), Wide{1.0, double(i)});
+  }
+};
+KKMX_TEST(atomics_subword_and_wide) {
+  const int n = 100000;
+  Kokkos::View<int8_t*, Exec> c("c", 4);
+  Kokkos::View<int16_t*, Exec> s("s", 3);
+  Kokkos::View<Wide*, Exec> w("w", 5);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), SubWordWideF{c, s, w});
+  auto hc = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), c);
+  auto hs = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), s);
+  auto hw = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), w);
+  for (int k = 0; k < 4; ++k) {
+    int cnt = 0;
+    for (int i = k; i < n; i += 4) ++cnt;
+    CHECK_EQ(int(hc(k)), int(int8_t(cnt)));
+  }
+  for (int k = 0; k < 3; ++k) {
+    int cnt = 0;
+    for (int i = k; i < n; i += 3) ++cnt;
+    CHECK_EQ(int(hs(k)), int(int16_t(2 * cnt)));
+  }
+  for (int k = 0; k < 5; ++k) {
+    double cnt = 0, sum = 0;
+    for (int i = k; i < n; i += 5) {
+      cnt += 1;
+      sum += i;
+    }
+    CHECK_EQ(hw(k).a, cnt);
+    CHECK_EQ(hw(k).b, sum);
+  }
+}
+
+// MemoryTraits<Atomic> Views (core/unit_test/TestAtomicViews.hpp).
+struct AtomicViewF {
+  Kokkos::View<int64_t*, Exec, Kokkos::MemoryTraits<Kokkos::Atomic>> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
+    v(i % 17) += i;
+    v(17 + i % 3)++;
+  }
+};
+KKMX_TEST(atomic_views) {
+  const int n = 200000;
+  Kokkos::View<int64_t*, Exec, Kokkos::MemoryTraits<Kokkos::Atomic>> v("v", 20);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), AtomicViewF{v});
+  Kokkos::View<int64_t*, Exec> plain = v;
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), plain);
+  for (int k = 0; k < 17; ++k) {
+    int64_t want = 0;
+    for (int i = k; i < n; i += 17) want += i;
+    CHECK_EQ(h(k), want);
+  }
+  for (int k = 0; k < 3; ++k) {
+    int64_t want = 0;
+    for (int i = k; i < n; i += 3) ++want;
+    CHECK_EQ(h(17 + k), want);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C-ABI
+extern "C" {
+
+int kkmx_selftest_count(void) { return static_cast<int>(registry().size()); }
+
+const char* kkmx_selftest_name(int i) {
+  if (i < 0 || i >= static_cast<int>(registry().size())) return nullptr;
+  return registry()[i].name;
+}
+
+// Runs test i.  Returns 0 on pass, 1 on a failed check, 2 on an exception
+// from the runtime, -1 for a bad index; msg receives the failure text.
+int kkmx_selftest_run(int i, char* msg, int msg_len) {
+  if (i < 0 || i >= static_cast<int>(registry().size())) return -1;
+  auto put = [&](const std::string& s) {
+    if (msg && msg_len > 0) {
+      std::strncpy(msg, s.c_str(), msg_len - 1);
+      msg[msg_len - 1] = 0;
+    }
+  };
+  put("");
+  try {
+    if (!Kokkos::is_initialized()) Kokkos::initialize();
+    registry()[i].fn();
+    Kokkos::fence();
+    return 0;
+  } catch (const Failure& f) {
+    put(f.what());
+    return 1;
+  } catch (const std::exception& e) {
+    put(std::string("exception: ") + e.what());
+    return 2;
+  }
+}
+
+}  // extern "C"

@@ -103,6 +103,19 @@ inline std::enable_if_t<!is_view<T>::value> deep_copy(T& dst, const View<S, P...
   Kokkos::fence("Kokkos::deep_copy: copy into scalar, pre copy fence");
   KKMX_SAFE_CALL(mx_memcpy(&dst, src.data(), sizeof(T)));
 }
+// scalar <- rank-0 View, ordered on exec's stream; the value is valid after
+// exec.fence().
+template <class E, class T, class S, class... P,
+          class = std::enable_if_t<is_execution_space<E>::value && !is_view<T>::value>>
+inline void deep_copy(const E& exec, T& dst, const View<S, P...>& src) {
+  static_assert(View<S, P...>::rank == 0, "deep_copy(exec, scalar, View) needs a rank-0 View");
+  if constexpr (std::is_same<E, MI355X>::value) {
+    KKMX_SAFE_CALL(mx_memcpy_async(&dst, src.data(), sizeof(T), exec.stream()));
+  } else {
+    exec.fence();
+    KKMX_SAFE_CALL(mx_memcpy(&dst, src.data(), sizeof(T)));
+  }
+}
 // rank-0 View <- scalar handled by the scalar fill overload above.
 
 // View <- View

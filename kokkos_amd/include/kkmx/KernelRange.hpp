@@ -38,9 +38,15 @@ namespace Kokkos {
 namespace Impl {
 
 // ------------------------------------------------------------------ launch helpers
+// When set, dispatches size and allocate their scratch but launch nothing:
+// graph construction runs every node once this way before stream capture, so
+// no allocation or memset is ever recorded into a graph.
+inline thread_local bool g_size_only_dispatch = false;
+
 template <class... Args>
 inline void mx_launch_kernel(const void* fn, unsigned grid, unsigned block, size_t lds,
                              mx_stream_t stream, Args&... args) {
+  if (g_size_only_dispatch) return;
   void* argv[] = {static_cast<void*>(&args)..., nullptr};
   KKMX_SAFE_CALL(mx_launch(fn, grid, 1, 1, block, 1, 1, lds, argv, stream));
 }
@@ -299,10 +305,11 @@ struct scan_state_kind {
 // whole workgroup: thread j of a window reads the status of tile
 // hi-(T-1)+j, so one round trip covers T predecessors and the ordered block
 // reduction keeps tile order.  The nearest tile that already holds an
-// inclusive prefix ends the walk.  Returns the same value in every thread.
+// inclusive prefix ends the walk; tiles before `first` (another sequence's
+// tiles) count as the identity.  Returns the same value in every thread.
 template <class State, class V, class Join>
 __device__ V scan_lookback(const State& st, int64_t tile, uint32_t epoch, const V& identity,
-                           const Join& join, V* lds_vals, int* lds_pos) {
+                           const Join& join, V* lds_vals, int* lds_pos, int64_t first = 0) {
   const int T         = blockDim.x;
   const int tid       = threadIdx.x;
   const uint32_t want = (epoch & 0x3fffffffu);
@@ -312,7 +319,7 @@ __device__ V scan_lookback(const State& st, int64_t tile, uint32_t epoch, const 
     const int64_t t = hi - (T - 1) + tid;
     V v             = identity;
     uint32_t s      = kScanIncl;
-    if (t >= 0) {
+    if (t >= first) {
       uint32_t f = st.read(t, v);
       int spins  = 0;
       while ((f >> 2) != want || (f & 3) == 0) {

@@ -207,6 +207,152 @@ inline void parallel_reduce(const Policy& policy, const F& functor) {
   parallel_reduce(std::string(), policy, functor);
 }
 
+// ------------------------------------------------------------------ several results
+// parallel_reduce(label, policy, f, r1, r2, ...): f's operator() takes one
+// value reference per result (core/src/impl/Kokkos_CombinedReducer.hpp).
+// Each result keeps its own join: a reducer's own, Sum for a scalar or a
+// View.  The values travel as one struct through the ordinary reduction
+// kernels and are written back to each result after a fence.
+namespace Impl {
+
+template <class... Ts>
+struct CombinedValue;
+template <>
+struct CombinedValue<> {};
+template <class T, class... Ts>
+struct CombinedValue<T, Ts...> {
+  T head;
+  CombinedValue<Ts...> tail;
+};
+template <size_t I, class CV>
+KOKKOS_FORCEINLINE_FUNCTION auto& cv_get(CV& v) {
+  if constexpr (I == 0) return v.head;
+  else return cv_get<I - 1>(v.tail);
+}
+template <size_t I, class CV>
+KOKKOS_FORCEINLINE_FUNCTION const auto& cv_get(const CV& v) {
+  if constexpr (I == 0) return v.head;
+  else return cv_get<I - 1>(v.tail);
+}
+
+template <class R, class = void>
+struct result_reducer {  // host scalar
+  using type = Sum<remove_cvref_t<R>, HostSpace>;
+};
+template <class R>
+struct result_reducer<R, std::enable_if_t<is_view<remove_cvref_t<R>>::value>> {
+  using V    = remove_cvref_t<R>;
+  using type = Sum<typename V::non_const_value_type, typename V::memory_space>;
+};
+template <class R>
+struct result_reducer<R, std::enable_if_t<is_reducer<remove_cvref_t<R>>::value>> {
+  using type = remove_cvref_t<R>;
+};
+
+template <class F, class... Red>
+struct CombinedResultFunctor {
+  using value_type = CombinedValue<typename Red::value_type...>;
+  using Seq        = std::index_sequence_for<Red...>;
+  F f;
+  CombinedValue<Red...> reducers;
+
+  template <size_t... I>
+  KOKKOS_FORCEINLINE_FUNCTION void init_(value_type& v, std::index_sequence<I...>) const {
+    (cv_get<I>(reducers).init(cv_get<I>(v)), ...);
+  }
+  template <size_t... I>
+  KOKKOS_FORCEINLINE_FUNCTION void join_(value_type& d, const value_type& s, std::index_sequence<I...>) const {
+    (cv_get<I>(reducers).join(cv_get<I>(d), cv_get<I>(s)), ...);
+  }
+  template <size_t... I, class... A>
+  KOKKOS_FORCEINLINE_FUNCTION void call_(value_type& v, std::index_sequence<I...>, const A&... a) const {
+    f(a..., cv_get<I>(v)...);
+  }
+  KOKKOS_INLINE_FUNCTION void init(value_type& v) const { init_(v, Seq{}); }
+  KOKKOS_INLINE_FUNCTION void join(value_type& d, const value_type& s) const { join_(d, s, Seq{}); }
+
+  template <class A0>
+  KOKKOS_INLINE_FUNCTION void operator()(const A0& a0, value_type& v) const { call_(v, Seq{}, a0); }
+  template <class A0, class A1>
+  KOKKOS_INLINE_FUNCTION void operator()(const A0& a0, const A1& a1, value_type& v) const {
+    call_(v, Seq{}, a0, a1);
+  }
+  template <class A0, class A1, class A2>
+  KOKKOS_INLINE_FUNCTION void operator()(const A0& a0, const A1& a1, const A2& a2, value_type& v) const {
+    call_(v, Seq{}, a0, a1, a2);
+  }
+  template <class A0, class A1, class A2, class A3>
+  KOKKOS_INLINE_FUNCTION void operator()(const A0& a0, const A1& a1, const A2& a2, const A3& a3,
+                                         value_type& v) const {
+    call_(v, Seq{}, a0, a1, a2, a3);
+  }
+  template <class A0, class A1, class A2, class A3, class A4>
+  KOKKOS_INLINE_FUNCTION void operator()(const A0& a0, const A1& a1, const A2& a2, const A3& a3, const A4& a4,
+                                         value_type& v) const {
+    call_(v, Seq{}, a0, a1, a2, a3, a4);
+  }
+  template <class A0, class A1, class A2, class A3, class A4, class A5>
+  KOKKOS_INLINE_FUNCTION void operator()(const A0& a0, const A1& a1, const A2& a2, const A3& a3, const A4& a4,
+                                         const A5& a5, value_type& v) const {
+    call_(v, Seq{}, a0, a1, a2, a3, a4, a5);
+  }
+  template <class A0, class A1, class A2, class A3, class A4, class A5, class A6>
+  KOKKOS_INLINE_FUNCTION void operator()(const A0& a0, const A1& a1, const A2& a2, const A3& a3, const A4& a4,
+                                         const A5& a5, const A6& a6, value_type& v) const {
+    call_(v, Seq{}, a0, a1, a2, a3, a4, a5, a6);
+  }
+};
+
+template <class Red, class R>
+Red make_result_reducer(R& r) {
+  if constexpr (is_reducer<remove_cvref_t<R>>::value) return r;
+  else return Red(r);
+}
+template <class T, class R>
+void write_back_result(const T& value, R& r) {
+  T* dst;
+  if constexpr (is_reducer<remove_cvref_t<R>>::value) dst = r.view().data();
+  else if constexpr (is_view<remove_cvref_t<R>>::value) dst = const_cast<T*>(r.data());
+  else dst = &r;
+  KKMX_SAFE_CALL(mx_memcpy(dst, &value, sizeof(T)));
+}
+
+inline CombinedValue<> make_combined_reducers() { return {}; }
+template <class R, class... Rest>
+auto make_combined_reducers(R& r, Rest&... rest) {
+  using Red = typename result_reducer<R>::type;
+  return CombinedValue<Red, typename result_reducer<Rest>::type...>{make_result_reducer<Red>(r),
+                                                                     make_combined_reducers(rest...)};
+}
+template <class CV, size_t... I, class... R>
+void write_back_results(const CV& v, std::index_sequence<I...>, R&... rs) {
+  (write_back_result(cv_get<I>(v), rs), ...);
+}
+
+template <class Policy, class F, class... R>
+void combined_reduce(const std::string& label, const Policy& policy, const F& f, R&... rs) {
+  using CF = CombinedResultFunctor<F, typename result_reducer<R>::type...>;
+  CF cf{f, make_combined_reducers(rs...)};
+  typename CF::value_type result;
+  reduce_dispatch(label, policy, cf, result);
+  write_back_results(result, std::index_sequence_for<R...>{}, rs...);
+}
+
+}  // namespace Impl
+
+template <class Policy, class F, class R1, class R2, class... Rs>
+inline std::enable_if_t<!std::is_convertible<Policy, std::string>::value> parallel_reduce(
+    const std::string& label, const Policy& policy_in, const F& functor, R1&& r1, R2&& r2, Rs&&... rs) {
+  auto policy = Impl::to_policy<F>(policy_in);
+  Impl::TrackingDisabledScope nt;
+  Impl::combined_reduce(label, policy, functor, r1, r2, rs...);
+}
+template <class Policy, class F, class R1, class R2, class... Rs,
+          class = std::enable_if_t<is_execution_policy<Policy>::value || std::is_integral<Policy>::value>>
+inline void parallel_reduce(const Policy& policy, const F& functor, R1&& r1, R2&& r2, Rs&&... rs) {
+  parallel_reduce(std::string(), policy, functor, r1, r2, rs...);
+}
+
 // ------------------------------------------------------------------ parallel_scan
 namespace Impl {
 template <class Policy, class F, class Hint>

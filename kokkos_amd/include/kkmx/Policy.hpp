@@ -217,6 +217,8 @@ class RangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
   KOKKOS_INLINE_FUNCTION member_type begin() const { return m_begin; }
   KOKKOS_INLINE_FUNCTION member_type end() const { return m_end; }
   const execution_space& space() const { return m_space; }
+  // Rebind to another instance of the same space (used by graph nodes).
+  void impl_set_space(const execution_space& s) { m_space = s; }
   member_type chunk_size() const { return m_chunk; }
   RangePolicy& set_chunk_size(int c) {
     if (c > 0 && !Impl::is_pow2(static_cast<uint64_t>(c)))
@@ -287,7 +289,7 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
   static_assert(!std::is_void<iteration_pattern>::value, "MDRangePolicy requires a Kokkos::Rank<N>");
   static constexpr unsigned rank = iteration_pattern::rank;
   using index_type        = typename traits::index_type;
-  using array_index_type  = std::make_signed_t<index_type>;
+  using array_index_type  = std::int64_t;  // as the reference, whatever index_type is
   using point_type        = Kokkos::Array<array_index_type, rank>;
   using tile_type         = Kokkos::Array<array_index_type, rank>;
   using execution_policy  = MDRangePolicy<Properties...>;
@@ -320,6 +322,14 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
     for (; r < rank; ++r) m_tile[r] = 0;
     init_helper();
   }
+  // Double-braced bounds: MDRangePolicy({{0, 0}}, {{n, m}}).
+  MDRangePolicy(const point_type& lower, const point_type& upper, const tile_type& tile = tile_type{})
+      : MDRangePolicy(execution_space(), lower, upper, tile) {}
+  MDRangePolicy(const execution_space& space, const point_type& lower, const point_type& upper,
+                const tile_type& tile = tile_type{})
+      : m_space(space), m_lower(lower), m_upper(upper), m_tile(tile) {
+    init_helper();
+  }
   template <class AT, size_t N>
   MDRangePolicy(const Kokkos::Array<AT, N>& lower, const Kokkos::Array<AT, N>& upper,
                 const Kokkos::Array<AT, N>& tile = Kokkos::Array<AT, N>{})
@@ -338,6 +348,7 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
   }
 
   const execution_space& space() const { return m_space; }
+  void impl_set_space(const execution_space& s) { m_space = s; }
   KOKKOS_INLINE_FUNCTION const point_type& lower() const { return m_lower; }
   KOKKOS_INLINE_FUNCTION const point_type& upper() const { return m_upper; }
   KOKKOS_INLINE_FUNCTION const tile_type& tile_size() const { return m_tile; }
@@ -501,6 +512,7 @@ class TeamPolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
   }
 
   const execution_space& space() const { return m_space; }
+  void impl_set_space(const execution_space& s) { m_space = s; }
   int league_size() const { return m_league; }
   int team_size() const { return m_team > 0 ? m_team : impl_default_team_size(); }
   int impl_team_size_raw() const { return m_team; }

@@ -289,6 +289,9 @@ class MI355X {
       : m_inst(std::make_shared<Impl::MI355XInstance>(
             stream, manage_stream, Impl::mi355x_globals().next_id++)) {}
 
+  // Instance wrapping existing per-instance state (graph nodes own one each).
+  explicit MI355X(std::shared_ptr<Impl::MI355XInstance> inst) : m_inst(std::move(inst)) {}
+
   static constexpr const char* name() { return "MI355X"; }
   int concurrency() const {
     // Resident lanes: 8 wavefronts of 64 lanes per SIMD, 4 SIMDs per CU

@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
             "or_scan_incl_i32": ([vp, vp, i64, C.c_int, vp], None),
             "or_scan_excl_i64": ([vp, vp, i64, C.c_int, vp], None),
             "or_scan_excl_f64": ([vp, vp, i64, C.c_int, vp], None),
+            "or_scan_excl_f64_compensated": ([vp, vp, i64, vp], None),
             "or_stream_copy": ([vp, vp, i64, C.c_int], None),
             "or_stream_scale": ([vp, vp, i64, f64, C.c_int], None),
             "or_stream_add": ([vp, vp, vp, i64, C.c_int], None),
@@ -164,6 +165,14 @@ def scan_excl_f64(x, nthreads=1):
     out = np.empty(x.size, np.float64)
     tot = np.zeros(1, np.float64)
     lib().or_scan_excl_f64(_p(x), _p(out), x.size, nthreads, _p(tot))
+    return out, float(tot[0])
+
+
+def scan_excl_f64_compensated(x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty(x.size, np.float64)
+    tot = np.zeros(1, np.float64)
+    lib().or_scan_excl_f64_compensated(_p(x), _p(out), x.size, _p(tot))
     return out, float(tot[0])
 
 

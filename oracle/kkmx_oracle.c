@@ -291,6 +291,21 @@ EXPORT void or_scan_excl_f64(const double* in, double* out, int64_t n, int nthre
   OR_SCAN_BODY(double, double, double, 0)
 }
 
+/* Exclusive prefix sums of doubles with a Neumaier-compensated running sum
+ * (Neumaier 1974): out[i] is the compensated sum of in[0..i-1], accurate to
+ * about one rounding, the reference value for a device scan's tolerance. */
+EXPORT void or_scan_excl_f64_compensated(const double* in, double* out, int64_t n, double* total) {
+  double s = 0.0, c = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    out[i] = s + c;
+    const double x = in[i], t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  if (total) *total = s + c;
+}
+
 /* ------------------------------------------------------------ STREAM */
 EXPORT void or_stream_copy(const double* a, double* b, int64_t n, int nthreads) {
 #pragma omp parallel for num_threads(nthreads) schedule(static)

@@ -1,0 +1,193 @@
+"""Batched uneven dispatch (one persistent fused kernel with a device work
+queue, its hipGraph replay, the per-op plain path and the per-op graph)
+against the oracle on the same seeded inputs.
+
+Integer reductions and scans: bit-exact.  Float64 reductions: within 1e-12
+relative of the exact (Shewchuk) sum; float64 exclusive scans: every prefix
+within 1e-12 relative of a Neumaier-compensated prefix (inputs are in [0, 1),
+so prefixes are non-negative and the first one is exactly 0).
+"""
+import numpy as np
+import pytest
+
+import kokkos_amd as K
+from kokkos_amd import batch as B
+from kokkos_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def expected(oracle, wl, inputs):
+    """Per-op expected results and expected scan output pools."""
+    nops = len(wl["n"])
+    res_i = np.zeros(nops, np.int64)
+    res_f = np.zeros(nops, np.float64)
+    out_i = np.zeros(wl["out_tot"]["i64"], np.int64)
+    out_f = np.zeros(wl["out_tot"]["f64"], np.float64)
+    csum = {}
+    for t in (W.T_I32, W.T_I64):
+        x = inputs[t].astype(np.int64)
+        c = np.zeros(x.size + 1, np.int64)
+        np.cumsum(x, out=c[1:])
+        csum[t] = c
+    for i in range(nops):
+        t, n, op = int(wl["type"][i]), int(wl["n"][i]), int(wl["op"][i])
+        a = int(wl["in_off"][i])
+        if t != W.T_F64:
+            c = csum[t]
+            res_i[i] = c[a + n] - c[a]
+            if op == W.OP_SCAN_EXCL:
+                o = int(wl["out_off"][i])
+                out_i[o:o + n] = c[a:a + n] - c[a]
+        else:
+            x = inputs[t][a:a + n]
+            if op == W.OP_SUM:
+                res_f[i] = oracle.sum_exact(x) if n else 0.0
+            else:
+                o = int(wl["out_off"][i])
+                pre, tot = oracle.scan_excl_f64_compensated(x)
+                out_f[o:o + n] = pre
+                res_f[i] = tot
+    return res_i, res_f, out_i, out_f
+
+
+def check_run(run, oracle, wl, exp):
+    res_i, res_f, out_i, out_f = exp
+    gi, gf = run.results()
+    isint = wl["type"] != W.T_F64
+    bad = np.nonzero(isint & (gi != res_i))[0]
+    assert bad.size == 0, f"int results differ at ops {bad[:10]}: {gi[bad[:5]]} vs {res_i[bad[:5]]}"
+    isf = ~isint
+    ref = res_f[isf]
+    err = np.abs(gf[isf] - ref) / np.maximum(ref, 1e-300)
+    err[ref == 0] = np.abs(gf[isf][ref == 0])
+    assert err.size == 0 or err.max() <= TOL, f"f64 result rel err {err.max():.2e}"
+    oi, of = run.outputs()
+    assert np.array_equal(oi, out_i), "int scan outputs differ"
+    e = np.abs(of - out_f)
+    assert np.all(e <= TOL * out_f), f"f64 scan max abs err {e.max():.2e}"
+    return float(err.max()) if err.size else 0.0
+
+
+def small_workload(nops, seed, lo, hi):
+    return W.batch_workload(nops, seed, lo, hi)
+
+
+@pytest.mark.parametrize("nops,seed,lo,hi", [(1, 3, 0, 0), (17, 5, 0, 7), (300, 10, 0, 14), (64, 7, 12, 17)])
+def test_batch_modes_match_oracle(paths, oracle, nops, seed, lo, hi):
+    wl = small_workload(nops, seed, lo, hi)
+    run = B.BatchRun(wl)
+    exp = expected(oracle, wl, run.inputs())
+    f64_ref = None
+    for mode in (B.MODE_PLAIN, B.MODE_FUSED, B.MODE_FUSED_GRAPH, B.MODE_PLAIN_GRAPH):
+        for rep in range(2):
+            run.clear_outputs()
+            print(f"batch nops={nops} seed={seed} sizes 2^[{lo},{hi}] mode={mode} rep={rep}", flush=True)
+            run.run(mode, 1 + rep)
+            K.fence()
+            err = check_run(run, oracle, wl, exp)
+            gi, gf = run.results()
+            if mode in (B.MODE_FUSED, B.MODE_FUSED_GRAPH):
+                # the fused kernel's combination order is fixed: replays and
+                # the graph give the same float64 bits
+                if f64_ref is None:
+                    f64_ref = gi.copy()
+                assert np.array_equal(gi, f64_ref)
+    print(f"max f64 rel err {err:.2e}, tiles {run.tiles}")
+    run.close()
+
+
+def test_batch_all_op_kinds_and_edges(paths, oracle):
+    """Min / Max / inclusive scan / axpb and zero-length ops, every type."""
+    P = paths
+    rng_sizes = [0, 1, 63, 64, 65, 4095, 4096, 4097, 100000]
+    ops = []
+    for t in (W.T_I32, W.T_I64, W.T_F64):
+        for n in rng_sizes:
+            for op in (W.OP_SUM, W.OP_MIN, W.OP_MAX, W.OP_SCAN_EXCL, W.OP_SCAN_INCL, W.OP_AXPB):
+                ops.append((t, n, op))
+    esz = {W.T_I32: 4, W.T_I64: 8, W.T_F64: 8}
+    npd = {W.T_I32: np.int32, W.T_I64: np.int64, W.T_F64: np.float64}
+    import ctypes as C
+    h = C.c_void_p()
+    K.check(P.kkmx_batch_create(C.byref(h)), P)
+    keep, cases = [], []
+    res = K.DeviceBuffer(8 * len(ops))
+    for k, (t, n, op) in enumerate(ops):
+        if t == W.T_F64:
+            x = oracle.fill_uniform_f64(n, 100 + k, -1.0, 1.0)
+        elif t == W.T_I64:
+            x = oracle.fill_uniform_i64(n, 100 + k, -(1 << 40), 1 << 40)
+        else:
+            x = oracle.fill_uniform_i32(n, 100 + k, -1000, 1000)
+        X = K.DeviceBuffer.from_array(x) if n else K.DeviceBuffer(8)
+        Y = K.DeviceBuffer(max(n, 1) * 8)
+        keep += [X, Y]
+        alpha, beta = (3.0, -2.0) if op == W.OP_AXPB else (1.0, 0.0)
+        K.check(P.kkmx_batch_add(h, op, t, X.ptr, n, Y.ptr, res.ptr + 8 * k, alpha, beta), P)
+        cases.append((t, n, op, x, Y, alpha, beta))
+    K.check(P.kkmx_batch_plan(h, None), P)
+    for mode in (B.MODE_PLAIN, B.MODE_FUSED, B.MODE_FUSED_GRAPH, B.MODE_PLAIN_GRAPH):
+        print(f"batch all-kinds mode={mode}", flush=True)
+        K.check(P.kkmx_batch_run(h, mode, 2), P)
+        K.fence()
+        r = res.download(np.int64, len(ops))
+        for k, (t, n, op, x, Y, alpha, beta) in enumerate(cases):
+            acc = np.float64 if t == W.T_F64 else np.int64
+            xa = x.astype(acc)
+            got = r[k:k + 1].view(acc)[0] if op != W.OP_AXPB else None
+            if op == W.OP_SUM:
+                want = oracle.sum_exact(xa) if t == W.T_F64 else int(xa.sum())
+                if t == W.T_F64:
+                    assert abs(got - want) <= TOL * max(np.abs(xa).sum(), 1e-300), (t, n, mode)
+                else:
+                    assert got == want, (t, n, op, mode)
+            elif op in (W.OP_MIN, W.OP_MAX):
+                if n == 0:
+                    ident = {W.OP_MIN: (np.finfo(np.float64).max if t == W.T_F64 else np.iinfo(np.int64).max),
+                             W.OP_MAX: (-np.finfo(np.float64).max if t == W.T_F64 else np.iinfo(np.int64).min)}[op]
+                    assert got == ident, (t, op, mode)
+                else:
+                    assert got == (xa.min() if op == W.OP_MIN else xa.max()), (t, n, op, mode)
+            elif op in (W.OP_SCAN_EXCL, W.OP_SCAN_INCL):
+                y = Y.download(acc, n)
+                if t == W.T_F64:
+                    pre, tot = oracle.scan_excl_f64_compensated(xa)
+                    absp = np.concatenate([[0.0], np.cumsum(np.abs(xa))])
+                    ref = pre if op == W.OP_SCAN_EXCL else pre + xa
+                    scale = absp[:n] if op == W.OP_SCAN_EXCL else absp[1:n + 1]
+                    assert np.all(np.abs(y - ref) <= TOL * np.maximum(scale, 1e-300) + 1e-300), (n, op, mode)
+                    assert abs(got - tot) <= TOL * max(absp[-1], 1e-300), (n, op, mode)
+                else:
+                    c = np.concatenate([[0], np.cumsum(xa)])
+                    ref = c[:n] if op == W.OP_SCAN_EXCL else c[1:]
+                    assert np.array_equal(y, ref), (t, n, op, mode)
+                    assert got == c[-1], (t, n, op, mode)
+            else:
+                y = Y.download(acc, n)
+                if t == W.T_F64:
+                    assert np.all(np.abs(y - (alpha * xa + beta)) <= 1e-15 * (np.abs(alpha * xa) + 2)), (n, mode)
+                else:
+                    assert np.array_equal(y, 3 * xa - 2), (t, n, mode)
+    K.check(P.kkmx_batch_destroy(h), P)
+
+
+def test_batch_full_workload_4096_ops(paths, oracle):
+    """The full workload: 4096 ops, sizes floor(2^U), U ~ U[8, 20], seed 10;
+    the fused kernel replayed from its hipGraph, every result checked."""
+    wl = W.batch_workload(4096, 10)
+    run = B.BatchRun(wl)
+    print(f"batch 4096 ops, {int(wl['n'].sum())} elements, {run.tiles} tiles", flush=True)
+    exp = expected(oracle, wl, run.inputs())
+    run.clear_outputs()
+    run.run(B.MODE_FUSED_GRAPH, 3)
+    K.fence()
+    err = check_run(run, oracle, wl, exp)
+    print(f"fused graph x3: max f64 rel err {err:.2e}")
+    run.clear_outputs()
+    run.run(B.MODE_PLAIN, 1)
+    K.fence()
+    check_run(run, oracle, wl, exp)
+    run.close()

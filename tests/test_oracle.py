@@ -170,3 +170,21 @@ def test_checksum_sensitivity(oracle):
     y[100] += 1
     assert oracle.checksum(y) != h
     assert oracle.checksum(x.copy()) == h
+
+
+def test_compensated_f64_scan_against_fsum(oracle):
+    """Every prefix of the compensated scan is within 2 ulp of math.fsum of
+    the same prefix (fsum is correctly rounded)."""
+    import math
+
+    x = oracle.fill_uniform_f64(3000, 77, -1.0, 1.0) * np.exp2(np.arange(3000) % 40 - 20)
+    pre, tot = oracle.scan_excl_f64_compensated(x)
+    for i in range(0, 3000, 7):
+        ref = math.fsum(x[:i])
+        assert abs(pre[i] - ref) <= 2 * np.spacing(abs(ref)) + 1e-300, i
+    assert abs(tot - math.fsum(x)) <= 2 * np.spacing(abs(math.fsum(x)))
+    assert pre[0] == 0.0
+    # a planted error (dropping the compensation term) fails the same check
+    naive = np.concatenate([[0.0], np.cumsum(x)[:-1]])
+    worst = max(abs(naive[i] - math.fsum(x[:i])) / (np.spacing(abs(math.fsum(x[:i]))) + 1e-300) for i in range(1, 3000, 7))
+    assert worst > 2
