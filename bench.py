@@ -9,7 +9,15 @@ dispatches; `value` is the algorithmic bytes of all steps divided by the
 time, in GB/s.  Per-dispatch times come from HIP events on the stream the
 dispatches run on.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Beside the headline line the same run reports, as extra fields, the other
+workloads of the path on the same GPU: the small plain-method pair (reduce
+2^24 doubles, exclusive scan 2^24 int32 -> int64), the MDRange stencil and
+Team SpMV, the atomics (GUPS, Zipf histogram, FP64 scatter) and batched
+uneven dispatch (4096 reduce/scan ops per step: per-call launches, the
+persistent fused kernel replayed from a hipGraph, and per-call launches
+with the learned tuner).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
 """
 from __future__ import annotations
 
@@ -19,6 +27,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -106,12 +116,167 @@ def cpu_baseline(seconds_budget=12.0):
     }
 
 
+def _timeit(K, timer, fn, reps=5, warm=2):
+    for _ in range(warm):
+        fn()
+    K.fence()
+    ts = []
+    for _ in range(reps):
+        timer.start()
+        fn()
+        ts.append(timer.stop())
+    return min(ts), float(np.median(ts))
+
+
+def leg_small(K, P, timer):
+    """Plain-method pair: reduce sum of 2^24 U[0,1) doubles (seed 0) and the
+    exclusive scan of 2^24 U[0,15] int32 (seed 1) into int64."""
+    n = 1 << 24
+    chk = lambda rc: K.check(rc, P)  # noqa: E731
+    x, r = K.DeviceBuffer(8 * n), K.DeviceBuffer(8)
+    s, o, t = K.DeviceBuffer(4 * n), K.DeviceBuffer(8 * n), K.DeviceBuffer(8)
+    chk(P.kkmx_fill_uniform_f64(x.ptr, n, 0, 0.0, 1.0))
+    chk(P.kkmx_fill_uniform_i32(s.ptr, n, 1, 0, 15))
+    K.fence()
+    red = _timeit(K, timer, lambda: chk(P.kkmx_reduce_sum_f64(x.ptr, n, r.ptr)), reps=20)
+    scn = _timeit(K, timer, lambda: chk(P.kkmx_scan_excl_i32_i64(s.ptr, o.ptr, n, t.ptr)), reps=20)
+    import math
+
+    got = float(r.download(np.float64, 1)[0])
+    exact = math.fsum(x.download(np.float64, n))
+    tot = int(t.download(np.int64, 1)[0])
+    tot_ref = int(s.download(np.int32, n).astype(np.int64).sum())
+    return {
+        "reduce_sum_f64_2^24": {"ms": round(red[0], 4), "gbps": round(8 * n / red[0] / 1e6, 1),
+                                "rel_err_vs_exact": abs(got - exact) / exact},
+        "scan_excl_i32_i64_2^24": {"ms": round(scn[0], 4), "gbps": round(12 * n / scn[0] / 1e6, 1),
+                                   "total_exact": tot == tot_ref},
+    }
+
+
+def leg_md_team(K, P, timer):
+    """7-point stencil on 512^3 (+halo), 10 sweeps; CSR SpMV with 2^20 rows."""
+    import ctypes as C
+
+    chk = lambda rc: K.check(rc, P)  # noqa: E731
+    n, m = 512, 514
+    U, V = K.DeviceBuffer(m**3 * 8), K.DeviceBuffer(m**3 * 8)
+    chk(P.kkmx_fill_uniform_f64(U.ptr, m**3, 5, 0.0, 1.0))
+    chk(P.kkmx_fill_uniform_f64(V.ptr, m**3, 5, 0.0, 1.0))
+    st = _timeit(K, timer, lambda: chk(P.kkmx_stencil7_sweeps(U.ptr, V.ptr, n, 10, 0.4, 0.1, 0, 0, 0)), reps=3, warm=1)
+    del U, V
+    nrows = 1 << 20
+    rp = K.DeviceBuffer((nrows + 1) * 8)
+    nnz = C.c_int64()
+    chk(P.kkmx_csr_generate(rp.ptr, None, None, nrows, nrows, 4, 64, 6, C.byref(nnz)))
+    col, val = K.DeviceBuffer(nnz.value * 4), K.DeviceBuffer(nnz.value * 8)
+    chk(P.kkmx_csr_fill(col.ptr, val.ptr, nnz.value, nrows, 6))
+    X, Y = K.DeviceBuffer(nrows * 8), K.DeviceBuffer(nrows * 8)
+    chk(P.kkmx_fill_uniform_f64(X.ptr, nrows, 606, -1.0, 1.0))
+    sp = _timeit(K, timer, lambda: chk(P.kkmx_spmv_csr(rp.ptr, col.ptr, val.ptr, X.ptr, Y.ptr, nrows, nnz.value,
+                                                       1, 0, 0)), reps=20)
+    spmv_bytes = nnz.value * 12 + nrows * 16 + nrows * 8
+    return {
+        "stencil7_512^3_x10": {"ms": round(st[0], 3), "gbps": round(16 * n**3 * 10 / st[0] / 1e6, 1),
+                               "bytes_per_point_sweep": 16},
+        "spmv_team_2^20_rows": {"ms": round(sp[0], 4), "gbps": round(spmv_bytes / sp[0] / 1e6, 1),
+                                "nnz": int(nnz.value), "algorithmic_bytes": int(spmv_bytes)},
+    }
+
+
+def leg_atomics(K, P, timer):
+    """GUPS (2^30 int64 table, 2^28 updates), Zipf(1.1) histogram of 2^28
+    keys over 65536 bins (LDS-privatised), FP64 scatter of 2^26 into 2^20."""
+    chk = lambda rc: K.check(rc, P)  # noqa: E731
+    rt = K.load("kkmx_rt")
+    T, U = 1 << 30, 1 << 28
+    S, Tb = K.DeviceBuffer(2 * U * 8), K.DeviceBuffer(T * 8)
+    chk(P.kkmx_xorshift64_stream(S.ptr, 2 * U, 7))
+    K.check(rt.mx_memset_async(Tb.ptr, 0, T * 8, P.kkmx_stream()))
+    g = _timeit(K, timer, lambda: chk(P.kkmx_gups(Tb.ptr, T, S.ptr, U)), reps=3, warm=1)
+    del S, Tb
+    nk, nb = 1 << 28, 65536
+    Kb, Bb = K.DeviceBuffer(nk * 4), K.DeviceBuffer(nb * 4)
+    chk(P.kkmx_zipf_keys(Kb.ptr, nk, 1.1, nb, 8))
+    h = _timeit(K, timer, lambda: chk(P.kkmx_histogram(Kb.ptr, nk, Bb.ptr, nb, 1, 32768)), reps=5)
+    del Kb, Bb
+    ns, nbins = 1 << 26, 1 << 20
+    Vb, Ib, Bb = K.DeviceBuffer(ns * 8), K.DeviceBuffer(ns * 4), K.DeviceBuffer(nbins * 8)
+    chk(P.kkmx_fill_uniform_f64(Vb.ptr, ns, 9, 0.0, 1.0))
+    chk(P.kkmx_fill_uniform_i32(Ib.ptr, ns, 9 ^ 0xABCDEF, 0, nbins - 1))
+    sc = _timeit(K, timer, lambda: chk(P.kkmx_scatter_add_f64(Vb.ptr, Ib.ptr, ns, Bb.ptr, nbins)), reps=5)
+    return {
+        "gups_2^30_table_2^28_updates": {"ms": round(g[0], 3), "gups": round(U / g[0] / 1e6, 3)},
+        "histogram_zipf_2^28_keys": {"ms": round(h[0], 3), "gbps": round(4 * nk / h[0] / 1e6, 1),
+                                     "keys_per_s": round(nk / h[0] * 1e3, 1)},
+        "scatter_f64_2^26_into_2^20": {"ms": round(sc[0], 3), "updates_per_s": round(ns / sc[0] * 1e3, 1),
+                                       "gbps": round(12 * ns / sc[0] / 1e6, 1)},
+    }
+
+
+def leg_batched(K, P, timer, replays=1000, plain_steps=5):
+    """4096 uneven reduce/scan ops per step (seed 10): per-call launches,
+    the persistent fused kernel (direct and replayed from a hipGraph) and,
+    when a trained model is present, per-call launches with the learned
+    tuner loaded as a Kokkos Tools library."""
+    from kokkos_amd import batch as B
+    from kokkos_amd import workloads as W
+
+    wl = W.batch_workload(4096, 10)
+    nbytes = W.batch_bytes(wl)
+    run = B.BatchRun(wl)
+    out = {"ops_per_step": 4096, "elements_per_step": int(wl["n"].sum()), "bytes_per_step": int(nbytes),
+           "tiles": int(run.tiles)}
+
+    def per_step(mode, steps):
+        run.run(mode, 1)
+        K.fence()
+        timer.start()
+        run.run(mode, steps)
+        return timer.stop() / steps
+
+    ms = per_step(B.MODE_FUSED_GRAPH, replays)
+    gi, gf = (a.copy() for a in run.results())
+    out["fused_graph"] = {"ms_per_step": round(ms, 4), "ops_per_s": round(4096 / ms * 1e3, 1),
+                          "gbps": round(nbytes / ms / 1e6, 1), "replays": replays}
+    ms = per_step(B.MODE_FUSED, 50)
+    out["fused"] = {"ms_per_step": round(ms, 4), "ops_per_s": round(4096 / ms * 1e3, 1)}
+    ms = per_step(B.MODE_PLAIN, plain_steps)
+    pi, pf = (a.copy() for a in run.results())
+    out["plain"] = {"ms_per_step": round(ms, 3), "ops_per_s": round(4096 / ms * 1e3, 1)}
+    isint = wl["type"] != W.T_F64
+    out["int_results_identical"] = bool(np.array_equal(gi[isint], pi[isint]))
+    ref = np.abs(pf[~isint])
+    out["f64_max_rel_diff_fused_vs_plain"] = float(np.max(np.abs(gf[~isint] - pf[~isint]) / np.maximum(ref, 1e-300)))
+    ms = per_step(B.MODE_PLAIN_GRAPH, plain_steps)
+    out["plain_graph"] = {"ms_per_step": round(ms, 3), "ops_per_s": round(4096 / ms * 1e3, 1)}
+    model = ROOT / "kokkos_amd" / "tuner" / "mlp_gfx950.bin"
+    if model.exists():
+        from kokkos_amd import tuner as T
+
+        lib = str(K.lib_path("kkmx_tuner")).encode()
+        K.check(P.kkmx_tools_load(lib), P)
+        try:
+            T.load_into_tool(model)
+            ms = per_step(B.MODE_PLAIN, plain_steps)
+            ti, tf = run.results()
+            out["plain_tuned"] = {"ms_per_step": round(ms, 3), "ops_per_s": round(4096 / ms * 1e3, 1),
+                                  "tool": T.tool_stats(),
+                                  "int_results_identical": bool(np.array_equal(ti[isint], pi[isint]))}
+        finally:
+            K.check(P.kkmx_tools_unload(), P)
+    out["speedup_fused_graph_vs_plain"] = round(out["fused_graph"]["ops_per_s"] / out["plain"]["ops_per_s"], 2)
+    run.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -192,19 +357,26 @@ def main():
     avg_ms = per_op_ms.mean(axis=0)
     op_gbps = {name: round(bpe * N / (avg_ms[k] * 1e-3) / 1e9, 1) for k, (name, bpe) in enumerate(OPS)}
 
-    # correctness of the timed results (bit-exact integer reductions)
-    import oracle as O
-
+    # correctness of the timed results (bit-exact integer reductions, checked
+    # with numpy on the downloaded input; int64 sums wrap like the device's)
     hx = x.download(np.int64, N)
     got = res.download(np.int64, 3)
-    ok = (
-        int(got[0]) == O.reduce_i64(hx, 2, 8)
-        and int(got[1]) == O.reduce_i64(hx, 1, 8)
-        and int(got[2]) == O.reduce_i64(hx, 0, 8)
-    )
+    ok = int(got[0]) == int(hx.max()) and int(got[1]) == int(hx.min()) and int(got[2]) == int(hx.sum())
     del hx
 
     peak = measured_copy_peak(K, a.ptr, b.ptr, N * 8, K.EventTimer())
+    del a, b, c, x, s, so
+
+    extra = {}
+    if not args.no_extra:
+        timer = K.EventTimer()
+        for name, fn in [("small_reduce_scan", lambda: leg_small(K, P, timer)),
+                         ("mdrange_team", lambda: leg_md_team(K, P, timer)),
+                         ("atomics", lambda: leg_atomics(K, P, timer)),
+                         ("batched", lambda: leg_batched(K, P, timer))]:
+            t_leg = time.perf_counter()
+            extra[name] = fn()
+            print(f"bench: {name} done in {time.perf_counter() - t_leg:.1f} s", file=sys.stderr, flush=True)
 
     # dominant kernel: the dispatch with the largest share of the step
     dom = int(np.argmax(avg_ms))
@@ -258,6 +430,15 @@ def main():
         },
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
+    if extra:
+        line["workloads"] = extra
+        bt = extra["batched"]
+        line["batched_ops_per_s"] = {
+            "fused_graph": bt["fused_graph"]["ops_per_s"],
+            "plain": bt["plain"]["ops_per_s"],
+            "plain_tuned": bt.get("plain_tuned", {}).get("ops_per_s"),
+            "speedup_vs_plain": bt["speedup_fused_graph_vs_plain"],
+        }
     print(json.dumps(line))
 
 

@@ -129,6 +129,13 @@ int kkmx_batch_add(kkmx_batch* b, int op, int type, const void* in_dev, int64_t 
 int kkmx_batch_plan(kkmx_batch* b, int64_t* tiles_out);
 int kkmx_batch_run(kkmx_batch* b, int mode, int steps);
 
+/* ---- Kokkos Tools ----
+ * Load tool libraries (';'-separated) into the running process, as
+ * KOKKOS_TOOLS_LIBS does at initialisation (core/src/impl/Kokkos_Profiling.cpp:
+ * 520-700); a tuning tool then answers the dispatches' tuning requests. */
+int kkmx_tools_load(const char* libs);
+int kkmx_tools_unload(void);
+
 /* ---- launch-shape overrides (what the tuning hooks set), -1 = heuristic ---- */
 int kkmx_set_range_config(int pattern, int block, int items, int max_blocks);
 

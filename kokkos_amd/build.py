@@ -40,6 +40,8 @@ LIBS = {
     "kkmx_paths": (["csrc/paths.hip"], [], True),
     "kkmx_peak": (["csrc/peak.hip"], [], True),
     "kkmx_selftest": (["csrc/selftest.hip"], [], True),
+    "kkmx_tuner": (["csrc/tuner.hip"], [], True),
+    "kkmx_sweep": (["csrc/sweep.hip"], [], True),
 }
 
 

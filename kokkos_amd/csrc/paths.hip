@@ -790,4 +790,15 @@ int kkmx_batch_run(kkmx_batch* b, int mode, int steps) {
   });
 }
 
+// ---------------------------------------------------------------- tools
+// Loads Kokkos Tools libraries (';'-separated paths) into the running
+// process, as KOKKOS_TOOLS_LIBS does at initialisation; unload stops them.
+int kkmx_tools_load(const char* libs) {
+  if (!libs) return reject("kkmx_tools_load: null path");
+  return guarded([&] { Kokkos::Tools::Impl::tools_start(libs, ""); });
+}
+int kkmx_tools_unload(void) {
+  return guarded([&] { Kokkos::Tools::Impl::tools_stop(); });
+}
+
 }  // extern "C"

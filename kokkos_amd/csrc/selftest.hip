@@ -1428,48 +1428,46 @@ KKMX_TEST(atomics_min_max_add_sub) {
   atomic_minmax_case<float>(1000);
 }
 
-// Sub-word (char, short) and wide (16-byte) types.
+// Sub-word (char, short) and wide (16-byte) types.  These go through
+// compare-exchange loops on the containing 32-bit word and through the lock
+// array: every success makes all concurrent attempts on the same word or lock
+// retry, so the retry count grows with the square of the threads per word.
+// 4096 threads over 256 slots keep it to a few hundred thousand attempts.
 struct Wide {
   double a, b;
   KOKKOS_INLINE_FUNCTION Wide operator+(const Wide& o) const { return Wide{a + o.a, b + o.b}; }
   KOKKOS_INLINE_FUNCTION bool operator==(const Wide& o) const { return a == o.a && b == o.b; }
 };
+constexpr int kSlots = 256;
 struct SubWordWideF {
   Kokkos::View<int8_t*, Exec> c;
   Kokkos::View<int16_t*, Exec> s;
   Kokkos::View<Wide*, Exec> w;
   KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
-    Kokkos::atomic_add(&c(i % 4), int8_t(1));
-    Kokkos::atomic_add(&s(i % 3), int16_t(2));
-    Kokkos::atomic_add(&w(i % 5), Wide{1.0, double(i)});
+    Kokkos::atomic_add(&c(i % kSlots), int8_t(1));
+    Kokkos::atomic_add(&s(i % kSlots), int16_t(2));
+    Kokkos::atomic_add(&w(i % kSlots), Wide{1.0, double(i)});
   }
 };
 KKMX_TEST(atomics_subword_and_wide) {
-  const int n = 100000;
-  Kokkos::View<int8_t*, Exec> c("c", 4);
-  Kokkos::View<int16_t*, Exec> s("s", 3);
-  Kokkos::View<Wide*, Exec> w("w", 5);
+  const int n = 4096 + 77;
+  Kokkos::View<int8_t*, Exec> c("c", kSlots);
+  Kokkos::View<int16_t*, Exec> s("s", kSlots);
+  Kokkos::View<Wide*, Exec> w("w", kSlots);
   Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), SubWordWideF{c, s, w});
   auto hc = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), c);
   auto hs = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), s);
   auto hw = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), w);
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < kSlots; ++k) {
     int cnt = 0;
-    for (int i = k; i < n; i += 4) ++cnt;
-    CHECK_EQ(int(hc(k)), int(int8_t(cnt)));
-  }
-  for (int k = 0; k < 3; ++k) {
-    int cnt = 0;
-    for (int i = k; i < n; i += 3) ++cnt;
-    CHECK_EQ(int(hs(k)), int(int16_t(2 * cnt)));
-  }
-  for (int k = 0; k < 5; ++k) {
-    double cnt = 0, sum = 0;
-    for (int i = k; i < n; i += 5) {
-      cnt += 1;
+    double sum = 0;
+    for (int i = k; i < n; i += kSlots) {
+      ++cnt;
       sum += i;
     }
-    CHECK_EQ(hw(k).a, cnt);
+    CHECK_EQ(int(hc(k)), int(int8_t(cnt)));
+    CHECK_EQ(int(hs(k)), int(int16_t(2 * cnt)));
+    CHECK_EQ(hw(k).a, double(cnt));
     CHECK_EQ(hw(k).b, sum);
   }
 }

@@ -1,0 +1,395 @@
+// Launch-shape sweep: the timings the learned tuner is trained on, collected
+// by the runtime itself.
+//
+// A family is one Range dispatch (parallel_for, parallel_reduce or
+// parallel_scan) with a fixed functor whose bytes and flops per index are
+// declared as cost hints.  kkmx_sweep_time runs one family at one extent
+// under each of the 16 launch-shape candidates the tuning hooks choose from
+// and returns the best-of-reps kernel time of each, measured with HIP events
+// on the instance stream.  Plain Kokkos user code: the candidates are applied
+// through the same launch-shape override the tuning hooks use.
+#include <Kokkos_Core.hpp>
+
+#include <cmath>
+#include <string>
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class T>
+using Vec = Kokkos::View<T*, Kokkos::LayoutRight, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+template <class T>
+using Scalar = Kokkos::View<T, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+
+struct Buffers {
+  int64_t n   = 0;
+  double* a   = nullptr;
+  double* b   = nullptr;
+  double* c   = nullptr;
+  double* res = nullptr;  // 64 bytes of reduction results
+} g_buf;
+
+// ---------------------------------------------------------------- families
+struct CopyF64 {
+  static constexpr double kkmx_bytes_per_index = 16, kkmx_flops_per_index = 0;
+  Vec<const double> x;
+  Vec<double> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { y(i) = x(i); }
+};
+struct TriadF64 {
+  static constexpr double kkmx_bytes_per_index = 24, kkmx_flops_per_index = 2;
+  Vec<double> a;
+  Vec<const double> b, c;
+  double s;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { a(i) = b(i) + s * c(i); }
+};
+struct ScaleI32 {
+  static constexpr double kkmx_bytes_per_index = 8, kkmx_flops_per_index = 1;
+  Vec<const int32_t> x;
+  Vec<int32_t> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { y(i) = 3 * x(i); }
+};
+template <int K>
+struct PolyF64 {
+  static constexpr double kkmx_bytes_per_index = 16, kkmx_flops_per_index = 2 * K;
+  Vec<const double> x;
+  Vec<double> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const {
+    const double v = x(i);
+    double p       = 1.0;
+#pragma unroll 8
+    for (int k = 0; k < K; ++k) p = p * v + 0.5;
+    y(i) = p;
+  }
+};
+struct FillI64 {
+  static constexpr double kkmx_bytes_per_index = 8, kkmx_flops_per_index = 1;
+  Vec<int64_t> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { y(i) = i * 7; }
+};
+struct Stencil1D {
+  static constexpr double kkmx_bytes_per_index = 16, kkmx_flops_per_index = 2;
+  Vec<const double> x;
+  Vec<double> y;
+  int64_t n;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const {
+    const int64_t l = i > 0 ? i - 1 : 0, r = i + 1 < n ? i + 1 : n - 1;
+    y(i)            = x(l) + x(i) + x(r);
+  }
+};
+struct SumF64 {
+  static constexpr double kkmx_bytes_per_index = 8, kkmx_flops_per_index = 1;
+  Vec<const double> x;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, double& s) const { s += x(i); }
+};
+struct SumI32 {
+  static constexpr double kkmx_bytes_per_index = 4, kkmx_flops_per_index = 1;
+  Vec<const int32_t> x;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int64_t& s) const { s += x(i); }
+};
+struct SumI64 {
+  static constexpr double kkmx_bytes_per_index = 8, kkmx_flops_per_index = 1;
+  Vec<const int64_t> x;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int64_t& s) const { s += x(i); }
+};
+struct SumF32 {
+  static constexpr double kkmx_bytes_per_index = 4, kkmx_flops_per_index = 1;
+  Vec<const float> x;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, float& s) const { s += x(i); }
+};
+struct DotF64 {
+  static constexpr double kkmx_bytes_per_index = 16, kkmx_flops_per_index = 2;
+  Vec<const double> x, y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, double& s) const { s += x(i) * y(i); }
+};
+struct MaxI64 {
+  static constexpr double kkmx_bytes_per_index = 8, kkmx_flops_per_index = 1;
+  Vec<const int64_t> x;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int64_t& m) const {
+    if (x(i) > m) m = x(i);
+  }
+};
+struct MinMaxSum {
+  double mn, mx, sum;
+};
+struct MinMaxSumF64 {
+  static constexpr double kkmx_bytes_per_index = 8, kkmx_flops_per_index = 3;
+  using value_type = MinMaxSum;
+  Vec<const double> x;
+  KOKKOS_INLINE_FUNCTION void init(MinMaxSum& v) const {
+    v.mn  = 1e300;
+    v.mx  = -1e300;
+    v.sum = 0;
+  }
+  KOKKOS_INLINE_FUNCTION void join(MinMaxSum& a, const MinMaxSum& b) const {
+    a.mn = b.mn < a.mn ? b.mn : a.mn;
+    a.mx = b.mx > a.mx ? b.mx : a.mx;
+    a.sum += b.sum;
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, MinMaxSum& v) const {
+    const double t = x(i);
+    v.mn           = t < v.mn ? t : v.mn;
+    v.mx           = t > v.mx ? t : v.mx;
+    v.sum += t;
+  }
+};
+template <int K>
+struct PolySumF64 {
+  static constexpr double kkmx_bytes_per_index = 8, kkmx_flops_per_index = 2 * K;
+  Vec<const double> x;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, double& s) const {
+    const double v = x(i);
+    double p       = 1.0;
+#pragma unroll 8
+    for (int k = 0; k < K; ++k) p = p * v + 0.5;
+    s += p;
+  }
+};
+struct ScanExclI32I64 {
+  static constexpr double kkmx_bytes_per_index = 12, kkmx_flops_per_index = 1;
+  Vec<const int32_t> x;
+  Vec<int64_t> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int64_t& u, const bool final) const {
+    if (final) y(i) = u;
+    u += x(i);
+  }
+};
+struct ScanInclI32 {
+  static constexpr double kkmx_bytes_per_index = 8, kkmx_flops_per_index = 1;
+  Vec<const int32_t> x;
+  Vec<int32_t> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int32_t& u, const bool final) const {
+    u += x(i);
+    if (final) y(i) = u;
+  }
+};
+struct ScanExclI64 {
+  static constexpr double kkmx_bytes_per_index = 16, kkmx_flops_per_index = 1;
+  Vec<const int64_t> x;
+  Vec<int64_t> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int64_t& u, const bool final) const {
+    if (final) y(i) = u;
+    u += x(i);
+  }
+};
+struct ScanExclF64 {
+  static constexpr double kkmx_bytes_per_index = 16, kkmx_flops_per_index = 1;
+  Vec<const double> x;
+  Vec<double> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, double& u, const bool final) const {
+    if (final) y(i) = u;
+    u += x(i);
+  }
+};
+struct CountSum {
+  int64_t count, sum;
+  KOKKOS_INLINE_FUNCTION CountSum& operator+=(const CountSum& o) {
+    count += o.count;
+    sum += o.sum;
+    return *this;
+  }
+};
+struct ScanPairI64 {
+  static constexpr double kkmx_bytes_per_index = 16, kkmx_flops_per_index = 2;
+  Vec<const int64_t> x;
+  Vec<int64_t> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, CountSum& u, const bool final) const {
+    u.count += x(i) & 1;
+    u.sum += x(i);
+    if (final) y(i) = u.sum - u.count;
+  }
+};
+
+struct InitBuffers {
+  Vec<double> a, b, c;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const {
+    a(i) = double(i % 1000) * 1e-3;
+    b(i) = double(i % 77) * 1e-2;
+    c(i) = 0.25;
+  }
+};
+
+constexpr int kFamilies = 20;
+
+// pattern, value bytes, floating, functor bytes, bytes/index, flops/index
+struct FamilyInfo {
+  int pattern;
+  int value_bytes;
+  int floating;
+  int functor_bytes;
+  double bytes, flops;
+};
+
+template <class F>
+FamilyInfo info(int pattern, int value_bytes, int floating) {
+  return FamilyInfo{pattern, value_bytes, floating, int(sizeof(F)), F::kkmx_bytes_per_index, F::kkmx_flops_per_index};
+}
+
+FamilyInfo family_info(int f) {
+  switch (f) {
+    case 0: return info<CopyF64>(0, 0, 0);
+    case 1: return info<TriadF64>(0, 0, 0);
+    case 2: return info<ScaleI32>(0, 0, 0);
+    case 3: return info<PolyF64<8>>(0, 0, 0);
+    case 4: return info<PolyF64<64>>(0, 0, 0);
+    case 5: return info<FillI64>(0, 0, 0);
+    case 6: return info<Stencil1D>(0, 0, 0);
+    case 7: return info<SumF64>(1, 8, 1);
+    case 8: return info<SumI32>(1, 8, 0);
+    case 9: return info<SumI64>(1, 8, 0);
+    case 10: return info<SumF32>(1, 4, 1);
+    case 11: return info<DotF64>(1, 8, 1);
+    case 12: return info<MaxI64>(1, 8, 0);
+    case 13: return info<MinMaxSumF64>(1, 24, 1);
+    case 14: return info<PolySumF64<32>>(1, 8, 1);
+    case 15: return info<ScanExclI32I64>(2, 8, 0);
+    case 16: return info<ScanInclI32>(2, 4, 0);
+    case 17: return info<ScanExclI64>(2, 8, 0);
+    case 18: return info<ScanExclF64>(2, 8, 1);
+    default: return info<ScanPairI64>(2, 16, 0);
+  }
+}
+
+using P = Kokkos::RangePolicy<Kokkos::MI355X, Kokkos::IndexType<int64_t>>;
+
+void run_family(int f, int64_t n) {
+  const P p(0, n);
+  double *a = g_buf.a, *b = g_buf.b, *c = g_buf.c;
+  Scalar<double> rd(g_buf.res);
+  Scalar<int64_t> ri(reinterpret_cast<int64_t*>(g_buf.res));
+  Scalar<float> rf(reinterpret_cast<float*>(g_buf.res));
+  Scalar<MinMaxSum> rm(reinterpret_cast<MinMaxSum*>(g_buf.res));
+  auto i32 = [](double* q, int64_t m) { return Vec<int32_t>(reinterpret_cast<int32_t*>(q), m); };
+  auto i64 = [](double* q, int64_t m) { return Vec<int64_t>(reinterpret_cast<int64_t*>(q), m); };
+  switch (f) {
+    case 0: Kokkos::parallel_for("sweep.copy", p, CopyF64{Vec<const double>(a, n), Vec<double>(b, n)}); break;
+    case 1:
+      Kokkos::parallel_for("sweep.triad", p, TriadF64{Vec<double>(a, n), Vec<const double>(b, n), Vec<const double>(c, n), 3.0});
+      break;
+    case 2: Kokkos::parallel_for("sweep.scale_i32", p, ScaleI32{i32(a, n), i32(b, n)}); break;
+    case 3: Kokkos::parallel_for("sweep.poly8", p, PolyF64<8>{Vec<const double>(a, n), Vec<double>(b, n)}); break;
+    case 4: Kokkos::parallel_for("sweep.poly64", p, PolyF64<64>{Vec<const double>(a, n), Vec<double>(b, n)}); break;
+    case 5: Kokkos::parallel_for("sweep.fill_i64", p, FillI64{i64(a, n)}); break;
+    case 6: Kokkos::parallel_for("sweep.stencil1d", p, Stencil1D{Vec<const double>(a, n), Vec<double>(b, n), n}); break;
+    case 7: Kokkos::parallel_reduce("sweep.sum_f64", p, SumF64{Vec<const double>(a, n)}, rd); break;
+    case 8: Kokkos::parallel_reduce("sweep.sum_i32", p, SumI32{i32(a, n)}, ri); break;
+    case 9: Kokkos::parallel_reduce("sweep.sum_i64", p, SumI64{i64(a, n)}, ri); break;
+    case 10:
+      Kokkos::parallel_reduce("sweep.sum_f32", p, SumF32{Vec<const float>(reinterpret_cast<float*>(a), n)}, rf);
+      break;
+    case 11: Kokkos::parallel_reduce("sweep.dot", p, DotF64{Vec<const double>(a, n), Vec<const double>(b, n)}, rd); break;
+    case 12:
+      Kokkos::parallel_reduce("sweep.max_i64", p, MaxI64{i64(a, n)}, Kokkos::Max<int64_t, Kokkos::MI355XSpace>(ri));
+      break;
+    case 13: Kokkos::parallel_reduce("sweep.minmaxsum", p, MinMaxSumF64{Vec<const double>(a, n)}, rm); break;
+    case 14: Kokkos::parallel_reduce("sweep.polysum32", p, PolySumF64<32>{Vec<const double>(a, n)}, rd); break;
+    case 15: Kokkos::parallel_scan("sweep.scan_i32_i64", p, ScanExclI32I64{i32(a, n), i64(b, n)}); break;
+    case 16: Kokkos::parallel_scan("sweep.scan_incl_i32", p, ScanInclI32{i32(a, n), i32(b, n)}); break;
+    case 17: Kokkos::parallel_scan("sweep.scan_i64", p, ScanExclI64{i64(a, n), i64(b, n)}); break;
+    case 18: Kokkos::parallel_scan("sweep.scan_f64", p, ScanExclF64{Vec<const double>(a, n), Vec<double>(b, n)}); break;
+    default: Kokkos::parallel_scan("sweep.scan_pair", p, ScanPairI64{i64(a, n), i64(b, n)}); break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* kkmx_sweep_last_error(void) { return g_err.c_str(); }
+int kkmx_sweep_family_count(void) { return kFamilies; }
+
+// pattern, value bytes, floating, functor bytes, bytes/index, flops/index of a family.
+int kkmx_sweep_family_info(int f, int* pattern, int* value_bytes, int* floating, int* functor_bytes, double* bytes,
+                           double* flops) {
+  if (f < 0 || f >= kFamilies) return -1;
+  const FamilyInfo i = family_info(f);
+  *pattern           = i.pattern;
+  *value_bytes       = i.value_bytes;
+  *floating          = i.floating;
+  *functor_bytes     = i.functor_bytes;
+  *bytes             = i.bytes;
+  *flops             = i.flops;
+  return 0;
+}
+
+// Allocates three arrays of max_n doubles (inputs filled with small values
+// so every family's arithmetic stays finite).
+int kkmx_sweep_setup(int64_t max_n) {
+  if (max_n < 1) return -1;
+  try {
+    if (!Kokkos::is_initialized()) Kokkos::initialize();
+    double** bufs[3] = {&g_buf.a, &g_buf.b, &g_buf.c};
+    for (auto* q : bufs) KKMX_SAFE_CALL(mx_malloc(reinterpret_cast<void**>(q), size_t(max_n) * 8, nullptr));
+    KKMX_SAFE_CALL(mx_malloc(reinterpret_cast<void**>(&g_buf.res), 64, nullptr));
+    g_buf.n = max_n;
+    Kokkos::parallel_for("sweep.init", P(0, max_n), InitBuffers{Vec<double>(g_buf.a, max_n), Vec<double>(g_buf.b, max_n),
+                                                                Vec<double>(g_buf.c, max_n)});
+    Kokkos::fence();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int kkmx_sweep_teardown(void) {
+  double** bufs[4] = {&g_buf.a, &g_buf.b, &g_buf.c, &g_buf.res};
+  for (auto* q : bufs) {
+    if (*q) (void)mx_free(*q, nullptr);
+    *q = nullptr;
+  }
+  g_buf.n = 0;
+  return 0;
+}
+
+// Best-of-`reps` kernel time (ms) of family f at extent n under each of the
+// 16 launch-shape candidates (ms_out[16]), after one untimed warm-up each.
+// candidate < 0 times all 16; otherwise only that one (ms_out[0]).  heuristic
+// = 1 adds the time of the untuned heuristic shape in ms_out[16].
+int kkmx_sweep_time(int f, int64_t n, int reps, int candidate, int heuristic, float* ms_out) {
+  if (f < 0 || f >= kFamilies || n < 1 || n > g_buf.n || reps < 1 || !ms_out) {
+    g_err = "kkmx_sweep_time: bad arguments";
+    return -1;
+  }
+  try {
+    Kokkos::MI355X ex;
+    mx_event_t e0, e1;
+    KKMX_SAFE_CALL(mx_event_create(&e0));
+    KKMX_SAFE_CALL(mx_event_create(&e1));
+    const int pattern = family_info(f).pattern;
+    auto& ov          = Kokkos::Impl::range_overrides();
+    auto time_one     = [&]() {
+      float best = 1e30f;
+      run_family(f, n);
+      for (int r = 0; r < reps; ++r) {
+        KKMX_SAFE_CALL(mx_event_record(e0, ex.stream()));
+        run_family(f, n);
+        KKMX_SAFE_CALL(mx_event_record(e1, ex.stream()));
+        KKMX_SAFE_CALL(mx_event_sync(e1));
+        float ms = 0;
+        KKMX_SAFE_CALL(mx_event_elapsed_ms(&ms, e0, e1));
+        best = ms < best ? ms : best;
+      }
+      return best;
+    };
+    const int c0 = candidate < 0 ? 0 : candidate, c1 = candidate < 0 ? Kokkos::Impl::kRangeCandidates : candidate + 1;
+    for (int c = c0; c < c1; ++c) {
+      ov.cfg[pattern] = Kokkos::Impl::range_candidate(pattern, c, n);
+      ov.set[pattern] = true;
+      ms_out[c - c0]  = time_one();
+    }
+    ov.set[pattern] = false;
+    if (heuristic) ms_out[c1 - c0] = time_one();
+    KKMX_SAFE_CALL(mx_event_destroy(e0));
+    KKMX_SAFE_CALL(mx_event_destroy(e1));
+    return 0;
+  } catch (const std::exception& e) {
+    Kokkos::Impl::range_overrides().set[0] = Kokkos::Impl::range_overrides().set[1] =
+        Kokkos::Impl::range_overrides().set[2] = false;
+    g_err = e.what();
+    return 1;
+  }
+}
+
+}  // extern "C"

@@ -549,6 +549,112 @@ inline RangeConfig default_reduce_config(int64_t n) {
   return c;
 }
 
+// ------------------------------------------------------------------ tuning hooks
+// When a Kokkos Tools tuning tool is loaded, Range dispatches without an
+// explicit launch shape ask it for one of 16 candidates through the
+// reference's tuning protocol (core/src/impl/Kokkos_Profiling.cpp:1032-1211:
+// declare_input_type / declare_output_type, set_input_values,
+// request_output_values inside a context).  The request carries the
+// heuristic's own candidate as the default value, so a tool that does not
+// answer leaves the heuristic in charge.
+//   for / reduce: block = 128 << (c / 4), items per lane = 1 << (c % 4);
+//   scan:         block = 128 << (c / 4), items per lane = 8 << (c % 4).
+constexpr int kRangeCandidates = 16;
+inline RangeConfig range_candidate(int pattern, int c, int64_t n) {
+  const int block = 128 << (c >> 2);
+  if (pattern == 2) return RangeConfig{block, 8 << (c & 3), 0};
+  RangeConfig r{block, 1 << (c & 3), 0};
+  if (pattern == 1) {  // same resident-thread cap as the heuristic
+    const int cus = MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256;
+    r.max_blocks  = cus * 4 * 256 / block;
+    if (n < static_cast<int64_t>(r.block) * r.items * r.max_blocks) r.max_blocks = 0;
+  }
+  return r;
+}
+inline int range_candidate_index(int pattern, const RangeConfig& c) {
+  int bi = -1;
+  for (int k = 0; k < 4; ++k)
+    if (c.block == (128 << k)) bi = k;
+  int ii = -1;
+  for (int k = 0; k < 4; ++k)
+    if (c.items == (pattern == 2 ? 8 << k : 1 << k)) ii = k;
+  return (bi < 0 || ii < 0) ? -1 : bi * 4 + ii;
+}
+
+// Optional cost hints a functor may declare for the tuner:
+//   static constexpr double kkmx_bytes_per_index, kkmx_flops_per_index;
+template <class F, class = void>
+struct functor_cost_hint {
+  static constexpr double bytes = 0.0, flops = 0.0;
+};
+template <class F>
+struct functor_cost_hint<F, std::void_t<decltype(F::kkmx_bytes_per_index), decltype(F::kkmx_flops_per_index)>> {
+  static constexpr double bytes = F::kkmx_bytes_per_index, flops = F::kkmx_flops_per_index;
+};
+
+struct RangeTuningIds {
+  size_t out = 0, pattern = 0, extent = 0, value_bytes = 0, functor_bytes = 0, floating = 0, bytes = 0, flops = 0,
+         cus = 0;
+};
+inline const RangeTuningIds& range_tuning_ids() {
+  static const RangeTuningIds ids = [] {
+    using namespace Tools::Experimental;
+    RangeTuningIds r;
+    static int64_t cand[kRangeCandidates];
+    for (int k = 0; k < kRangeCandidates; ++k) cand[k] = k;
+    VariableInfo o{};
+    o.type          = kokkos_value_int64;
+    o.category      = kokkos_value_categorical;
+    o.valueQuantity = kokkos_value_set;
+    o.candidates    = make_candidate_set(kRangeCandidates, cand);
+    r.out           = declare_output_type("kokkos.mi355x.range_launch", o);
+    auto in         = [](const char* name, ValueType t, StatisticalCategory cat) {
+      VariableInfo i{};
+      i.type          = t;
+      i.category      = cat;
+      i.valueQuantity = kokkos_value_unbounded;
+      return declare_input_type(name, i);
+    };
+    r.pattern       = in("kokkos.mi355x.pattern", kokkos_value_int64, kokkos_value_categorical);
+    r.extent        = in("kokkos.mi355x.extent", kokkos_value_int64, kokkos_value_ratio);
+    r.value_bytes   = in("kokkos.mi355x.value_bytes", kokkos_value_int64, kokkos_value_ratio);
+    r.functor_bytes = in("kokkos.mi355x.functor_bytes", kokkos_value_int64, kokkos_value_ratio);
+    r.floating      = in("kokkos.mi355x.floating", kokkos_value_int64, kokkos_value_categorical);
+    r.bytes         = in("kokkos.mi355x.bytes_per_index", kokkos_value_double, kokkos_value_ratio);
+    r.flops         = in("kokkos.mi355x.flops_per_index", kokkos_value_double, kokkos_value_ratio);
+    r.cus           = in("kokkos.mi355x.cu_count", kokkos_value_int64, kokkos_value_ratio);
+    return r;
+  }();
+  return ids;
+}
+
+// pattern: 0 for, 1 reduce, 2 scan.
+template <class F>
+RangeConfig tuned_range_config(int pattern, int64_t n, size_t value_bytes, bool floating,
+                               const RangeConfig& heuristic) {
+  using namespace Tools::Experimental;
+  if (!have_tuning_tool()) return heuristic;
+  const RangeTuningIds& id = range_tuning_ids();
+  const size_t ctx         = get_new_context_id();
+  begin_context(ctx);
+  VariableValue in[8] = {make_variable_value(id.pattern, int64_t(pattern)),
+                         make_variable_value(id.extent, int64_t(n)),
+                         make_variable_value(id.value_bytes, int64_t(value_bytes)),
+                         make_variable_value(id.functor_bytes, int64_t(sizeof(F))),
+                         make_variable_value(id.floating, int64_t(floating ? 1 : 0)),
+                         make_variable_value(id.bytes, functor_cost_hint<F>::bytes),
+                         make_variable_value(id.flops, functor_cost_hint<F>::flops),
+                         make_variable_value(id.cus, int64_t(MI355X::impl_cu_count()))};
+  set_input_values(ctx, 8, in);
+  const int h       = range_candidate_index(pattern, heuristic);
+  VariableValue out = make_variable_value(id.out, int64_t(h < 0 ? -1 : h));
+  request_output_values(ctx, 1, &out);
+  end_context(ctx);
+  const int64_t c = out.value.int_value;
+  if (c < 0 || c >= kRangeCandidates || c == h) return heuristic;
+  return range_candidate(pattern, static_cast<int>(c), n);
+}
+
 }  // namespace Impl
 
 // ================================================================== closures
@@ -569,8 +675,9 @@ class ParallelFor<FunctorType, RangePolicy<Traits...>, MI355X> {
     const int64_t e = static_cast<int64_t>(m_policy.end());
     if (e <= b) return;
     RangeConfig c = m_has_config ? m_config
-                    : range_overrides().set[0] ? range_overrides().cfg[0]
-                                               : default_for_config(e - b);
+                    : range_overrides().set[0]
+                        ? range_overrides().cfg[0]
+                        : tuned_range_config<FunctorType>(0, e - b, 0, false, default_for_config(e - b));
     c.block       = normalise_block(c.block);
     c.items       = normalise_items(c.items);
     const void* k = range_for_kernel<FunctorType, Tag, Index>(c.items);
@@ -624,8 +731,10 @@ class ParallelReduce<CR, RangePolicy<Traits...>, MI355X> {
     unsigned* counter = inst->control;
     if constexpr (!CR::is_array) {
       RangeConfig c = m_has_config ? m_config
-                      : range_overrides().set[1] ? range_overrides().cfg[1]
-                                                 : default_reduce_config(e - b);
+                      : range_overrides().set[1]
+                          ? range_overrides().cfg[1]
+                          : tuned_range_config<typename CR::functor_type>(
+                                1, e - b, sizeof(V), std::is_floating_point<V>::value, default_reduce_config(e - b));
       c.block       = normalise_block(c.block);
       c.items       = normalise_items(c.items);
       if (e <= b) {
@@ -700,8 +809,10 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
     return;
   }
   RangeConfig c = cfg ? *cfg
-                  : range_overrides().set[2] ? range_overrides().cfg[2]
-                                             : RangeConfig{1024, 32, 0};
+                  : range_overrides().set[2]
+                      ? range_overrides().cfg[2]
+                      : tuned_range_config<typename CR::functor_type>(2, e - b, sizeof(V), std::is_floating_point<V>::value,
+                                                                     RangeConfig{1024, 32, 0});
   c.block       = normalise_block(c.block);
   const int ipt = c.items >= 64 ? 64 : c.items >= 32 ? 32 : c.items >= 16 ? 16 : 8;
   const int64_t tile   = static_cast<int64_t>(c.block) * ipt;

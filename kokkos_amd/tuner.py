@@ -1,0 +1,213 @@
+"""The learned launch-shape tuner: data collection, training and evaluation.
+
+* collect(): the runtime sweeps itself. For each dispatch family
+  (libkkmx_sweep.so) and a log-uniform set of extents, it times the 16
+  launch-shape candidates and the heuristic with HIP events. One sample is
+  one (family, extent, candidate, time) record.
+* train(): soft-target cross-entropy on the 16 candidate logits, run by the
+  HIP training kernels in libkkmx_tuner.so (kkmx_mlp_train).
+* evaluate(): regret of the MLP's arg-max and of the heuristic against the
+  exhaustive best, on held-out (family, extent) groups. Inference uses the
+  HIP forward kernel (kkmx_mlp_predict).
+* save_model() / load: the file format the tuning tool reads
+  (kkmx_tuner_load_model, or the KKMX_TUNER_MODEL environment variable).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import struct
+from pathlib import Path
+
+import numpy as np
+
+from . import KkmxError, load
+
+NF, NH, NC = 12, 64, 16
+MODEL_DIR = Path(__file__).resolve().parent / "tuner"
+MODEL_PATH = MODEL_DIR / "mlp_gfx950.bin"
+MODEL_META = MODEL_DIR / "mlp_gfx950.json"
+
+f32p = C.POINTER(C.c_float)
+
+
+def _libs():
+    S = load("kkmx_sweep")
+    T = load("kkmx_tuner")
+    S.kkmx_sweep_family_count.restype = C.c_int
+    S.kkmx_sweep_family_info.argtypes = [C.c_int] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_double)] * 2
+    S.kkmx_sweep_setup.argtypes = [C.c_int64]
+    S.kkmx_sweep_time.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, f32p]
+    S.kkmx_sweep_last_error.restype = C.c_char_p
+    T.kkmx_mlp_param_count.restype = C.c_int
+    T.kkmx_mlp_train.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_int,
+                                 f32p, f32p]
+    T.kkmx_mlp_predict.argtypes = [f32p, f32p, C.c_int, f32p]
+    T.kkmx_tuner_features.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                      C.c_int, f32p]
+    T.kkmx_tuner_load_model.argtypes = [C.c_char_p]
+    T.kkmx_tuner_stats.argtypes = [C.POINTER(C.c_uint64)] * 3
+    T.kkmx_tuner_last_error.restype = C.c_char_p
+    return S, T
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(f32p)
+
+
+def families():
+    S, _ = _libs()
+    out = []
+    for f in range(S.kkmx_sweep_family_count()):
+        p, vb, fl, fb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        by, fo = C.c_double(), C.c_double()
+        S.kkmx_sweep_family_info(f, C.byref(p), C.byref(vb), C.byref(fl), C.byref(fb), C.byref(by), C.byref(fo))
+        out.append(dict(pattern=p.value, value_bytes=vb.value, floating=fl.value, functor_bytes=fb.value,
+                        bytes=by.value, flops=fo.value))
+    return out
+
+
+def raw_features(fam: dict, extent: float, cus: int) -> np.ndarray:
+    _, T = _libs()
+    f = np.zeros(NF, np.float32)
+    T.kkmx_tuner_features(fam["pattern"], float(extent), float(fam["value_bytes"]), float(fam["functor_bytes"]),
+                          fam["floating"], fam["bytes"], fam["flops"], cus, _fp(f))
+    return f
+
+
+def collect(extents_per_family: int = 160, seed: int = 2024, lo: float = 8.0, hi: float = 27.0, reps: int = 2,
+            cus: int = 256, progress=None) -> dict:
+    """Times every family at `extents_per_family` log-uniform extents under the
+    16 candidates and the heuristic.  Returns the groups as arrays."""
+    S, _ = _libs()
+    fams = families()
+    max_n = int(2 ** hi) + 1
+    if S.kkmx_sweep_setup(max_n) != 0:
+        raise KkmxError(S.kkmx_sweep_last_error().decode())
+    rng = np.random.default_rng(seed)
+    fam_ids, extents, times, feats = [], [], [], []
+    buf = np.zeros(NC + 1, np.float32)
+    try:
+        for f in range(len(fams)):
+            ns = np.unique(np.floor(np.exp2(rng.uniform(lo, hi, extents_per_family))).astype(np.int64))
+            for n in ns:
+                if S.kkmx_sweep_time(f, int(n), reps, -1, 1, _fp(buf)) != 0:
+                    raise KkmxError(S.kkmx_sweep_last_error().decode())
+                fam_ids.append(f)
+                extents.append(int(n))
+                times.append(buf.copy())
+                feats.append(raw_features(fams[f], n, cus))
+            if progress:
+                progress(f"family {f} ({len(ns)} extents) done, {len(times)} groups", )
+    finally:
+        S.kkmx_sweep_teardown()
+    return dict(family=np.array(fam_ids, np.int32), extent=np.array(extents, np.int64),
+                times=np.array(times, np.float32), feats=np.array(feats, np.float32), families=fams)
+
+
+def soft_targets(times: np.ndarray, tau: float = 0.03) -> np.ndarray:
+    """Target distribution over the 16 candidates: softmax of -(t/t_best - 1)/tau."""
+    t = times[:, :NC].astype(np.float64)
+    rel = t / t.min(axis=1, keepdims=True) - 1.0
+    z = np.exp(-rel / tau)
+    return (z / z.sum(axis=1, keepdims=True)).astype(np.float32)
+
+
+def normaliser(feats: np.ndarray):
+    mean = feats.mean(axis=0)
+    std = feats.std(axis=0)
+    inv = np.where(std > 1e-6, 1.0 / np.maximum(std, 1e-6), 0.0)
+    return mean.astype(np.float32), inv.astype(np.float32)
+
+
+def train(feats, targets, epochs=400, batch=128, lr=3e-3, weight_decay=1e-4, seed=7):
+    _, T = _libs()
+    X = np.ascontiguousarray(feats, np.float32)
+    Y = np.ascontiguousarray(targets, np.float32)
+    params = np.zeros(T.kkmx_mlp_param_count(), np.float32)
+    loss = np.zeros(epochs, np.float32)
+    rc = T.kkmx_mlp_train(_fp(X), _fp(Y), X.shape[0], epochs, batch, lr, weight_decay, seed, 1, _fp(params), _fp(loss))
+    if rc != 0:
+        raise KkmxError(T.kkmx_tuner_last_error().decode())
+    return params, loss
+
+
+def predict(params, feats) -> np.ndarray:
+    _, T = _libs()
+    X = np.ascontiguousarray(feats, np.float32)
+    Z = np.zeros((X.shape[0], NC), np.float32)
+    if T.kkmx_mlp_predict(_fp(params), _fp(X), X.shape[0], _fp(Z)) != 0:
+        raise KkmxError(T.kkmx_tuner_last_error().decode())
+    return Z
+
+
+def regret(times: np.ndarray, choice: np.ndarray) -> np.ndarray:
+    """t(choice)/t(best) - 1 per group; choice 16 = the heuristic column."""
+    best = times[:, :NC].min(axis=1)
+    return times[np.arange(len(times)), choice] / best - 1.0
+
+
+def summary(r: np.ndarray) -> dict:
+    return dict(mean=float(r.mean()), median=float(np.median(r)), p90=float(np.quantile(r, 0.9)),
+                max=float(r.max()), within_5pct=float((r <= 0.05).mean()))
+
+
+def save_model(params, mean, inv_std, meta: dict, path: Path = MODEL_PATH, meta_path: Path = MODEL_META):
+    path.parent.mkdir(parents=True, exist_ok=True)
+    with open(path, "wb") as fh:
+        fh.write(b"KKMXMLP1")
+        fh.write(struct.pack("<4i", NF, NH, NH, NC))
+        fh.write(np.asarray(mean, "<f4").tobytes())
+        fh.write(np.asarray(inv_std, "<f4").tobytes())
+        fh.write(np.asarray(params, "<f4").tobytes())
+    meta_path.write_text(json.dumps(meta, indent=1, sort_keys=True) + "\n")
+
+
+def read_model(path: Path = MODEL_PATH):
+    raw = path.read_bytes()
+    assert raw[:8] == b"KKMXMLP1"
+    dims = struct.unpack("<4i", raw[8:24])
+    assert dims == (NF, NH, NH, NC)
+    off = 24
+    mean = np.frombuffer(raw, "<f4", NF, off)
+    inv = np.frombuffer(raw, "<f4", NF, off + 4 * NF)
+    params = np.frombuffer(raw, "<f4", offset=off + 8 * NF).copy()
+    return params, mean.copy(), inv.copy()
+
+
+def load_into_tool(path: Path = MODEL_PATH) -> None:
+    _, T = _libs()
+    if T.kkmx_tuner_load_model(str(path).encode()) != 0:
+        raise KkmxError(T.kkmx_tuner_last_error().decode())
+
+
+def tool_stats():
+    _, T = _libs()
+    a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    T.kkmx_tuner_stats(C.byref(a), C.byref(b), C.byref(c))
+    return dict(requests=a.value, answered=b.value, inferences=c.value)
+
+
+def fit_and_evaluate(data: dict, seed: int = 11, holdout: float = 0.2, **train_kw) -> dict:
+    """Split groups, train on the rest, report regret of MLP vs heuristic on
+    the held-out groups (and on all groups)."""
+    rng = np.random.default_rng(seed)
+    G = len(data["times"])
+    idx = rng.permutation(G)
+    nt = int(G * (1 - holdout))
+    tr, te = idx[:nt], idx[nt:]
+    mean, inv = normaliser(data["feats"][tr])
+    Xn = (data["feats"] - mean) * inv
+    Y = soft_targets(data["times"])
+    params, loss = train(Xn[tr], Y[tr], **train_kw)
+    Z = predict(params, Xn)
+    choice = Z.argmax(axis=1)
+    r_mlp = regret(data["times"], choice)
+    r_heur = regret(data["times"], np.full(G, NC))
+    res = dict(
+        groups=G, samples=int(G * NC), train_groups=int(nt), test_groups=int(G - nt),
+        loss_first=float(loss[0]), loss_last=float(loss[-1]),
+        test=dict(mlp=summary(r_mlp[te]), heuristic=summary(r_heur[te])),
+        train=dict(mlp=summary(r_mlp[tr]), heuristic=summary(r_heur[tr])),
+    )
+    return dict(params=params, mean=mean, inv_std=inv, loss=loss, report=res, choice=choice)

@@ -188,3 +188,81 @@ def test_compensated_f64_scan_against_fsum(oracle):
     naive = np.concatenate([[0.0], np.cumsum(x)[:-1]])
     worst = max(abs(naive[i] - math.fsum(x[:i])) / (np.spacing(abs(math.fsum(x[:i]))) + 1e-300) for i in range(1, 3000, 7))
     assert worst > 2
+
+
+# ---------------------------------------------------------------- MDRange / Team / atomics restatements
+def test_stencil7_constant_field_and_brute_force(oracle):
+    """A constant field maps to (c0 + 6 c1) times itself in the interior and
+    the halo is untouched; a direct triple loop agrees on a random grid."""
+    n, c0, c1 = 6, 0.4, 0.1
+    m = n + 2
+    u = np.full(m**3, 2.0)
+    v = oracle.stencil7(u, n, c0, c1, 1)
+    V = v.reshape(m, m, m)
+    assert np.allclose(V[1:-1, 1:-1, 1:-1], 2.0 * (c0 + 6 * c1), rtol=0, atol=1e-15)
+    assert np.all(V[0] == 2.0) and np.all(V[:, :, -1] == 2.0)
+    u = oracle.fill_uniform_f64(m**3, 5, 0.0, 1.0)
+    U = u.reshape(m, m, m)  # index [k][j][i], i fastest
+    ref = U.copy()
+    for k in range(1, n + 1):
+        for j in range(1, n + 1):
+            for i in range(1, n + 1):
+                nb = U[k, j, i - 1] + U[k, j, i + 1] + U[k, j - 1, i] + U[k, j + 1, i] + U[k - 1, j, i] + U[k + 1, j, i]
+                ref[k, j, i] = c0 * U[k, j, i] + c1 * nb
+    got = oracle.stencil7(u, n, c0, c1, 4).reshape(m, m, m)
+    assert np.max(np.abs(got - ref)) <= 1e-15
+    # a transposed neighbour (wrong stride) is caught by the same check
+    bad = ref.copy()
+    bad[1:-1, 1:-1, 1:-1] += c1 * (U[1:-1, 1:-1, 2:] - U[2:, 1:-1, 1:-1])
+    assert np.max(np.abs(got - bad)) > 1e-3
+
+
+def test_csr_shape_and_spmv_against_dense(oracle):
+    nrows, ncols = 37, 53
+    rp, col, val = oracle.csr(nrows, ncols, 4, 9, 6)
+    nnz_row = np.diff(rp)
+    assert rp[0] == 0 and np.all(nnz_row >= 4) and np.all(nnz_row <= 9)
+    assert np.all((col >= 0) & (col < ncols)) and np.all((val >= -1) & (val < 1))
+    A = np.zeros((nrows, ncols))
+    for r in range(nrows):
+        for k in range(rp[r], rp[r + 1]):
+            A[r, col[k]] += val[k]
+    x = oracle.fill_uniform_f64(ncols, 606, -1.0, 1.0)
+    y, absrow = oracle.spmv(rp, col, val, x, 3)
+    assert np.allclose(y, A @ x, rtol=0, atol=1e-14)
+    absref = np.array([sum(abs(val[k] * x[col[k]]) for k in range(rp[r], rp[r + 1])) for r in range(nrows)])
+    assert np.allclose(absrow, absref, rtol=1e-14, atol=0)
+
+
+def test_gups_brute_force(oracle):
+    T, U = 64, 1000
+    s = oracle.xorshift64star(2 * U, 7)
+    acc = [0] * T
+    for u in range(U):
+        t = int(s[2 * u]) & (T - 1)
+        acc[t] = (acc[t] + int(s[2 * u + 1])) % (1 << 64)  # two's-complement wrap
+    want = np.array(acc, np.uint64).view(np.int64)
+    assert np.array_equal(oracle.gups(T, s, U, 4), want)
+
+
+def test_histogram_and_zipf_distribution(oracle):
+    nb = 64
+    k = oracle.zipf_keys(200000, 1.1, nb, 8)
+    assert k.min() >= 0 and k.max() < nb
+    h = oracle.histogram(k, nb)
+    assert np.array_equal(h, np.bincount(k, minlength=nb))
+    pmf = np.arange(1, nb + 1, dtype=np.float64) ** -1.1
+    pmf /= pmf.sum()
+    # every bin within 5 binomial standard deviations of the Zipf(1.1) pmf
+    sd = np.sqrt(k.size * pmf * (1 - pmf))
+    assert np.all(np.abs(h - k.size * pmf) <= 5 * sd + 1)
+
+
+def test_scatter_against_sorted_sums(oracle):
+    n, nb = 20000, 97
+    v = oracle.fill_uniform_f64(n, 9, 0.0, 1.0)
+    i = oracle.fill_uniform_i32(n, 9 ^ 0xABCDEF, 0, nb - 1)
+    b, a = oracle.scatter_f64(v, i, nb)
+    ref = np.array([math.fsum(v[i == k]) for k in range(nb)])
+    assert np.max(np.abs(b - ref) / np.maximum(ref, 1e-300)) <= 2e-16
+    assert np.allclose(a, ref)  # inputs are non-negative: sum of |v| = sum of v
