@@ -126,6 +126,10 @@ int kkmx_batch_create(kkmx_batch** out);
 int kkmx_batch_destroy(kkmx_batch* b);
 int kkmx_batch_add(kkmx_batch* b, int op, int type, const void* in_dev, int64_t n, void* out_dev, void* result_dev,
                    double alpha, double beta);
+/* Tile shape of the fused kernel, before plan: 0 = 256 lanes x 16 single
+ * elements, 1 = 256 lanes x 4 runs of 4 (default), 2 = 256 x 8 runs of 4,
+ * 3 = 512 x 4 runs of 4. */
+int kkmx_batch_set_shape(kkmx_batch* b, int shape);
 int kkmx_batch_plan(kkmx_batch* b, int64_t* tiles_out);
 int kkmx_batch_run(kkmx_batch* b, int mode, int steps);
 
@@ -138,6 +142,9 @@ int kkmx_tools_unload(void);
 
 /* ---- launch-shape overrides (what the tuning hooks set), -1 = heuristic ---- */
 int kkmx_set_range_config(int pattern, int block, int items, int max_blocks);
+/* MDRange parallel_for coarsening: tiles per workgroup along the slowest tile
+ * dimension (0 = heuristic). */
+int kkmx_set_mdrange_depth(int depth);
 
 #ifdef __cplusplus
 }

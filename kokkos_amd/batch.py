@@ -13,7 +13,7 @@ MODE_PLAIN, MODE_FUSED, MODE_FUSED_GRAPH, MODE_PLAIN_GRAPH = range(4)
 
 
 class BatchRun:
-    def __init__(self, wl: dict):
+    def __init__(self, wl: dict, shape: int | None = None):
         P = load("kkmx_paths")
         self.P, self.wl = P, wl
         tot = wl["in_tot"]
@@ -42,6 +42,8 @@ class BatchRun:
             if op == OP_SCAN_EXCL:
                 dst = self.out["f64" if t == T_F64 else "i64"].ptr + int(wl["out_off"][i]) * 8
             check(P.kkmx_batch_add(h, op, t, src, n, dst, self.res.ptr + 8 * i, 1.0, 0.0), P)
+        if shape is not None:
+            check(P.kkmx_batch_set_shape(h, shape), P)
         tiles = C.c_int64()
         check(P.kkmx_batch_plan(h, C.byref(tiles)), P)
         self.tiles = tiles.value

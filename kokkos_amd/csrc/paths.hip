@@ -706,6 +706,11 @@ int kkmx_scatter_add_f64(const double* vals, const int32_t* idx, int64_t n, doub
   });
 }
 
+int kkmx_set_mdrange_depth(int depth) {
+  Kokkos::Impl::md_depth_override() = depth > 0 ? depth : 0;
+  return 0;
+}
+
 int kkmx_set_range_config(int pattern, int block, int items, int max_blocks) {
   if (pattern < 0 || pattern > 2) return reject("kkmx_set_range_config: pattern must be 0, 1 or 2");
   auto& o = Kokkos::Impl::range_overrides();
@@ -747,6 +752,10 @@ int kkmx_batch_add(kkmx_batch* b, int op, int type, const void* in, int64_t n, v
     b->batch.add(Kokkos::Experimental::BatchOp(op), Kokkos::Experimental::BatchType(type), in, n, out, result, alpha,
                  beta);
   });
+}
+int kkmx_batch_set_shape(kkmx_batch* b, int shape) {
+  if (!b) return reject("kkmx_batch_set_shape: null handle");
+  return guarded([&] { b->batch.set_shape(shape); });
 }
 int kkmx_batch_plan(kkmx_batch* b, int64_t* tiles_out) {
   if (!b) return reject("kkmx_batch_plan: null handle");

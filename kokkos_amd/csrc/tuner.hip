@@ -220,13 +220,28 @@ struct Model {
   float mean[F] = {}, inv_std[F] = {};
 };
 
+struct Key {
+  int64_t v[8];
+  bool operator==(const Key& o) const { return std::memcmp(v, o.v, sizeof(v)) == 0; }
+};
+struct KeyHash {
+  size_t operator()(const Key& k) const {
+    uint64_t h = 0x9E3779B97F4A7C15ull;
+    for (int64_t x : k.v) h = (h ^ uint64_t(x)) * 0x100000001B3ull;
+    return size_t(h ^ (h >> 29));
+  }
+};
+enum Field { kPattern, kExtent, kValueBytes, kFunctorBytes, kFloating, kBytes, kFlops, kCus };
+
 struct ToolState {
   std::mutex mtx;
   Model model;
-  std::unordered_map<size_t, std::string> input_names;
+  // Declared input ids -> feature field (looked up per request, so no string
+  // compares on the dispatch path).
+  std::unordered_map<size_t, int> input_field;
   size_t range_launch_id = 0;
   bool have_range_var    = false;
-  std::map<std::vector<int64_t>, int> cache;
+  std::unordered_map<Key, int, KeyHash> cache;
   DevBuf* dev_params = nullptr;
   mx_stream_t stream = nullptr;
   uint64_t requests = 0, answered = 0, inferences = 0;
@@ -467,9 +482,15 @@ void kokkosp_request_tool_settings(const uint32_t, Kokkos_Tools_ToolSettings* se
 }
 
 void kokkosp_declare_input_type(const char* name, const size_t id, Kokkos_Tools_VariableInfo*) {
+  static const std::pair<const char*, int> names[] = {
+      {"kokkos.mi355x.pattern", kPattern},         {"kokkos.mi355x.extent", kExtent},
+      {"kokkos.mi355x.value_bytes", kValueBytes},   {"kokkos.mi355x.functor_bytes", kFunctorBytes},
+      {"kokkos.mi355x.floating", kFloating},        {"kokkos.mi355x.bytes_per_index", kBytes},
+      {"kokkos.mi355x.flops_per_index", kFlops},    {"kokkos.mi355x.cu_count", kCus}};
   auto& s = tool();
   std::lock_guard<std::mutex> lk(s.mtx);
-  s.input_names[id] = name;
+  for (const auto& nf : names)
+    if (std::strcmp(name, nf.first) == 0) s.input_field[id] = nf.second;
 }
 void kokkosp_declare_output_type(const char* name, const size_t id, Kokkos_Tools_VariableInfo*) {
   auto& s = tool();
@@ -492,30 +513,25 @@ void kokkosp_request_values(const size_t, const size_t n_in, const Kokkos_Tools_
     if (!s.model.loaded) continue;  // the runtime's heuristic value stands
     Inputs x;
     for (size_t i = 0; i < n_in; ++i) {
-      auto it = s.input_names.find(in[i].type_id);
-      if (it == s.input_names.end()) continue;
-      const std::string& nm = it->second;
-      const auto& v         = in[i].value;
-      if (nm == "kokkos.mi355x.pattern") x.pattern = int(v.int_value);
-      else if (nm == "kokkos.mi355x.extent") x.extent = double(v.int_value);
-      else if (nm == "kokkos.mi355x.value_bytes") x.value_bytes = double(v.int_value);
-      else if (nm == "kokkos.mi355x.functor_bytes") x.functor_bytes = double(v.int_value);
-      else if (nm == "kokkos.mi355x.floating") x.floating = double(v.int_value);
-      else if (nm == "kokkos.mi355x.bytes_per_index") x.bytes = v.double_value;
-      else if (nm == "kokkos.mi355x.flops_per_index") x.flops = v.double_value;
-      else if (nm == "kokkos.mi355x.cu_count") x.cus = double(v.int_value);
+      auto it = s.input_field.find(in[i].type_id);
+      if (it == s.input_field.end()) continue;
+      const auto& v = in[i].value;
+      switch (it->second) {
+        case kPattern: x.pattern = int(v.int_value); break;
+        case kExtent: x.extent = double(v.int_value); break;
+        case kValueBytes: x.value_bytes = double(v.int_value); break;
+        case kFunctorBytes: x.functor_bytes = double(v.int_value); break;
+        case kFloating: x.floating = double(v.int_value); break;
+        case kBytes: x.bytes = v.double_value; break;
+        case kFlops: x.flops = v.double_value; break;
+        default: x.cus = double(v.int_value); break;
+      }
     }
     // Cache key: everything but the extent, which is bucketed in quarter
     // octaves.
     const int64_t bucket = int64_t(std::floor(std::log2(std::max(x.extent, 1.0)) * 4.0));
-    std::vector<int64_t> key = {x.pattern,
-                                int64_t(x.value_bytes),
-                                int64_t(x.functor_bytes),
-                                int64_t(x.floating),
-                                int64_t(x.bytes * 16),
-                                int64_t(x.flops * 16),
-                                int64_t(x.cus),
-                                bucket};
+    const Key key{{x.pattern, int64_t(x.value_bytes), int64_t(x.functor_bytes), int64_t(x.floating),
+                   int64_t(x.bytes * 16), int64_t(x.flops * 16), int64_t(x.cus), bucket}};
     int c;
     auto hit = s.cache.find(key);
     if (hit != s.cache.end()) {

@@ -9,7 +9,7 @@
 // (core/src/HIP/Kokkos_HIP_Graph_Impl.hpp:82-221).  For thousands of small
 // ops launch latency dominates; here the whole batch is one launch:
 //
-//  * plan(): every op is cut into tiles of kTileElems elements; the tiles of
+//  * plan(): every op is cut into tiles (BatchShape); the tiles of
 //    all ops form one global sequence (op 0's tiles, then op 1's, ...) and a
 //    tile -> op table is uploaded with the op descriptors.  All device
 //    memory is allocated here; submit() allocates and clears nothing, so it
@@ -60,9 +60,17 @@ struct alignas(16) BatchDesc {
   int32_t op, type;
 };
 
-constexpr int kBatchBlock = 256;
-constexpr int kBatchIPT   = 16;
-constexpr int64_t kTileElems = int64_t(kBatchBlock) * kBatchIPT;
+// Tile shapes of the fused kernel: `block` lanes, each holding `k`
+// consecutive elements in each of `r` rounds, so a tile is block*k*r
+// elements.  k = 4 lets the compiler move each lane's run with one 16-byte
+// (int32) or two 16-byte (8-byte types) accesses.
+struct BatchShape {
+  int block, k, r;
+  constexpr int64_t tile() const { return int64_t(block) * k * r; }
+};
+constexpr int kBatchShapes = 4;
+constexpr BatchShape kBatchShape[kBatchShapes] = {{256, 1, 16}, {256, 4, 4}, {256, 4, 8}, {512, 4, 4}};
+constexpr int kBatchDefaultShape = 1;
 
 template <class Acc>
 __device__ __forceinline__ Acc batch_identity(int op) {
@@ -88,65 +96,107 @@ KOKKOS_FORCEINLINE_FUNCTION Acc batch_bits(int64_t b) {
   return a;
 }
 
+template <class T>
+using GlobalPtr = __attribute__((address_space(1))) T*;
+
+template <int K, class T>
+__device__ __forceinline__ void batch_store(GlobalPtr<T> p, const T& v) {
+  if constexpr (K == 1) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 struct BatchLds {
   LdsSlots16<int64_t> wave;  // per-wave totals
   LdsSlots16<int64_t> red;   // look-back / block reduction scratch
   int pos;
 };
 
-// One tile of one op.  Wavefront w of the workgroup owns kBatchIPT rounds of
-// 64 consecutive elements: element (tile, w, r, lane) is
-// tile_base + w*64*IPT + r*64 + lane, so every load and store of a round is
-// one contiguous 64-element run.
-template <class In, class Acc>
-__device__ __forceinline__ void batch_tile(const BatchDesc& d, int64_t tile, uint32_t epoch, uint64_t* status, BatchLds& lds) {
+// One tile of one op.  Wavefront w owns R rounds of 64*K consecutive
+// elements; in round r lane l holds elements
+//   tile_base + w*64*K*R + r*64*K + l*K + [0, K),
+// so each round is one contiguous run and each lane's part of it is K
+// consecutive elements.
+template <class In, class Acc, int BLOCK, int K, int R>
+__device__ __forceinline__ void batch_tile(const BatchDesc& d, int64_t tile, uint32_t epoch, uint64_t* status,
+                                           BatchLds& lds) {
   using Experimental::BatchOp;
-  const int op         = d.op;
+  constexpr int64_t TILE = int64_t(BLOCK) * K * R;
+  constexpr int nw       = BLOCK / 64;
+  const int op           = d.op;
   const BatchJoin join{op};
-  const Acc identity   = batch_identity<Acc>(op);
-  const int64_t local  = tile - d.tile0;
-  const int w          = threadIdx.x >> 6;
-  const int l          = wave::lane();
-  constexpr int nw     = kBatchBlock / 64;
-  const int64_t base   = local * kTileElems + int64_t(w) * (64 * kBatchIPT) + l;
-  const In* __restrict__ in = static_cast<const In*>(d.in);
-  const int64_t n      = d.n;
+  const Acc identity  = batch_identity<Acc>(op);
+  const int64_t local = tile - d.tile0;
+  const int w         = threadIdx.x >> 6;
+  const int l         = wave::lane();
+  const int64_t base  = local * TILE + int64_t(w) * (64 * K * R) + int64_t(l) * K;
+  // The descriptors hold generic pointers; the arrays are device (global)
+  // memory, so address them as such (global_load/store rather than flat).
+  const GlobalPtr<const In> in = (GlobalPtr<const In>)(d.in);
+  const int64_t n              = d.n;
+  const bool full              = (local + 1) * TILE <= n;
 
-  Acc v[kBatchIPT];
+  // Inputs are read once and outputs never re-read: single-element lanes
+  // use non-temporal accesses; K-element lanes use plain ones, which the
+  // compiler merges into 16-byte accesses.
+  auto ld = [&](int64_t i) -> Acc {
+    if constexpr (K == 1) return static_cast<Acc>(__builtin_nontemporal_load(in + i));
+    else return static_cast<Acc>(in[i]);
+  };
+  Acc v[R][K];
+  if (full) {
 #pragma unroll
-  for (int r = 0; r < kBatchIPT; ++r) {
-    const int64_t i = base + r * 64;
-    v[r]            = i < n ? static_cast<Acc>(__builtin_nontemporal_load(in + i)) : identity;
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[r][k] = ld(base + r * 64 * K + k);
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int64_t i = base + r * 64 * K + k;
+        v[r][k]         = i < n ? ld(i) : identity;
+      }
   }
 
   if (op == int(BatchOp::Axpb)) {
     const Acc a = batch_bits<Acc>(d.alpha_bits), b = batch_bits<Acc>(d.beta_bits);
-    Acc* __restrict__ out = static_cast<Acc*>(d.out);
+    const GlobalPtr<Acc> out = (GlobalPtr<Acc>)(d.out);
 #pragma unroll
-    for (int r = 0; r < kBatchIPT; ++r) {
-      const int64_t i = base + r * 64;
-      if (i < n) __builtin_nontemporal_store(a * v[r] + b, out + i);
-    }
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int64_t i = base + r * 64 * K + k;
+        if (full || i < n) batch_store<K>(out + i, a * v[r][k] + b);
+      }
     return;
   }
 
   ScanTileState<Acc> st;
-  st.words = status;
-  Acc* s_wave = reinterpret_cast<Acc*>(lds.wave.get());
-  Acc* s_red  = reinterpret_cast<Acc*>(lds.red.get());
+  st.words           = status;
+  Acc* s_wave        = reinterpret_cast<Acc*>(lds.wave.get());
+  Acc* s_red         = reinterpret_cast<Acc*>(lds.red.get());
   const bool is_scan = op == int(BatchOp::ScanExclusive) || op == int(BatchOp::ScanInclusive);
 
-  // Wavefront total, rounds combined in order.
+  // Lane totals per round; for scans their wave-inclusive scans are kept for
+  // the output pass.  The wavefront total combines the rounds in order.
+  Acc inc[R];
   Acc wave_total = identity;
   if (is_scan) {
 #pragma unroll
-    for (int r = 0; r < kBatchIPT; ++r) join(wave_total, wave::reduce(v[r], identity, join));
-  } else {
-    // Reductions: lane-sequential over its rounds first (fewer cross-lane
-    // steps), then one ordered wave reduction.
-    Acc mine = v[0];
+    for (int r = 0; r < R; ++r) {
+      Acc s = v[r][0];
 #pragma unroll
-    for (int r = 1; r < kBatchIPT; ++r) join(mine, v[r]);
+      for (int k = 1; k < K; ++k) join(s, v[r][k]);
+      inc[r] = wave::inclusive_scan(s, identity, join);
+      join(wave_total, wave::readlane(inc[r], 63));
+    }
+  } else {
+    Acc mine = v[0][0];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (r | k) join(mine, v[r][k]);
     wave_total = wave::reduce(mine, identity, join);
   }
   if (l == 0) s_wave[w] = wave_total;
@@ -181,28 +231,43 @@ __device__ __forceinline__ void batch_tile(const BatchDesc& d, int64_t tile, uin
   }
   if (!is_scan) return;
 
-  const bool exclusive = op == int(BatchOp::ScanExclusive);
-  Acc* __restrict__ out = static_cast<Acc*>(d.out);
+  const bool exclusive     = op == int(BatchOp::ScanExclusive);
+  const GlobalPtr<Acc> out = (GlobalPtr<Acc>)(d.out);
   Acc running           = prefix;
   join(running, wpre);
 #pragma unroll
-  for (int r = 0; r < kBatchIPT; ++r) {
-    const Acc inc = wave::inclusive_scan(v[r], identity, join);
-    Acc o         = running;
-    join(o, exclusive ? wave::shift_up(inc, identity) : inc);
-    join(running, wave::readlane(inc, 63));
-    const int64_t i = base + r * 64;
-    if (i < n) __builtin_nontemporal_store(o, out + i);
+  for (int r = 0; r < R; ++r) {
+    Acc e = running;  // exclusive prefix of the lane's first element
+    join(e, wave::shift_up(inc[r], identity));
+    join(running, wave::readlane(inc[r], 63));
+    Acc o[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (exclusive) {
+        o[k] = e;
+        join(e, v[r][k]);
+      } else {
+        join(e, v[r][k]);
+        o[k] = e;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t i = base + r * 64 * K + k;
+      if (full || i < n) batch_store<K>(out + i, o[k]);
+    }
   }
 }
 
 // control[0..1]: {tile counter (low 32), epoch (high 32)}; control[2]: number
-// of workgroups that have left.  (A template only so that the definition in
-// a header has vague linkage.)
-template <int Block = kBatchBlock>
-__global__ __launch_bounds__(Block) void k_batch(const BatchDesc* __restrict__ ops,
-                                                       const int32_t* __restrict__ tile_op, int64_t total_tiles,
-                                                       uint64_t* status, unsigned* control) {
+// of workgroups that have left.  A workgroup claims its next tile only when
+// it is ready to start it: a claim made earlier (prefetched) would hold a
+// tile whose aggregate later tiles' look-backs wait for, until the current
+// tile is done, and measured 8x slower.
+template <int BLOCK, int K, int R>
+__global__ __launch_bounds__(BLOCK) void k_batch(const BatchDesc* __restrict__ ops,
+                                                 const int32_t* __restrict__ tile_op, int64_t total_tiles,
+                                                 uint64_t* status, unsigned* control) {
   __shared__ BatchLds lds;
   __shared__ uint64_t s_claim;
   uint64_t* claim_word = reinterpret_cast<uint64_t*>(control);
@@ -218,9 +283,13 @@ __global__ __launch_bounds__(Block) void k_batch(const BatchDesc* __restrict__ o
     if (tile >= total_tiles) break;
     const BatchDesc d = ops[tile_op[tile]];
     switch (d.type) {
-      case int(Experimental::BatchType::Int32): batch_tile<int32_t, int64_t>(d, tile, epoch, status, lds); break;
-      case int(Experimental::BatchType::Int64): batch_tile<int64_t, int64_t>(d, tile, epoch, status, lds); break;
-      default: batch_tile<double, double>(d, tile, epoch, status, lds); break;
+      case int(Experimental::BatchType::Int32):
+        batch_tile<int32_t, int64_t, BLOCK, K, R>(d, tile, epoch, status, lds);
+        break;
+      case int(Experimental::BatchType::Int64):
+        batch_tile<int64_t, int64_t, BLOCK, K, R>(d, tile, epoch, status, lds);
+        break;
+      default: batch_tile<double, double, BLOCK, K, R>(d, tile, epoch, status, lds); break;
     }
   }
   if (threadIdx.x == 0) {
@@ -232,6 +301,20 @@ __global__ __launch_bounds__(Block) void k_batch(const BatchDesc* __restrict__ o
       __hip_atomic_store(claim_word, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     }
+  }
+}
+
+template <int S>
+inline const void* batch_kernel() {
+  constexpr BatchShape sh = kBatchShape[S];
+  return reinterpret_cast<const void*>(&k_batch<sh.block, sh.k, sh.r>);
+}
+inline const void* batch_kernel(int shape) {
+  switch (shape) {
+    case 0: return batch_kernel<0>();
+    case 1: return batch_kernel<1>();
+    case 2: return batch_kernel<2>();
+    default: return batch_kernel<3>();
   }
 }
 
@@ -411,11 +494,13 @@ class BatchDispatch {
   // one tile so their result (the identity) is still written.
   void plan() {
     if (m_planned) return;
-    int64_t t = 0;
+    const Impl::BatchShape shape = Impl::kBatchShape[m_shape];
+    const int64_t tile_elems     = shape.tile();
+    int64_t t                    = 0;
     std::vector<int32_t> tile_op;
     for (size_t o = 0; o < m_ops.size(); ++o) {
       auto& d  = m_ops[o];
-      d.ntiles = d.n > 0 ? (d.n + Impl::kTileElems - 1) / Impl::kTileElems : 1;
+      d.ntiles = d.n > 0 ? (d.n + tile_elems - 1) / tile_elems : 1;
       d.tile0  = t;
       for (int64_t k = 0; k < d.ntiles; ++k) tile_op.push_back(static_cast<int32_t>(o));
       t += d.ntiles;
@@ -438,9 +523,9 @@ class BatchDispatch {
     unsigned ctrl[16] = {};
     ctrl[1]           = 1;  // epoch 1: zeroed status words are never valid
     KKMX_SAFE_CALL(mx_memcpy_async(m_control, ctrl, sizeof(ctrl), s));
-    const void* k = reinterpret_cast<const void*>(&Impl::k_batch<>);
+    const void* k = Impl::batch_kernel(m_shape);
     int per_cu    = 0;
-    KKMX_SAFE_CALL(mx_occupancy(k, Impl::kBatchBlock, 0, &per_cu));
+    KKMX_SAFE_CALL(mx_occupancy(k, shape.block, 0, &per_cu));
     if (per_cu < 1) per_cu = 1;
     const int64_t cap = int64_t(MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256) * per_cu;
     m_grid            = static_cast<unsigned>(std::min<int64_t>(cap, m_tiles));
@@ -458,7 +543,7 @@ class BatchDispatch {
     int64_t total              = m_tiles;
     uint64_t* status           = static_cast<uint64_t*>(m_status);
     unsigned* control          = static_cast<unsigned*>(m_control);
-    Impl::mx_launch_kernel(reinterpret_cast<const void*>(&Impl::k_batch<>), m_grid, Impl::kBatchBlock, 0,
+    Impl::mx_launch_kernel(Impl::batch_kernel(m_shape), m_grid, Impl::kBatchShape[m_shape].block, 0,
                            ex.stream(), ops, tile_op, total, status, control);
   }
   void submit() const { submit(m_exec); }
@@ -490,6 +575,15 @@ class BatchDispatch {
     });
   }
 
+  // Tile shape of the fused kernel (Impl::kBatchShape), before plan().
+  void set_shape(int shape) {
+    if (m_planned) Impl::throw_runtime_exception("Kokkos::Experimental::BatchDispatch: set_shape() after plan()");
+    if (shape < 0 || shape >= Impl::kBatchShapes)
+      Impl::throw_runtime_exception("Kokkos::Experimental::BatchDispatch: bad tile shape");
+    m_shape = shape;
+  }
+  int shape() const { return m_shape; }
+
   size_t size() const { return m_ops.size(); }
   int64_t tiles() const { return m_tiles; }
   int64_t elements() const { return m_elements; }
@@ -509,6 +603,7 @@ class BatchDispatch {
   MI355X m_exec;
   std::vector<Impl::BatchDesc> m_ops;
   bool m_planned     = false;
+  int m_shape        = Impl::kBatchDefaultShape;
   int64_t m_tiles    = 0;
   int64_t m_elements = 0;
   unsigned m_grid    = 0;

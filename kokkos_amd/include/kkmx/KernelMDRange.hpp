@@ -28,6 +28,13 @@ struct MDLaunch {
   int64_t ntiles;
   int tile_points;      // product of tile extents
   int xcds;             // XCD count used by the tile remap (1 = identity)
+  // Coarsening of parallel_for: each workgroup runs `depth` consecutive tiles
+  // along the slowest tile dimension (loop_dim), so the planes a tile shares
+  // with its successor are still in the CU's L1 and the XCD's L2 when it is
+  // reached.  groups = number of workgroup tiles.
+  int depth;
+  int loop_dim;
+  int64_t groups;
 };
 
 __device__ __forceinline__ int64_t md_logical_tile(int64_t b, int64_t ntiles, int xcds) {
@@ -72,6 +79,46 @@ __device__ __forceinline__ bool md_point(const MDLaunch& p, int64_t t, int lane,
   return in;
 }
 
+// Point of lane `lane` in repeat q of workgroup tile g (coarsened launch).
+template <unsigned R, bool InnerLeft, bool OuterLeft>
+__device__ __forceinline__ bool md_point_group(const MDLaunch& p, int64_t g, int q, int lane, int64_t* idx) {
+  int64_t tc[6];
+  const int ld = p.loop_dim;
+  if (OuterLeft) {
+    for (unsigned d = 0; d < R; ++d) {
+      const int64_t te = (int(d) == ld) ? (p.tile_end[d] + p.depth - 1) / p.depth : p.tile_end[d];
+      tc[d]            = g % te;
+      g /= te;
+    }
+  } else {
+    for (int d = int(R) - 1; d >= 0; --d) {
+      const int64_t te = (d == ld) ? (p.tile_end[d] + p.depth - 1) / p.depth : p.tile_end[d];
+      tc[d]            = g % te;
+      g /= te;
+    }
+  }
+  tc[ld] = tc[ld] * p.depth + q;
+  if (tc[ld] >= p.tile_end[ld]) return false;
+  int64_t loc = lane;
+  bool in     = true;
+  if (InnerLeft) {
+    for (unsigned d = 0; d < R; ++d) {
+      const int64_t o = loc % p.tile[d];
+      loc /= p.tile[d];
+      idx[d] = p.lower[d] + tc[d] * p.tile[d] + o;
+      in &= idx[d] < p.upper[d];
+    }
+  } else {
+    for (int d = int(R) - 1; d >= 0; --d) {
+      const int64_t o = loc % p.tile[d];
+      loc /= p.tile[d];
+      idx[d] = p.lower[d] + tc[d] * p.tile[d] + o;
+      in &= idx[d] < p.upper[d];
+    }
+  }
+  return in;
+}
+
 template <class Tag, class Index, unsigned R, class F, class... Extra>
 __device__ __forceinline__ void md_invoke(const F& f, const int64_t* i, Extra&... extra) {
   if constexpr (R == 2) call_functor<Tag>(f, Index(i[0]), Index(i[1]), extra...);
@@ -86,10 +133,12 @@ __device__ __forceinline__ void md_invoke(const F& f, const int64_t* i, Extra&..
 template <class F, class Tag, class Index, unsigned R, bool InnerLeft, bool OuterLeft>
 __global__ __launch_bounds__(1024) void k_md_for(const F f, const MDLaunch p) {
   if (static_cast<int>(threadIdx.x) >= p.tile_points) return;
-  for (int64_t b = blockIdx.x; b < p.ntiles; b += gridDim.x) {
-    const int64_t t = (gridDim.x >= p.ntiles) ? md_logical_tile(b, p.ntiles, p.xcds) : b;
-    int64_t idx[6];
-    if (md_point<R, InnerLeft, OuterLeft>(p, t, threadIdx.x, idx)) md_invoke<Tag, Index, R>(f, idx);
+  for (int64_t b = blockIdx.x; b < p.groups; b += gridDim.x) {
+    const int64_t g = (gridDim.x >= p.groups) ? md_logical_tile(b, p.groups, p.xcds) : b;
+    for (int q = 0; q < p.depth; ++q) {
+      int64_t idx[6];
+      if (md_point_group<R, InnerLeft, OuterLeft>(p, g, q, threadIdx.x, idx)) md_invoke<Tag, Index, R>(f, idx);
+    }
   }
 }
 
@@ -158,8 +207,33 @@ MDLaunch md_launch(const Policy& p) {
   }
   for (unsigned d = 0; d < R; ++d)
     if (l.upper[d] <= l.lower[d]) l.ntiles = 0;
-  l.xcds = MI355X::impl_props().xcc_count > 0 ? MI355X::impl_props().xcc_count : 1;
+  l.xcds     = MI355X::impl_props().xcc_count > 0 ? MI355X::impl_props().xcc_count : 1;
+  l.loop_dim = (Policy::outer_direction == Iterate::Left) ? int(R) - 1 : 0;
+  l.depth    = 1;
+  l.groups   = l.ntiles;
   return l;
+}
+
+// Tiles per workgroup along the slowest tile dimension for parallel_for.
+// Heuristic: 8 while at least 16 workgroup tiles per resident workgroup slot
+// remain; a tuning override (md_depth_override() > 0) replaces it.
+inline int& md_depth_override() {
+  static int d = 0;
+  return d;
+}
+inline void md_set_depth(MDLaunch& l, int block) {
+  const int64_t te = l.tile_end[l.loop_dim];
+  int depth        = md_depth_override();
+  if (depth <= 0) {
+    const int cus       = MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256;
+    const int64_t slots = int64_t(cus) * (2048 / (block > 0 ? block : 64));
+    depth               = 8;
+    while (depth > 1 && l.ntiles / depth < 16 * slots) depth /= 2;
+  }
+  if (depth > te) depth = int(te > 0 ? te : 1);
+  if (depth < 1) depth = 1;
+  l.depth  = depth;
+  l.groups = l.ntiles / te * ((te + depth - 1) / depth);
 }
 
 template <class F, class... Traits>
@@ -174,7 +248,8 @@ class ParallelFor<F, MDRangePolicy<Traits...>, MI355X> {
     MDLaunch l = md_launch(m_p);
     if (l.ntiles == 0) return;
     const int block = ((l.tile_points + 63) / 64) * 64;
-    int64_t grid    = l.ntiles > 0x7fffffff ? 0x7fffffff : l.ntiles;
+    md_set_depth(l, block);
+    int64_t grid    = l.groups > 0x7fffffff ? 0x7fffffff : l.groups;
     MI355XInstance* inst = m_p.space().impl_internal_space_instance();
     F f = m_f;
     const void* k = reinterpret_cast<const void*>(

@@ -537,16 +537,30 @@ inline RangeOverrides& range_overrides() {
 
 // Heuristic launch shapes (the learned tuner may replace them through the
 // tuning hooks).
-// Measured on MI355X at 2^28 doubles: two indices per lane at 256 lanes per
-// workgroup keeps STREAM at 6.2-6.4 TB/s; reductions want four loads in flight
-// per lane and four resident workgroups per CU.
-inline RangeConfig default_for_config(int64_t) { return RangeConfig{256, 2, 0}; }
+// Default launch shapes, per octave of the extent, from the launch-shape
+// sweep on MI355X (20 dispatch families x 160 extents x 16 candidates): the
+// candidate with the least mean slowdown against the per-case best.
+inline RangeConfig default_for_config(int64_t n) {
+  return n >= (int64_t(1) << 23) ? RangeConfig{128, 4, 0} : RangeConfig{256, 2, 0};
+}
 inline RangeConfig default_reduce_config(int64_t n) {
-  RangeConfig c{256, 4, 0};
+  RangeConfig c = n < (int64_t(1) << 15)   ? RangeConfig{256, 4, 0}
+                  : n < (int64_t(1) << 18) ? RangeConfig{256, 8, 0}
+                  : n < (int64_t(1) << 21) ? RangeConfig{1024, 8, 0}
+                                           : RangeConfig{1024, 4, 0};
+  // At most as many resident lanes as four 256-lane workgroups per CU.
   const int cus = MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256;
-  c.max_blocks  = cus * 4;
+  c.max_blocks  = cus * 4 * 256 / c.block;
   if (n < static_cast<int64_t>(c.block) * c.items * c.max_blocks) c.max_blocks = 0;
   return c;
+}
+inline RangeConfig default_scan_config(int64_t n) {
+  return n < (int64_t(1) << 10)   ? RangeConfig{128, 8, 0}
+         : n < (int64_t(1) << 17) ? RangeConfig{256, 8, 0}
+         : n < (int64_t(1) << 20) ? RangeConfig{512, 8, 0}
+         : n < (int64_t(1) << 21) ? RangeConfig{1024, 8, 0}
+         : n < (int64_t(1) << 22) ? RangeConfig{1024, 16, 0}
+                                  : RangeConfig{1024, 32, 0};
 }
 
 // ------------------------------------------------------------------ tuning hooks
@@ -812,13 +826,14 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
                   : range_overrides().set[2]
                       ? range_overrides().cfg[2]
                       : tuned_range_config<typename CR::functor_type>(2, e - b, sizeof(V), std::is_floating_point<V>::value,
-                                                                     RangeConfig{1024, 32, 0});
+                                                                     default_scan_config(e - b));
   c.block       = normalise_block(c.block);
   const int ipt = c.items >= 64 ? 64 : c.items >= 32 ? 32 : c.items >= 16 ? 16 : 8;
   const int64_t tile   = static_cast<int64_t>(c.block) * ipt;
   const int64_t ntiles = ceil_div(e - b, tile);
   ScanTileState<V> st;
-  void* status = inst->scan_status_space(ScanTileState<V>::bytes(ntiles));
+  void* status = scan_state_kind<V>::value == 0 ? inst->scan_flags_space(ScanTileState<V>::bytes(ntiles))
+                                                : inst->scan_status_space(ScanTileState<V>::bytes(ntiles));
   void* vals   = nullptr;
   if constexpr (scan_state_kind<V>::value == 0)
     vals = inst->scan_values_space(ScanTileState<V>::value_bytes(ntiles));

@@ -192,6 +192,8 @@ struct MI355XInstance {
   void* scan_status       = nullptr;  // device, scan tile status words
   size_t scan_status_bytes = 0;
   void* scan_values       = nullptr;  // device, scan tile values (generic path)
+  void* scan_flags        = nullptr;  // device, scan tile flags (generic path)
+  size_t scan_flags_bytes = 0;
   size_t scan_values_bytes = 0;
   void* pinned            = nullptr;  // host-mapped result staging
   size_t pinned_bytes     = 0;
@@ -215,6 +217,7 @@ struct MI355XInstance {
     (void)mx_free(partials, nullptr);
     (void)mx_free(scan_status, nullptr);
     (void)mx_free(scan_values, nullptr);
+    (void)mx_free(scan_flags, nullptr);
     (void)mx_free(kernarg_stage, nullptr);
     (void)mx_free(team_l1, nullptr);
     (void)mx_host_free(pinned);
@@ -240,6 +243,10 @@ struct MI355XInstance {
   void* partials_space(size_t need) { return grow(partials, partials_bytes, need, false); }
   void* scan_status_space(size_t need) { return grow(scan_status, scan_status_bytes, need, true); }
   void* scan_values_space(size_t need) { return grow(scan_values, scan_values_bytes, need, false); }
+  // The generic scan state keeps 32-bit flags apart from the 64-bit
+  // {flag, value} words of the 4- and 8-byte states: a flag word read where
+  // another layout left a value could otherwise pass for a current flag.
+  void* scan_flags_space(size_t need) { return grow(scan_flags, scan_flags_bytes, need, true); }
   void* kernarg_space(size_t need) { return grow(kernarg_stage, kernarg_stage_bytes, need, false); }
   void* team_l1_space(size_t need) { return grow(team_l1, team_l1_bytes, need, false); }
   void* pinned_space(size_t need) {

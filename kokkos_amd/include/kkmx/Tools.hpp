@@ -25,6 +25,8 @@
 #include <dlfcn.h>
 
 #include <cstring>
+#include <algorithm>
+#include <deque>
 #include <map>
 #include <set>
 #include <string>
@@ -137,11 +139,16 @@ struct ToolsState {
   // tuning bookkeeping
   size_t context_counter  = 0;
   size_t variable_counter = 0;
-  std::unordered_map<size_t, Experimental::VariableInfo> metadata;
-  std::map<size_t, std::unordered_set<size_t>> features_per_context;
-  std::set<size_t> active_features;
-  std::unordered_map<size_t, Experimental::VariableValue> feature_values;
+  // Variable ids and context ids are small consecutive integers, so the
+  // per-request bookkeeping is kept in id-indexed arrays (a request on every
+  // dispatch must cost well under a microsecond).  metadata is a deque:
+  // tools keep pointers into it.
+  std::deque<Experimental::VariableInfo> metadata;             // [id]
+  std::vector<std::vector<size_t>> features_per_context;       // [ctx]
+  std::vector<size_t> active_features;                         // ids, insertion order
+  std::vector<Experimental::VariableValue> feature_values;     // [id]
   std::unordered_map<size_t, size_t> goals;
+  std::vector<Experimental::VariableValue> request_scratch;
   size_t kernel_name_id = 0, kernel_type_id = 0;
   bool ids_declared     = false;
   std::unordered_map<std::string, uint32_t> sections;
@@ -179,37 +186,52 @@ inline size_t get_current_context_id() { return Impl::tools_state().context_coun
 inline size_t get_new_variable_id() { return ++Impl::tools_state().variable_counter; }
 inline bool have_tuning_tool() { return Impl::tools_state().cb.request_output_values != nullptr; }
 
+inline void impl_register_variable(size_t id, const VariableInfo& info) {
+  auto& s = Tools::Impl::tools_state();
+  if (s.metadata.size() <= id) s.metadata.resize(id + 1);
+  s.metadata[id] = info;
+  if (s.feature_values.size() <= id) s.feature_values.resize(id + 1);
+}
+inline VariableInfo* impl_variable_info(size_t id) {
+  auto& s = Tools::Impl::tools_state();
+  return id < s.metadata.size() ? &s.metadata[id] : nullptr;
+}
+
 inline size_t declare_output_type(const std::string& name, VariableInfo info) {
   auto& s   = Impl::tools_state();
   size_t id = get_new_variable_id();
   Impl::invoke(Impl::MayFence::No, s.cb.declare_output_type, name.c_str(), id, &info);
-  s.metadata[id] = info;
+  impl_register_variable(id, info);
   return id;
 }
 inline size_t declare_input_type(const std::string& name, VariableInfo info) {
   auto& s   = Impl::tools_state();
   size_t id = get_new_variable_id();
   Impl::invoke(Impl::MayFence::No, s.cb.declare_input_type, name.c_str(), id, &info);
-  s.metadata[id] = info;
+  impl_register_variable(id, info);
   return id;
 }
 inline void set_input_values(size_t ctx, size_t count, VariableValue* values) {
   auto& s = Impl::tools_state();
+  if (s.features_per_context.size() <= ctx) s.features_per_context.resize(ctx + 1);
   auto& f = s.features_per_context[ctx];
   for (size_t i = 0; i < count; ++i) {
-    values[i].metadata = &s.metadata[values[i].type_id];
-    f.insert(values[i].type_id);
-    s.active_features.insert(values[i].type_id);
-    s.feature_values[values[i].type_id] = values[i];
+    const size_t id    = values[i].type_id;
+    values[i].metadata = impl_variable_info(id);
+    if (std::find(f.begin(), f.end(), id) == f.end()) f.push_back(id);
+    if (std::find(s.active_features.begin(), s.active_features.end(), id) == s.active_features.end())
+      s.active_features.push_back(id);
+    if (s.feature_values.size() <= id) s.feature_values.resize(id + 1);
+    s.feature_values[id] = values[i];
   }
 }
 inline void request_output_values(size_t ctx, size_t count, VariableValue* values) {
   auto& s = Impl::tools_state();
   if (!s.cb.request_output_values) return;
-  std::vector<VariableValue> in;
-  in.reserve(s.active_features.size());
+  auto& in = s.request_scratch;
+  in.clear();
   for (size_t id : s.active_features) in.push_back(s.feature_values[id]);
-  for (size_t i = 0; i < count; ++i) values[i].metadata = &s.metadata[values[i].type_id];
+  for (size_t i = 0; i < count; ++i) values[i].metadata = impl_variable_info(values[i].type_id);
   Impl::invoke(Impl::MayFence::No, s.cb.request_output_values, ctx, in.size(), in.data(), count,
                values);
 }
@@ -218,12 +240,18 @@ inline void begin_context(size_t ctx) {
 }
 inline void end_context(size_t ctx) {
   auto& s = Impl::tools_state();
-  for (size_t id : s.features_per_context[ctx]) s.active_features.erase(id);
-  s.features_per_context.erase(ctx);
+  if (ctx < s.features_per_context.size()) {
+    auto& f = s.features_per_context[ctx];
+    for (size_t id : f) {
+      auto it = std::find(s.active_features.begin(), s.active_features.end(), id);
+      if (it != s.active_features.end()) s.active_features.erase(it);
+    }
+    f.clear();
+  }
   VariableValue goal{};
   auto g = s.goals.find(ctx);
   if (g != s.goals.end()) {
-    goal = s.feature_values[g->second];
+    if (g->second < s.feature_values.size()) goal = s.feature_values[g->second];
     s.goals.erase(g);
   }
   Impl::invoke(Impl::MayFence::No, s.cb.end_tuning_context, ctx, goal);
