@@ -126,9 +126,10 @@ int kkmx_batch_create(kkmx_batch** out);
 int kkmx_batch_destroy(kkmx_batch* b);
 int kkmx_batch_add(kkmx_batch* b, int op, int type, const void* in_dev, int64_t n, void* out_dev, void* result_dev,
                    double alpha, double beta);
-/* Tile shape of the fused kernel, before plan: 0 = 256 lanes x 16 single
- * elements, 1 = 256 lanes x 4 runs of 4 (default), 2 = 256 x 8 runs of 4,
- * 3 = 512 x 4 runs of 4. */
+/* Tile shape of the fused kernel, before plan (lanes x consecutive elements
+ * per lane and round x rounds): 0 = 256x1x16, 1 = 256x4x4, 2 = 256x4x8,
+ * 3 = 512x4x4, 4 = 1024x1x16, 5 = 512x4x8 (the default), 6 = 1024x4x4,
+ * 7 = 512x1x32. */
 int kkmx_batch_set_shape(kkmx_batch* b, int shape);
 int kkmx_batch_plan(kkmx_batch* b, int64_t* tiles_out);
 int kkmx_batch_run(kkmx_batch* b, int mode, int steps);

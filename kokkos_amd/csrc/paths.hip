@@ -5,6 +5,7 @@
 // and parallel_for / parallel_reduce / parallel_scan on the default MI355X
 // instance.  Nothing here calls HIP directly.
 #include <Kokkos_Core.hpp>
+#include <Kokkos_Batch.hpp>
 
 #include <cmath>
 #include <memory>

@@ -10,6 +10,7 @@
 // through the same launch-shape override the tuning hooks use.
 #include <Kokkos_Core.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -201,6 +202,93 @@ struct ScanPairI64 {
   }
 };
 
+// ---- TeamPolicy families (extent = league size; one row of `width` per team)
+using TeamP  = Kokkos::TeamPolicy<Kokkos::MI355X>;
+using Member = TeamP::member_type;
+struct TeamRowSum {  // y(r) = sum of row r
+  static constexpr double kkmx_bytes_per_index = 256 * 8 + 8, kkmx_flops_per_index = 256;
+  Vec<const double> a;
+  Vec<double> y;
+  KOKKOS_INLINE_FUNCTION void operator()(const Member& m) const {
+    const int64_t r = m.league_rank();
+    double s        = 0;
+    Kokkos::parallel_reduce(
+        Kokkos::TeamThreadRange(m, 256), [&](const int k, double& u) { u += a(r * 256 + k); }, s);
+    Kokkos::single(Kokkos::PerTeam(m), [&]() { y(r) = s; });
+  }
+};
+struct TeamGather {  // y(r) = sum_k a(r*32+k) * x(col(r*32+k)), 32 random columns per row
+  static constexpr double kkmx_bytes_per_index = 32 * 20 + 8, kkmx_flops_per_index = 64;
+  Vec<const double> a, x;
+  Vec<const int32_t> col;
+  Vec<double> y;
+  int64_t ncols;
+  KOKKOS_INLINE_FUNCTION void operator()(const Member& m) const {
+    const int64_t r = m.league_rank();
+    double s        = 0;
+    Kokkos::parallel_reduce(
+        Kokkos::TeamThreadRange(m, 32),
+        [&](const int k, double& u) {
+          const int64_t j = r * 32 + k;
+          u += a(j) * x(int64_t(uint32_t(col(j))) % ncols);
+        },
+        s);
+    Kokkos::single(Kokkos::PerTeam(m), [&]() { y(r) = s; });
+  }
+};
+struct TeamCopy {  // 512 doubles per team, thread x vector lanes
+  static constexpr double kkmx_bytes_per_index = 512 * 16, kkmx_flops_per_index = 0;
+  Vec<const double> a;
+  Vec<double> b;
+  KOKKOS_INLINE_FUNCTION void operator()(const Member& m) const {
+    const int64_t r = m.league_rank();
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, 512), [&](const int k) { b(r * 512 + k) = a(r * 512 + k); });
+  }
+};
+struct TeamTotal {  // total of all rows: nested reduce inside a team reduction
+  static constexpr double kkmx_bytes_per_index = 128 * 8, kkmx_flops_per_index = 128;
+  Vec<const double> a;
+  KOKKOS_INLINE_FUNCTION void operator()(const Member& m, double& u) const {
+    const int64_t r = m.league_rank();
+    double s        = 0;
+    Kokkos::parallel_reduce(
+        Kokkos::TeamThreadRange(m, 128), [&](const int k, double& v) { v += a(r * 128 + k); }, s);
+    Kokkos::single(Kokkos::PerTeam(m), [&]() { u += s; });
+  }
+};
+// ---- MDRangePolicy families (extent = points)
+using Grid3 = Kokkos::View<double***, Kokkos::LayoutLeft, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+using Grid2 = Kokkos::View<double**, Kokkos::LayoutLeft, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+struct Md7Point {
+  static constexpr double kkmx_bytes_per_index = 16, kkmx_flops_per_index = 8;
+  Grid3 u, v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, const int64_t j, const int64_t k) const {
+    v(i, j, k) = 0.4 * u(i, j, k) + 0.1 * (u(i - 1, j, k) + u(i + 1, j, k) + u(i, j - 1, k) + u(i, j + 1, k) +
+                                           u(i, j, k - 1) + u(i, j, k + 1));
+  }
+};
+struct Md5Point2D {
+  static constexpr double kkmx_bytes_per_index = 16, kkmx_flops_per_index = 6;
+  Grid2 u, v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, const int64_t j) const {
+    v(i, j) = 0.5 * u(i, j) + 0.125 * (u(i - 1, j) + u(i + 1, j) + u(i, j - 1) + u(i, j + 1));
+  }
+};
+struct MdAxpy3 {
+  static constexpr double kkmx_bytes_per_index = 24, kkmx_flops_per_index = 2;
+  Grid3 x, y;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, const int64_t j, const int64_t k) const {
+    y(i, j, k) += 2.5 * x(i, j, k);
+  }
+};
+struct MdSumSq3 {
+  static constexpr double kkmx_bytes_per_index = 8, kkmx_flops_per_index = 2;
+  Grid3 x;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, const int64_t j, const int64_t k, double& s) const {
+    s += x(i, j, k) * x(i, j, k);
+  }
+};
+
 struct InitBuffers {
   Vec<double> a, b, c;
   KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const {
@@ -210,7 +298,8 @@ struct InitBuffers {
   }
 };
 
-constexpr int kFamilies = 20;
+constexpr int kRangeFamilies = 20;
+constexpr int kFamilies      = 28;
 
 // pattern, value bytes, floating, functor bytes, bytes/index, flops/index
 struct FamilyInfo {
@@ -219,11 +308,13 @@ struct FamilyInfo {
   int floating;
   int functor_bytes;
   double bytes, flops;
+  int kind;  // 0 RangePolicy, 1 TeamPolicy, 2 MDRangePolicy
 };
 
 template <class F>
-FamilyInfo info(int pattern, int value_bytes, int floating) {
-  return FamilyInfo{pattern, value_bytes, floating, int(sizeof(F)), F::kkmx_bytes_per_index, F::kkmx_flops_per_index};
+FamilyInfo info(int pattern, int value_bytes, int floating, int kind = 0) {
+  return FamilyInfo{pattern, value_bytes, floating, int(sizeof(F)), F::kkmx_bytes_per_index, F::kkmx_flops_per_index,
+                    kind};
 }
 
 FamilyInfo family_info(int f) {
@@ -247,7 +338,15 @@ FamilyInfo family_info(int f) {
     case 16: return info<ScanInclI32>(2, 4, 0);
     case 17: return info<ScanExclI64>(2, 8, 0);
     case 18: return info<ScanExclF64>(2, 8, 1);
-    default: return info<ScanPairI64>(2, 16, 0);
+    case 19: return info<ScanPairI64>(2, 16, 0);
+    case 20: return info<TeamRowSum>(0, 0, 0, 1);
+    case 21: return info<TeamGather>(0, 0, 0, 1);
+    case 22: return info<TeamCopy>(0, 0, 0, 1);
+    case 23: return info<TeamTotal>(1, 8, 1, 1);
+    case 24: return info<Md7Point>(0, 0, 0, 2);
+    case 25: return info<Md5Point2D>(0, 0, 0, 2);
+    case 26: return info<MdAxpy3>(0, 0, 0, 2);
+    default: return info<MdSumSq3>(1, 8, 1, 2);
   }
 }
 
@@ -292,6 +391,86 @@ void run_family(int f, int64_t n) {
   }
 }
 
+// Effective extent a family runs at for a requested extent (team families
+// bound the league by the buffer size; MDRange families use a cube or
+// square of the nearest side).
+// Columns of the gathered vector of the team gather family: 2^20, or a
+// quarter of the buffers when they are smaller.
+int64_t gather_cols() { return std::min<int64_t>(int64_t(1) << 20, g_buf.n / 4); }
+
+int64_t effective_extent(int f, int64_t n) {
+  const int64_t cap = g_buf.n;
+  switch (f) {
+    case 20: return std::min<int64_t>(n, cap / 256);
+    case 21: return std::min<int64_t>(n, (cap - gather_cols()) / 32);
+    case 22: return std::min<int64_t>(n, cap / 512);
+    case 23: return std::min<int64_t>(n, cap / 128);
+    case 24:
+    case 26:
+    case 27: {
+      int64_t sd = std::max<int64_t>(2, int64_t(std::cbrt(double(n))));
+      while ((sd + 2) * (sd + 2) * (sd + 2) > cap) --sd;
+      return sd * sd * sd;
+    }
+    case 25: {
+      int64_t sd = std::max<int64_t>(2, int64_t(std::sqrt(double(n))));
+      while ((sd + 2) * (sd + 2) > cap) --sd;
+      return sd * sd;
+    }
+    default: return std::min<int64_t>(n, cap);
+  }
+}
+
+// Team shape candidate c (or the policy's AUTO defaults for c < 0).
+TeamP team_policy(int64_t league, int c) {
+  if (c < 0) return TeamP(int(league), Kokkos::AUTO, Kokkos::AUTO);
+  return TeamP(int(league), Kokkos::Impl::kTeamCandidates[c][0], Kokkos::Impl::kTeamCandidates[c][1]);
+}
+
+void run_team(int f, int64_t L, int c) {
+  double *a = g_buf.a, *b = g_buf.b, *cc = g_buf.c;
+  const TeamP tp = team_policy(L, c);
+  switch (f) {
+    case 20: Kokkos::parallel_for("sweep.team_rowsum", tp, TeamRowSum{Vec<const double>(a, L * 256), Vec<double>(b, L)}); break;
+    case 21: {
+      const int64_t ncols = gather_cols();
+      Kokkos::parallel_for("sweep.team_gather", tp,
+                           TeamGather{Vec<const double>(a, L * 32), Vec<const double>(cc, ncols),
+                                      Vec<const int32_t>(reinterpret_cast<int32_t*>(b), L * 32),
+                                      Vec<double>(cc + ncols, L), ncols});
+      break;
+    }
+    case 22: Kokkos::parallel_for("sweep.team_copy", tp, TeamCopy{Vec<const double>(a, L * 512), Vec<double>(b, L * 512)}); break;
+    default:
+      Kokkos::parallel_reduce("sweep.team_total", tp, TeamTotal{Vec<const double>(a, L * 128)},
+                              Scalar<double>(g_buf.res));
+      break;
+  }
+}
+
+void run_md(int f, int64_t n, int c) {
+  double *a = g_buf.a, *b = g_buf.b;
+  if (f == 25) {
+    const int64_t sd = int64_t(std::llround(std::sqrt(double(n))));
+    using MP         = Kokkos::MDRangePolicy<Kokkos::MI355X, Kokkos::Rank<2>, Kokkos::IndexType<int64_t>>;
+    MP mp            = c < 0 ? MP({1, 1}, {sd + 1, sd + 1}) : MP({1, 1}, {sd + 1, sd + 1}, {16 << (c % 4), 1 << (c / 4)});
+    Kokkos::parallel_for("sweep.md_5pt", mp, Md5Point2D{Grid2(a, sd + 2, sd + 2), Grid2(b, sd + 2, sd + 2)});
+    return;
+  }
+  const int64_t sd = int64_t(std::llround(std::cbrt(double(n))));
+  using MP         = Kokkos::MDRangePolicy<Kokkos::MI355X, Kokkos::Rank<3>, Kokkos::IndexType<int64_t>>;
+  MP mp = c < 0 ? MP({1, 1, 1}, {sd + 1, sd + 1, sd + 1})
+                : MP({1, 1, 1}, {sd + 1, sd + 1, sd + 1}, {16 << (c % 4), 1 << (c / 4), 1});
+  const int64_t m = sd + 2;
+  switch (f) {
+    case 24: Kokkos::parallel_for("sweep.md_7pt", mp, Md7Point{Grid3(a, m, m, m), Grid3(b, m, m, m)}); break;
+    case 26: Kokkos::parallel_for("sweep.md_axpy", mp, MdAxpy3{Grid3(a, m, m, m), Grid3(b, m, m, m)}); break;
+    default:
+      Kokkos::parallel_reduce("sweep.md_sumsq", mp, MdSumSq3{Grid3(a, m, m, m)}, Scalar<double>(g_buf.res));
+      break;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -301,9 +480,10 @@ int kkmx_sweep_family_count(void) { return kFamilies; }
 
 // pattern, value bytes, floating, functor bytes, bytes/index, flops/index of a family.
 int kkmx_sweep_family_info(int f, int* pattern, int* value_bytes, int* floating, int* functor_bytes, double* bytes,
-                           double* flops) {
+                           double* flops, int* kind) {
   if (f < 0 || f >= kFamilies) return -1;
   const FamilyInfo i = family_info(f);
+  *kind              = i.kind;
   *pattern           = i.pattern;
   *value_bytes       = i.value_bytes;
   *floating          = i.floating;
@@ -347,24 +527,36 @@ int kkmx_sweep_teardown(void) {
 // 16 launch-shape candidates (ms_out[16]), after one untimed warm-up each.
 // candidate < 0 times all 16; otherwise only that one (ms_out[0]).  heuristic
 // = 1 adds the time of the untuned heuristic shape in ms_out[16].
-int kkmx_sweep_time(int f, int64_t n, int reps, int candidate, int heuristic, float* ms_out) {
-  if (f < 0 || f >= kFamilies || n < 1 || n > g_buf.n || reps < 1 || !ms_out) {
+int kkmx_sweep_time(int f, int64_t n, int reps, int candidate, int heuristic, float* ms_out, int64_t* n_eff) {
+  if (f < 0 || f >= kFamilies || n < 1 || reps < 1 || !ms_out || g_buf.n < 4096 ||
+      (f < 20 && n > g_buf.n)) {
     g_err = "kkmx_sweep_time: bad arguments";
     return -1;
   }
+  n = effective_extent(f, n);
+  if (n_eff) *n_eff = n;
   try {
     Kokkos::MI355X ex;
     mx_event_t e0, e1;
     KKMX_SAFE_CALL(mx_event_create(&e0));
     KKMX_SAFE_CALL(mx_event_create(&e1));
-    const int pattern = family_info(f).pattern;
-    auto& ov          = Kokkos::Impl::range_overrides();
-    auto time_one     = [&]() {
+    const FamilyInfo fi = family_info(f);
+    const int pattern   = fi.pattern;
+    auto& ov            = Kokkos::Impl::range_overrides();
+    // cand < 0: the untuned default shape.  Range families take the
+    // candidate through the launch-shape override, Team and MDRange
+    // families through their policy.
+    auto run = [&](int cand) {
+      if (fi.kind == 1) run_team(f, n, cand);
+      else if (fi.kind == 2) run_md(f, n, cand);
+      else run_family(f, n);
+    };
+    auto time_one = [&](int cand) {
       float best = 1e30f;
-      run_family(f, n);
+      run(cand);
       for (int r = 0; r < reps; ++r) {
         KKMX_SAFE_CALL(mx_event_record(e0, ex.stream()));
-        run_family(f, n);
+        run(cand);
         KKMX_SAFE_CALL(mx_event_record(e1, ex.stream()));
         KKMX_SAFE_CALL(mx_event_sync(e1));
         float ms = 0;
@@ -375,12 +567,14 @@ int kkmx_sweep_time(int f, int64_t n, int reps, int candidate, int heuristic, fl
     };
     const int c0 = candidate < 0 ? 0 : candidate, c1 = candidate < 0 ? Kokkos::Impl::kRangeCandidates : candidate + 1;
     for (int c = c0; c < c1; ++c) {
-      ov.cfg[pattern] = Kokkos::Impl::range_candidate(pattern, c, n);
-      ov.set[pattern] = true;
-      ms_out[c - c0]  = time_one();
+      if (fi.kind == 0) {
+        ov.cfg[pattern] = Kokkos::Impl::range_candidate(pattern, c, n);
+        ov.set[pattern] = true;
+      }
+      ms_out[c - c0] = time_one(c);
     }
     ov.set[pattern] = false;
-    if (heuristic) ms_out[c1 - c0] = time_one();
+    if (heuristic) ms_out[c1 - c0] = time_one(-1);
     KKMX_SAFE_CALL(mx_event_destroy(e0));
     KKMX_SAFE_CALL(mx_event_destroy(e1));
     return 0;

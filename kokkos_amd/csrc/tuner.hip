@@ -7,8 +7,10 @@
 // kokkosp_declare_input_type, kokkosp_declare_output_type,
 // kokkosp_request_values, kokkosp_begin_context, kokkosp_end_context), so it
 // loads like any tuning tool (KOKKOS_TOOLS_LIBS).  When the runtime asks for
-// the output variable "kokkos.mi355x.range_launch" (16 launch shapes of a
-// RangePolicy dispatch), the tool turns the declared input variables of the
+// one of the output variables "kokkos.mi355x.range_launch" (16 launch shapes
+// of a RangePolicy dispatch), "kokkos.mi355x.team_shape" (16 team-size x
+// vector-length pairs) or "kokkos.mi355x.mdrange_tile" (16 tiles), the tool
+// turns the declared input variables of the
 // request into the 12 features, runs the MLP on the GPU and answers the
 // arg-max config.  Decisions are cached per feature key, so the GPU sees one
 // inference launch per distinct kernel signature and extent bucket.  Without
@@ -231,7 +233,7 @@ struct KeyHash {
     return size_t(h ^ (h >> 29));
   }
 };
-enum Field { kPattern, kExtent, kValueBytes, kFunctorBytes, kFloating, kBytes, kFlops, kCus };
+enum Field { kPattern, kExtent, kValueBytes, kFunctorBytes, kFloating, kBytes, kFlops, kCus, kKind };
 
 struct ToolState {
   std::mutex mtx;
@@ -239,8 +241,8 @@ struct ToolState {
   // Declared input ids -> feature field (looked up per request, so no string
   // compares on the dispatch path).
   std::unordered_map<size_t, int> input_field;
-  size_t range_launch_id = 0;
-  bool have_range_var    = false;
+  size_t var_id[3]  = {0, 0, 0};  // range launch, team shape, MDRange tile
+  bool have_var[3]   = {false, false, false};
   std::unordered_map<Key, int, KeyHash> cache;
   DevBuf* dev_params = nullptr;
   mx_stream_t stream = nullptr;
@@ -256,10 +258,11 @@ ToolState& tool() {
 //  4 log2(value bytes + 1)/4  5 log2(functor bytes + 1)/10
 //  6 floating-point value     7 log2(bytes/index + 1)/6
 //  8 log2(flops/index + 1)/8  9 log2(extent / (CUs*2048) + 2^-20)/20
-// 10 log2(extent * bytes/index + 1)/40   11 CUs / 256
+// 10 log2(extent * bytes/index + 1)/40   11 policy kind / 2
 struct Inputs {
   double extent = 1, value_bytes = 0, functor_bytes = 0, floating = 0, bytes = 0, flops = 0, cus = 256;
   int pattern   = 0;
+  int kind      = 0;  // 0 RangePolicy launch shape, 1 Team shape, 2 MDRange tile
 };
 void features(const Inputs& in, float* f) {
   f[0] = float(std::log2(std::max(in.extent, 1.0)) / 32.0);
@@ -273,7 +276,7 @@ void features(const Inputs& in, float* f) {
   f[8] = float(std::log2(in.flops + 1.0) / 8.0);
   f[9] = float(std::log2(in.extent / (std::max(in.cus, 1.0) * 2048.0) + 0x1.0p-20) / 20.0);
   f[10] = float(std::log2(in.extent * in.bytes + 1.0) / 40.0);
-  f[11] = float(in.cus / 256.0);
+  f[11] = float(in.kind) * 0.5f;
 }
 
 int predict_one(const float* feat) {
@@ -430,8 +433,9 @@ int kkmx_mlp_predict(const float* params, const float* X, int n, float* Z) {
 
 // The 12 features of a request, before normalisation (for the sweep driver).
 void kkmx_tuner_features(int pattern, double extent, double value_bytes, double functor_bytes, int floating,
-                         double bytes_per_index, double flops_per_index, int cus, float* out) {
+                         double bytes_per_index, double flops_per_index, int cus, int kind, float* out) {
   Inputs in;
+  in.kind          = kind;
   in.pattern       = pattern;
   in.extent        = extent;
   in.value_bytes   = value_bytes;
@@ -486,7 +490,8 @@ void kokkosp_declare_input_type(const char* name, const size_t id, Kokkos_Tools_
       {"kokkos.mi355x.pattern", kPattern},         {"kokkos.mi355x.extent", kExtent},
       {"kokkos.mi355x.value_bytes", kValueBytes},   {"kokkos.mi355x.functor_bytes", kFunctorBytes},
       {"kokkos.mi355x.floating", kFloating},        {"kokkos.mi355x.bytes_per_index", kBytes},
-      {"kokkos.mi355x.flops_per_index", kFlops},    {"kokkos.mi355x.cu_count", kCus}};
+      {"kokkos.mi355x.flops_per_index", kFlops},    {"kokkos.mi355x.cu_count", kCus},
+      {"kokkos.mi355x.policy_kind", kKind}};
   auto& s = tool();
   std::lock_guard<std::mutex> lk(s.mtx);
   for (const auto& nf : names)
@@ -495,10 +500,12 @@ void kokkosp_declare_input_type(const char* name, const size_t id, Kokkos_Tools_
 void kokkosp_declare_output_type(const char* name, const size_t id, Kokkos_Tools_VariableInfo*) {
   auto& s = tool();
   std::lock_guard<std::mutex> lk(s.mtx);
-  if (std::strcmp(name, "kokkos.mi355x.range_launch") == 0) {
-    s.range_launch_id = id;
-    s.have_range_var  = true;
-  }
+  const char* names[3] = {"kokkos.mi355x.range_launch", "kokkos.mi355x.team_shape", "kokkos.mi355x.mdrange_tile"};
+  for (int v = 0; v < 3; ++v)
+    if (std::strcmp(name, names[v]) == 0) {
+      s.var_id[v]   = id;
+      s.have_var[v] = true;
+    }
 }
 void kokkosp_begin_context(const size_t) {}
 void kokkosp_end_context(const size_t, Kokkos_Tools_VariableValue) {}
@@ -508,10 +515,17 @@ void kokkosp_request_values(const size_t, const size_t n_in, const Kokkos_Tools_
   auto& s = tool();
   std::lock_guard<std::mutex> lk(s.mtx);
   for (size_t o = 0; o < n_out; ++o) {
-    if (!s.have_range_var || out[o].type_id != s.range_launch_id) continue;
+    int var = -1;
+    for (int v = 0; v < 3; ++v)
+      if (s.have_var[v] && out[o].type_id == s.var_id[v]) var = v;
+    if (var < 0) continue;
     ++s.requests;
     if (!s.model.loaded) continue;  // the runtime's heuristic value stands
+    // A model trained on RangePolicy samples only has no spread in the
+    // policy-kind feature; it answers RangePolicy requests alone.
+    if (var != 0 && s.model.inv_std[11] == 0.f) continue;
     Inputs x;
+    x.kind = var;
     for (size_t i = 0; i < n_in; ++i) {
       auto it = s.input_field.find(in[i].type_id);
       if (it == s.input_field.end()) continue;
@@ -524,13 +538,14 @@ void kokkosp_request_values(const size_t, const size_t n_in, const Kokkos_Tools_
         case kFloating: x.floating = double(v.int_value); break;
         case kBytes: x.bytes = v.double_value; break;
         case kFlops: x.flops = v.double_value; break;
+        case kKind: break;  // the requested variable already tells the kind
         default: x.cus = double(v.int_value); break;
       }
     }
     // Cache key: everything but the extent, which is bucketed in quarter
     // octaves.
     const int64_t bucket = int64_t(std::floor(std::log2(std::max(x.extent, 1.0)) * 4.0));
-    const Key key{{x.pattern, int64_t(x.value_bytes), int64_t(x.functor_bytes), int64_t(x.floating),
+    const Key key{{x.pattern + 4 * x.kind, int64_t(x.value_bytes), int64_t(x.functor_bytes), int64_t(x.floating),
                    int64_t(x.bytes * 16), int64_t(x.flops * 16), int64_t(x.cus), bucket}};
     int c;
     auto hit = s.cache.find(key);

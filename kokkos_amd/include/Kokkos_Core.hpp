@@ -27,7 +27,6 @@
 #include <kkmx/Parallel.hpp>
 #include <kkmx/CopyViews.hpp>
 #include <kkmx/Graph.hpp>
-#include <kkmx/Batch.hpp>
 #include <kkmx/SpacesImpl.hpp>
 
 namespace Kokkos {

@@ -21,8 +21,10 @@
 //    aggregates of its predecessors with the decoupled look-back of the
 //    range scan (tiles of other ops count as the identity) and writes the
 //    result.  Scan tiles publish aggregate, look back, publish the inclusive
-//    prefix and write their outputs.  Tile order is fixed, so results are
-//    bit-reproducible run to run;
+//    prefix (only for scans longer than kBatchDetTiles tiles) and write their
+//    outputs.  Reductions and scans of up to kBatchDetTiles tiles combine in
+//    a fixed order, so their floating-point results are bit-reproducible run
+//    to run; integer results are exact in every case;
 //  * the last workgroup to leave resets the queue and advances the epoch
 //    carried by every tile status word: the next replay needs no clearing.
 #ifndef KKMX_BATCH_HPP
@@ -68,9 +70,11 @@ struct BatchShape {
   int block, k, r;
   constexpr int64_t tile() const { return int64_t(block) * k * r; }
 };
-constexpr int kBatchShapes = 4;
-constexpr BatchShape kBatchShape[kBatchShapes] = {{256, 1, 16}, {256, 4, 4}, {256, 4, 8}, {512, 4, 4}};
-constexpr int kBatchDefaultShape = 1;
+constexpr int kBatchShapes = 8;
+constexpr BatchShape kBatchShape[kBatchShapes] = {{256, 1, 16}, {256, 4, 4}, {256, 4, 8},  {512, 4, 4},
+                                                  {1024, 1, 16}, {512, 4, 8}, {1024, 4, 4}, {512, 1, 32}};
+constexpr int kBatchDefaultShape = 5;
+constexpr int64_t kBatchDetTiles = 2048;
 
 template <class Acc>
 __device__ __forceinline__ Acc batch_identity(int op) {
@@ -207,7 +211,14 @@ __device__ __forceinline__ void batch_tile(const BatchDesc& d, int64_t tile, uin
   const Acc wpre = (w == 0) ? identity : wave::readlane(xs, w - 1);
   __syncthreads();  // s_wave is reused by the next tile
 
+  // Ops of up to kBatchDetTiles tiles never publish inclusive prefixes past
+  // their first tile, so every look-back sums all predecessor aggregates in
+  // the same fixed order: results are bit-reproducible run to run (a few
+  // look-back rounds per tile at most).  Longer scans also publish
+  // inclusive prefixes, which shortens the walk but makes the floating-point
+  // combination order depend on timing.
   const bool last = local == d.ntiles - 1;
+  const bool det  = d.ntiles <= kBatchDetTiles;
   Acc prefix      = identity;
   if (d.ntiles > 1) {
     if (local == 0) {
@@ -215,7 +226,7 @@ __device__ __forceinline__ void batch_tile(const BatchDesc& d, int64_t tile, uin
     } else if (is_scan || last) {
       if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanAgg), tagg);
       prefix = scan_lookback(st, tile, epoch, identity, join, s_red, &lds.pos, d.tile0);
-      if (is_scan && threadIdx.x == 0) {
+      if (is_scan && !det && threadIdx.x == 0) {
         Acc incl = prefix;
         join(incl, tagg);
         st.publish(tile, scan_flag(epoch, kScanIncl), incl);
@@ -314,18 +325,24 @@ inline const void* batch_kernel(int shape) {
     case 0: return batch_kernel<0>();
     case 1: return batch_kernel<1>();
     case 2: return batch_kernel<2>();
-    default: return batch_kernel<3>();
+    case 3: return batch_kernel<3>();
+    case 4: return batch_kernel<4>();
+    case 5: return batch_kernel<5>();
+    case 6: return batch_kernel<6>();
+    default: return batch_kernel<7>();
   }
 }
 
 // Per-call (plain) execution of one op through the public dispatch API.
 template <class In, class Acc>
 struct BatchReduceF {
+  static constexpr double kkmx_bytes_per_index = sizeof(In), kkmx_flops_per_index = 1;
   View<const In*, MI355XSpace, MemoryTraits<Unmanaged>> x;
   KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, Acc& u) const { u += static_cast<Acc>(x(i)); }
 };
 template <class In, class Acc>
 struct BatchMinF {
+  static constexpr double kkmx_bytes_per_index = sizeof(In), kkmx_flops_per_index = 1;
   View<const In*, MI355XSpace, MemoryTraits<Unmanaged>> x;
   KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, Acc& u) const {
     const Acc a = static_cast<Acc>(x(i));
@@ -334,6 +351,7 @@ struct BatchMinF {
 };
 template <class In, class Acc>
 struct BatchMaxF {
+  static constexpr double kkmx_bytes_per_index = sizeof(In), kkmx_flops_per_index = 1;
   View<const In*, MI355XSpace, MemoryTraits<Unmanaged>> x;
   KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, Acc& u) const {
     const Acc a = static_cast<Acc>(x(i));
@@ -342,6 +360,7 @@ struct BatchMaxF {
 };
 template <class In, class Acc, bool Exclusive>
 struct BatchScanF {
+  static constexpr double kkmx_bytes_per_index = sizeof(In) + sizeof(Acc), kkmx_flops_per_index = 1;
   View<const In*, MI355XSpace, MemoryTraits<Unmanaged>> x;
   View<Acc*, MI355XSpace, MemoryTraits<Unmanaged>> y;
   KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, Acc& u, const bool final) const {
@@ -353,6 +372,7 @@ struct BatchScanF {
 };
 template <class In, class Acc>
 struct BatchAxpbF {
+  static constexpr double kkmx_bytes_per_index = sizeof(In) + sizeof(Acc), kkmx_flops_per_index = 2;
   View<const In*, MI355XSpace, MemoryTraits<Unmanaged>> x;
   View<Acc*, MI355XSpace, MemoryTraits<Unmanaged>> y;
   Acc a, b;

@@ -236,6 +236,36 @@ inline void md_set_depth(MDLaunch& l, int block) {
   l.groups = l.ntiles / te * ((te + depth - 1) / depth);
 }
 
+// MDRange tiles a tuning tool chooses from when the policy leaves the tile
+// to the implementation: the fastest dimension takes 16 << (c % 4) points,
+// the next 1 << (c / 4), the others 1 (reference MDRangeTuner,
+// core/src/impl/Kokkos_Tools_Generic.hpp:106-122,340-353).
+template <class Policy, class F>
+Policy md_tuned_policy(const Policy& p, int pattern, size_t value_bytes, bool floating) {
+  if (!p.impl_tune_tile_size() || !Tools::Experimental::have_tuning_tool()) return p;
+  constexpr unsigned R = Policy::rank;
+  const bool left      = Policy::inner_direction == Iterate::Left;
+  const int fast = left ? 0 : int(R) - 1, next = left ? 1 : int(R) - 2;
+  int64_t points = 1;
+  for (unsigned r = 0; r < R; ++r) points *= (p.upper()[r] - p.lower()[r]);
+  int h = -1;
+  for (int c = 0; c < 16; ++c) {
+    bool match = p.tile_size()[fast] == (16 << (c % 4)) && p.tile_size()[next] == (1 << (c / 4));
+    for (int r = 0; r < int(R); ++r)
+      if (r != fast && r != next && p.tile_size()[r] != 1) match = false;
+    if (match) h = c;
+  }
+  const int c = request_launch_candidate<F>(LaunchVar::MDRange, pattern, points, value_bytes, floating, h);
+  if (c < 0 || c == h) return p;
+  typename Policy::tile_type t;
+  for (unsigned r = 0; r < R; ++r) t[r] = 1;
+  t[fast] = 16 << (c % 4);
+  t[next] = 1 << (c / 4);
+  Policy q = p;
+  q.impl_change_tile_size(t);
+  return q;
+}
+
 template <class F, class... Traits>
 class ParallelFor<F, MDRangePolicy<Traits...>, MI355X> {
  public:
@@ -243,7 +273,7 @@ class ParallelFor<F, MDRangePolicy<Traits...>, MI355X> {
   static constexpr unsigned R = Policy::rank;
   static constexpr bool IL = Policy::inner_direction == Iterate::Left;
   static constexpr bool OL = Policy::outer_direction == Iterate::Left;
-  ParallelFor(const F& f, const Policy& p) : m_f(f), m_p(p) {}
+  ParallelFor(const F& f, const Policy& p) : m_f(f), m_p(md_tuned_policy<Policy, F>(p, 0, 0, false)) {}
   void execute() const {
     MDLaunch l = md_launch(m_p);
     if (l.ntiles == 0) return;
@@ -278,7 +308,10 @@ class ParallelReduce<CR, MDRangePolicy<Traits...>, MI355X> {
   static constexpr bool OL = Policy::outer_direction == Iterate::Left;
   template <class Mem>
   ParallelReduce(const CR& cr, const Policy& p, V* result, Mem)
-      : m_cr(cr), m_p(p), m_result(result), m_on_device(exec_can_access<MI355X, Mem>::value) {}
+      : m_cr(cr),
+        m_p(md_tuned_policy<Policy, typename CR::functor_type>(p, 1, sizeof(V), std::is_floating_point<V>::value)),
+        m_result(result),
+        m_on_device(exec_can_access<MI355X, Mem>::value) {}
   void execute() const {
     static_assert(!CR::is_array, "MDRange reductions with runtime-length value types are not supported");
     MI355XInstance* inst = m_p.space().impl_internal_space_instance();

@@ -463,6 +463,112 @@ __global__ __launch_bounds__(1024) void k_range_scan(
   }
 }
 
+// Register-resident variant: each lane keeps its IPT contributions from the
+// first functor pass in registers, so the tile is read from memory once; the
+// second (final) pass only re-reads what the functor itself loads, from
+// cache.  Lane l of wavefront w holds index base + r*64 for r < IPT (one
+// contiguous 64-index run per round).  Used while IPT values fit in about
+// 256 bytes of registers per lane; larger IPT takes the two-pass tile above.
+template <class CR, class Index, int IPT, bool WithTotal>
+__global__ __launch_bounds__(1024) void k_range_scan_rr(
+    const CR cr, const int64_t begin, const int64_t end, const int64_t ntiles,
+    ScanTileState<typename CR::value_type> st, unsigned* control,
+    typename CR::value_type* total_out) {
+  using V   = typename CR::value_type;
+  using Tag = typename CR::work_tag;
+  __shared__ LdsSlots16<V> s_wave;
+  __shared__ LdsSlots16<V> s_red;
+  __shared__ uint64_t s_claim;
+  __shared__ int s_pos;
+  const auto join      = [&cr](V& a, const V& b) { cr.join(a, b); };
+  uint64_t* claim_word = reinterpret_cast<uint64_t*>(control + 2);
+  if (threadIdx.x == 0)
+    s_claim = __hip_atomic_fetch_add(claim_word, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const uint64_t claim = s_claim;
+  const int64_t tile   = static_cast<int64_t>(static_cast<uint32_t>(claim));
+  const uint32_t epoch = static_cast<uint32_t>(claim >> 32);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6, l = wave::lane();
+  V identity;
+  cr.init(identity);
+  const int64_t tile_elems = static_cast<int64_t>(blockDim.x) * IPT;
+  const int64_t tbase      = begin + tile * tile_elems;
+  const int64_t base       = tbase + static_cast<int64_t>(w) * (64 * IPT) + l;
+  const bool full          = tbase + tile_elems <= end;
+
+  // Separate unguarded loop for full tiles: all IPT loads issue back to back.
+  V v[IPT];
+  if (full) {
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+      v[r] = identity;
+      call_functor<Tag>(cr.functor(), static_cast<Index>(base + r * 64), v[r], false);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+      const int64_t i = base + r * 64;
+      v[r]            = identity;
+      if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), v[r], false);
+    }
+  }
+  V inc[IPT];
+  V wave_total = identity;
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) {
+    inc[r] = wave::inclusive_scan(v[r], identity, join);
+    join(wave_total, wave::readlane(inc[r], 63));
+  }
+  if (l == 0) s_wave.get()[w] = wave_total;
+  __syncthreads();
+  const V x    = (l < nw) ? s_wave.get()[l] : identity;
+  const V xs   = wave::inclusive_scan(x, identity, join);
+  const V tagg = wave::readlane(xs, nw - 1);
+  const V wpre = (w == 0) ? identity : wave::readlane(xs, w - 1);
+
+  V prefix = identity;
+  if (tile == 0) {
+    if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanIncl), tagg);
+  } else {
+    if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanAgg), tagg);
+    prefix = scan_lookback(st, tile, epoch, identity, join, s_red.get(), &s_pos);
+    if (threadIdx.x == 0) {
+      V t = prefix;
+      join(t, tagg);
+      st.publish(tile, scan_flag(epoch, kScanIncl), t);
+    }
+  }
+  if (threadIdx.x == 0 && tile == ntiles - 1) {
+    if constexpr (WithTotal) {
+      V tot = prefix;
+      join(tot, tagg);
+      *total_out = tot;
+    }
+    __hip_atomic_store(claim_word, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  V running = prefix;
+  join(running, wpre);
+  if (full) {
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+      V u = running;
+      join(u, wave::shift_up(inc[r], identity));
+      join(running, wave::readlane(inc[r], 63));
+      call_functor<Tag>(cr.functor(), static_cast<Index>(base + r * 64), u, true);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+      V u = running;
+      join(u, wave::shift_up(inc[r], identity));
+      join(running, wave::readlane(inc[r], 63));
+      const int64_t i = base + r * 64;
+      if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), u, true);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ host helpers
 template <class V>
 struct ResultSink {
@@ -606,23 +712,31 @@ struct functor_cost_hint<F, std::void_t<decltype(F::kkmx_bytes_per_index), declt
   static constexpr double bytes = F::kkmx_bytes_per_index, flops = F::kkmx_flops_per_index;
 };
 
-struct RangeTuningIds {
-  size_t out = 0, pattern = 0, extent = 0, value_bytes = 0, functor_bytes = 0, floating = 0, bytes = 0, flops = 0,
-         cus = 0;
+// The three launch-shape variables (Range launch shape, Team shape, MDRange
+// tile) share one set of declared inputs; policy_kind tells them apart.
+enum class LaunchVar : int { Range = 0, Team = 1, MDRange = 2 };
+
+struct LaunchTuningIds {
+  size_t out[3]  = {0, 0, 0};
+  size_t pattern = 0, extent = 0, value_bytes = 0, functor_bytes = 0, floating = 0, bytes = 0, flops = 0,
+         cus = 0, kind = 0;
 };
-inline const RangeTuningIds& range_tuning_ids() {
-  static const RangeTuningIds ids = [] {
+inline const LaunchTuningIds& launch_tuning_ids() {
+  static const LaunchTuningIds ids = [] {
     using namespace Tools::Experimental;
-    RangeTuningIds r;
+    LaunchTuningIds r;
     static int64_t cand[kRangeCandidates];
     for (int k = 0; k < kRangeCandidates; ++k) cand[k] = k;
-    VariableInfo o{};
-    o.type          = kokkos_value_int64;
-    o.category      = kokkos_value_categorical;
-    o.valueQuantity = kokkos_value_set;
-    o.candidates    = make_candidate_set(kRangeCandidates, cand);
-    r.out           = declare_output_type("kokkos.mi355x.range_launch", o);
-    auto in         = [](const char* name, ValueType t, StatisticalCategory cat) {
+    const char* names[3] = {"kokkos.mi355x.range_launch", "kokkos.mi355x.team_shape", "kokkos.mi355x.mdrange_tile"};
+    for (int v = 0; v < 3; ++v) {
+      VariableInfo o{};
+      o.type          = kokkos_value_int64;
+      o.category      = kokkos_value_categorical;
+      o.valueQuantity = kokkos_value_set;
+      o.candidates    = make_candidate_set(kRangeCandidates, cand);
+      r.out[v]        = declare_output_type(names[v], o);
+    }
+    auto in = [](const char* name, ValueType t, StatisticalCategory cat) {
       VariableInfo i{};
       i.type          = t;
       i.category      = cat;
@@ -637,36 +751,47 @@ inline const RangeTuningIds& range_tuning_ids() {
     r.bytes         = in("kokkos.mi355x.bytes_per_index", kokkos_value_double, kokkos_value_ratio);
     r.flops         = in("kokkos.mi355x.flops_per_index", kokkos_value_double, kokkos_value_ratio);
     r.cus           = in("kokkos.mi355x.cu_count", kokkos_value_int64, kokkos_value_ratio);
+    r.kind          = in("kokkos.mi355x.policy_kind", kokkos_value_int64, kokkos_value_categorical);
     return r;
   }();
   return ids;
 }
 
+// Asks the tuning tool for one of the 16 candidates of `var`; returns the
+// answer, or `dflt` when no tool is loaded or it leaves the default.
 // pattern: 0 for, 1 reduce, 2 scan.
 template <class F>
-RangeConfig tuned_range_config(int pattern, int64_t n, size_t value_bytes, bool floating,
-                               const RangeConfig& heuristic) {
+int request_launch_candidate(LaunchVar var, int pattern, int64_t n, size_t value_bytes, bool floating, int dflt) {
   using namespace Tools::Experimental;
-  if (!have_tuning_tool()) return heuristic;
-  const RangeTuningIds& id = range_tuning_ids();
-  const size_t ctx         = get_new_context_id();
+  if (!have_tuning_tool()) return dflt;
+  const LaunchTuningIds& id = launch_tuning_ids();
+  const size_t ctx          = get_new_context_id();
   begin_context(ctx);
-  VariableValue in[8] = {make_variable_value(id.pattern, int64_t(pattern)),
+  VariableValue in[9] = {make_variable_value(id.pattern, int64_t(pattern)),
                          make_variable_value(id.extent, int64_t(n)),
                          make_variable_value(id.value_bytes, int64_t(value_bytes)),
                          make_variable_value(id.functor_bytes, int64_t(sizeof(F))),
                          make_variable_value(id.floating, int64_t(floating ? 1 : 0)),
                          make_variable_value(id.bytes, functor_cost_hint<F>::bytes),
                          make_variable_value(id.flops, functor_cost_hint<F>::flops),
-                         make_variable_value(id.cus, int64_t(MI355X::impl_cu_count()))};
-  set_input_values(ctx, 8, in);
-  const int h       = range_candidate_index(pattern, heuristic);
-  VariableValue out = make_variable_value(id.out, int64_t(h < 0 ? -1 : h));
+                         make_variable_value(id.cus, int64_t(MI355X::impl_cu_count())),
+                         make_variable_value(id.kind, int64_t(var))};
+  set_input_values(ctx, 9, in);
+  VariableValue out = make_variable_value(id.out[int(var)], int64_t(dflt));
   request_output_values(ctx, 1, &out);
   end_context(ctx);
   const int64_t c = out.value.int_value;
-  if (c < 0 || c >= kRangeCandidates || c == h) return heuristic;
-  return range_candidate(pattern, static_cast<int>(c), n);
+  return (c >= 0 && c < kRangeCandidates) ? static_cast<int>(c) : dflt;
+}
+
+template <class F>
+RangeConfig tuned_range_config(int pattern, int64_t n, size_t value_bytes, bool floating,
+                               const RangeConfig& heuristic) {
+  if (!Tools::Experimental::have_tuning_tool()) return heuristic;
+  const int h = range_candidate_index(pattern, heuristic);
+  const int c = request_launch_candidate<F>(LaunchVar::Range, pattern, n, value_bytes, floating, h);
+  if (c < 0 || c == h) return heuristic;
+  return range_candidate(pattern, c, n);
 }
 
 }  // namespace Impl
@@ -789,11 +914,21 @@ class ParallelReduce<CR, RangePolicy<Traits...>, MI355X> {
 
 template <class CR, class Index, bool WithTotal>
 const void* range_scan_kernel(int ipt) {
+  // Register-resident while IPT values take at most 256 bytes per lane
+  // (measured: 32 int64 values per lane at 1024 lanes still run at 5 waves
+  // per SIMD and beat the two-pass tile by 2x).
+  constexpr int rr_max = int(256 / sizeof(typename CR::value_type));
   switch (ipt >= 64 ? 64 : ipt >= 32 ? 32 : ipt >= 16 ? 16 : 8) {
     case 64: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 64, WithTotal>);
-    case 32: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 32, WithTotal>);
-    case 16: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 16, WithTotal>);
-    default: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 8, WithTotal>);
+    case 32:
+      if constexpr (rr_max >= 32) return reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 32, WithTotal>);
+      else return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 32, WithTotal>);
+    case 16:
+      if constexpr (rr_max >= 16) return reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 16, WithTotal>);
+      else return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 16, WithTotal>);
+    default:
+      if constexpr (rr_max >= 8) return reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 8, WithTotal>);
+      else return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 8, WithTotal>);
   }
 }
 

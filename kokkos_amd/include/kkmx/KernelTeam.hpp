@@ -743,14 +743,37 @@ __global__ __launch_bounds__(1024) void k_team_reduce(const CR cr, const TeamLau
   }
 }
 
-// Launch shape of a team dispatch.
+// Team shapes a tuning tool chooses from (team_size, vector_length), all
+// within 1024 lanes.  The tool's answer only replaces the parts the policy
+// leaves to Kokkos::AUTO (reference TeamSizeTuner,
+// core/src/impl/Kokkos_Tools_Generic.hpp:219-230).
+constexpr int kTeamCandidates[16][2] = {{64, 1},  {128, 1}, {256, 1}, {512, 1}, {1024, 1}, {32, 2},
+                                        {64, 2},  {256, 2}, {16, 4},  {32, 4},  {64, 4},   {128, 4},
+                                        {16, 8},  {32, 8},  {64, 8},  {128, 8}};
+inline int team_candidate_index(int ts, int vl) {
+  for (int c = 0; c < 16; ++c)
+    if (kTeamCandidates[c][0] == ts && kTeamCandidates[c][1] == vl) return c;
+  return -1;
+}
+
+// Launch shape of a team dispatch.  pattern: 0 for, 1 reduce.
 template <class Policy, class F>
 TeamLaunch team_launch_shape(const Policy& policy, const F& f, MI355XInstance* inst, int& block, int& grid,
-                             size_t& lds) {
+                             size_t& lds, int pattern = 0, size_t value_bytes = 0, bool floating = false) {
   TeamLaunch p{};
   p.vector_length = policy.impl_vector_length() > 0 ? policy.impl_vector_length() : 1;
   int ts          = policy.impl_team_size_raw();
   if (ts <= 0) ts = policy.impl_default_team_size();
+  if ((policy.impl_auto_team_size() || policy.impl_auto_vector_length()) &&
+      Tools::Experimental::have_tuning_tool()) {
+    const int h = team_candidate_index(ts, p.vector_length);
+    const int c = request_launch_candidate<F>(LaunchVar::Team, pattern, policy.league_size(), value_bytes, floating, h);
+    if (c >= 0 && c != h) {
+      if (policy.impl_auto_team_size()) ts = kTeamCandidates[c][0];
+      if (policy.impl_auto_vector_length()) p.vector_length = kTeamCandidates[c][1];
+      if (ts * p.vector_length > 1024) ts = 1024 / p.vector_length;
+    }
+  }
   if (ts * p.vector_length > 1024)
     throw_runtime_exception("Kokkos::MI355X TeamPolicy: team_size * vector_length exceeds 1024");
   p.team_size   = ts;
@@ -814,7 +837,8 @@ class ParallelReduce<CR, TeamPolicy<Traits...>, MI355X> {
     std::lock_guard<std::mutex> lk(inst->mtx);
     int block = 0, grid = 0;
     size_t lds = 0;
-    TeamLaunch tl = team_launch_shape(m_p, m_cr.functor(), inst, block, grid, lds);
+    TeamLaunch tl = team_launch_shape(m_p, m_cr.functor(), inst, block, grid, lds, 1, sizeof(V),
+                                      std::is_floating_point<V>::value);
     if (grid == 0) grid = 1;
     V* dst         = m_result;
     bool copy_back = false;

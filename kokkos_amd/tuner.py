@@ -35,16 +35,17 @@ def _libs():
     S = load("kkmx_sweep")
     T = load("kkmx_tuner")
     S.kkmx_sweep_family_count.restype = C.c_int
-    S.kkmx_sweep_family_info.argtypes = [C.c_int] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_double)] * 2
+    S.kkmx_sweep_family_info.argtypes = ([C.c_int] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_double)] * 2 +
+                                         [C.POINTER(C.c_int)])
     S.kkmx_sweep_setup.argtypes = [C.c_int64]
-    S.kkmx_sweep_time.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, f32p]
+    S.kkmx_sweep_time.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, f32p, C.POINTER(C.c_int64)]
     S.kkmx_sweep_last_error.restype = C.c_char_p
     T.kkmx_mlp_param_count.restype = C.c_int
     T.kkmx_mlp_train.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_int,
                                  f32p, f32p]
     T.kkmx_mlp_predict.argtypes = [f32p, f32p, C.c_int, f32p]
     T.kkmx_tuner_features.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
-                                      C.c_int, f32p]
+                                      C.c_int, C.c_int, f32p]
     T.kkmx_tuner_load_model.argtypes = [C.c_char_p]
     T.kkmx_tuner_stats.argtypes = [C.POINTER(C.c_uint64)] * 3
     T.kkmx_tuner_last_error.restype = C.c_char_p
@@ -59,11 +60,12 @@ def families():
     S, _ = _libs()
     out = []
     for f in range(S.kkmx_sweep_family_count()):
-        p, vb, fl, fb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        p, vb, fl, fb, kd = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         by, fo = C.c_double(), C.c_double()
-        S.kkmx_sweep_family_info(f, C.byref(p), C.byref(vb), C.byref(fl), C.byref(fb), C.byref(by), C.byref(fo))
+        S.kkmx_sweep_family_info(f, C.byref(p), C.byref(vb), C.byref(fl), C.byref(fb), C.byref(by), C.byref(fo),
+                                 C.byref(kd))
         out.append(dict(pattern=p.value, value_bytes=vb.value, floating=fl.value, functor_bytes=fb.value,
-                        bytes=by.value, flops=fo.value))
+                        bytes=by.value, flops=fo.value, kind=kd.value))
     return out
 
 
@@ -71,7 +73,7 @@ def raw_features(fam: dict, extent: float, cus: int) -> np.ndarray:
     _, T = _libs()
     f = np.zeros(NF, np.float32)
     T.kkmx_tuner_features(fam["pattern"], float(extent), float(fam["value_bytes"]), float(fam["functor_bytes"]),
-                          fam["floating"], fam["bytes"], fam["flops"], cus, _fp(f))
+                          fam["floating"], fam["bytes"], fam["flops"], cus, fam.get("kind", 0), _fp(f))
     return f
 
 
@@ -87,16 +89,21 @@ def collect(extents_per_family: int = 160, seed: int = 2024, lo: float = 8.0, hi
     rng = np.random.default_rng(seed)
     fam_ids, extents, times, feats = [], [], [], []
     buf = np.zeros(NC + 1, np.float32)
+    neff = C.c_int64()
+    # extent ranges (log2) per policy kind: Range indices, Team leagues,
+    # MDRange points
+    ranges = {0: (lo, hi), 1: (6.0, 18.0), 2: (12.0, hi)}
     try:
         for f in range(len(fams)):
-            ns = np.unique(np.floor(np.exp2(rng.uniform(lo, hi, extents_per_family))).astype(np.int64))
+            a, b = ranges[fams[f]["kind"]]
+            ns = np.unique(np.floor(np.exp2(rng.uniform(a, b, extents_per_family))).astype(np.int64))
             for n in ns:
-                if S.kkmx_sweep_time(f, int(n), reps, -1, 1, _fp(buf)) != 0:
+                if S.kkmx_sweep_time(f, int(n), reps, -1, 1, _fp(buf), C.byref(neff)) != 0:
                     raise KkmxError(S.kkmx_sweep_last_error().decode())
                 fam_ids.append(f)
-                extents.append(int(n))
+                extents.append(int(neff.value))
                 times.append(buf.copy())
-                feats.append(raw_features(fams[f], n, cus))
+                feats.append(raw_features(fams[f], neff.value, cus))
             if progress:
                 progress(f"family {f} ({len(ns)} extents) done, {len(times)} groups", )
     finally:
@@ -188,26 +195,56 @@ def tool_stats():
     return dict(requests=a.value, answered=b.value, inferences=c.value)
 
 
-def fit_and_evaluate(data: dict, seed: int = 11, holdout: float = 0.2, **train_kw) -> dict:
+def hintless_features(data: dict, cus: int = 256) -> np.ndarray:
+    """Features of every group as a functor without the optional
+    kkmx_bytes_per_index / kkmx_flops_per_index hints would present them."""
+    fams = [dict(f, bytes=0.0, flops=0.0) for f in data["families"]]
+    return np.array([raw_features(fams[f], e, cus) for f, e in zip(data["family"], data["extent"])], np.float32)
+
+
+def fit_and_evaluate(data: dict, seed: int = 11, holdout: float = 0.2, hint_dropout: bool = True, cus: int = 256,
+                     **train_kw) -> dict:
     """Split groups, train on the rest, report regret of MLP vs heuristic on
-    the held-out groups (and on all groups)."""
+    the held-out groups (and on all groups).
+
+    hint_dropout: the training set also holds every training group with the
+    functor's cost hints removed, so functors that declare none (most user
+    code) still get answers learned from their pattern, types and extent."""
     rng = np.random.default_rng(seed)
     G = len(data["times"])
     idx = rng.permutation(G)
     nt = int(G * (1 - holdout))
     tr, te = idx[:nt], idx[nt:]
-    mean, inv = normaliser(data["feats"][tr])
-    Xn = (data["feats"] - mean) * inv
     Y = soft_targets(data["times"])
-    params, loss = train(Xn[tr], Y[tr], **train_kw)
-    Z = predict(params, Xn)
+    X_tr, Y_tr = data["feats"][tr], Y[tr]
+    hl = None
+    if hint_dropout:
+        hl = hintless_features(data, cus)
+        X_tr = np.concatenate([X_tr, hl[tr]])
+        Y_tr = np.concatenate([Y_tr, Y[tr]])
+    mean, inv = normaliser(X_tr)
+    params, loss = train((X_tr - mean) * inv, Y_tr, **train_kw)
+    Z = predict(params, (data["feats"] - mean) * inv)
     choice = Z.argmax(axis=1)
     r_mlp = regret(data["times"], choice)
     r_heur = regret(data["times"], np.full(G, NC))
     res = dict(
         groups=G, samples=int(G * NC), train_groups=int(nt), test_groups=int(G - nt),
-        loss_first=float(loss[0]), loss_last=float(loss[-1]),
+        loss_first=float(loss[0]), loss_last=float(loss[-1]), hint_dropout=bool(hint_dropout),
         test=dict(mlp=summary(r_mlp[te]), heuristic=summary(r_heur[te])),
         train=dict(mlp=summary(r_mlp[tr]), heuristic=summary(r_heur[tr])),
     )
+    if hl is not None:
+        c_hl = predict(params, (hl - mean) * inv).argmax(axis=1)
+        res["test_without_hints"] = dict(mlp=summary(regret(data["times"], c_hl)[te]))
+    # held-out regret per policy kind (RangePolicy launch shape, Team shape,
+    # MDRange tile)
+    kinds = np.array([data["families"][f].get("kind", 0) for f in data["family"]])
+    is_te = np.zeros(G, bool)
+    is_te[te] = True
+    res["test_by_kind"] = {
+        name: dict(groups=int((is_te & (kinds == k)).sum()), mlp=summary(r_mlp[is_te & (kinds == k)]),
+                   heuristic=summary(r_heur[is_te & (kinds == k)]))
+        for k, name in enumerate(("range", "team", "mdrange")) if (is_te & (kinds == k)).any()
+    }
     return dict(params=params, mean=mean, inv_std=inv, loss=loss, report=res, choice=choice)

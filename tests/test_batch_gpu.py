@@ -93,9 +93,30 @@ def test_batch_modes_match_oracle(paths, oracle, nops, seed, lo, hi):
                 # the fused kernel's combination order is fixed: replays and
                 # the graph give the same float64 bits
                 if f64_ref is None:
-                    f64_ref = gi.copy()
-                assert np.array_equal(gi, f64_ref)
+                    f64_ref = gf.copy()
+                assert np.array_equal(gf, f64_ref)
     print(f"max f64 rel err {err:.2e}, tiles {run.tiles}")
+    run.close()
+
+
+@pytest.mark.parametrize("shape", range(8))
+def test_batch_tile_shapes_match_oracle(paths, oracle, shape):
+    """Every tile shape of the fused kernel gives the oracle's results, and
+    replays of one shape give the same float64 bits."""
+    wl = small_workload(200, 11, 0, 16)
+    run = B.BatchRun(wl, shape)
+    exp = expected(oracle, wl, run.inputs())
+    bits = None
+    for mode in (B.MODE_FUSED, B.MODE_FUSED_GRAPH):
+        run.clear_outputs()
+        print(f"batch shape={shape} mode={mode}", flush=True)
+        run.run(mode, 2)
+        K.fence()
+        check_run(run, oracle, wl, exp)
+        gf = run.results()[1]
+        if bits is None:
+            bits = gf.copy()
+        assert np.array_equal(gf, bits)
     run.close()
 
 

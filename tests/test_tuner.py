@@ -29,7 +29,7 @@ def test_model_file_round_trip(tmp_path):
 
 
 def test_tool_features_match_their_definition():
-    fam = dict(pattern=1, value_bytes=8, floating=1, functor_bytes=24, bytes=8.0, flops=1.0)
+    fam = dict(pattern=1, value_bytes=8, floating=1, functor_bytes=24, bytes=8.0, flops=1.0, kind=2)
     f = T.raw_features(fam, 2.0 ** 20, 256)
     want = [20 / 32, 0, 1, 0, np.log2(9) / 4, np.log2(25) / 10, 1, np.log2(9) / 6, 1 / 8,
             np.log2(2.0 ** 20 / (256 * 2048) + 2.0 ** -20) / 20, np.log2(2.0 ** 23 + 1) / 40, 1.0]
