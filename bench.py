@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--only", choices=["small_reduce_scan", "mdrange_team", "atomics", "batched"],
+                    help="run one secondary workload alone and print its JSON (for profiling); no headline line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -291,6 +293,11 @@ def main():
     import kokkos_amd as K
 
     P = K.init(0)
+    if args.only:
+        legs = {"small_reduce_scan": leg_small, "mdrange_team": leg_md_team, "atomics": leg_atomics,
+                "batched": leg_batched}
+        print(json.dumps({args.only: legs[args.only](K, P, K.EventTimer())}), flush=True)
+        return
     props = K.device_props()
     stream = P.kkmx_stream()
     rt = K.load("kkmx_rt")
