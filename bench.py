@@ -217,8 +217,10 @@ def leg_atomics(K, P, timer):
 def leg_batched(K, P, timer, replays=1000, plain_steps=5):
     """4096 uneven reduce/scan ops per step (seed 10): per-call launches,
     the persistent fused kernel (direct and replayed from a hipGraph) and,
-    when a trained model is present, per-call launches with the learned
-    tuner loaded as a Kokkos Tools library."""
+    when a trained model is present, the learned tuner loaded as a Kokkos
+    Tools library: per-call launches with tuned launch shapes, the fused
+    kernel with the tuned tile shape, and a batch of small ops (2^4..2^12
+    elements) with the default against the tuned tile shape."""
     from kokkos_amd import batch as B
     from kokkos_amd import workloads as W
 
@@ -238,7 +240,7 @@ def leg_batched(K, P, timer, replays=1000, plain_steps=5):
     ms = per_step(B.MODE_FUSED_GRAPH, replays)
     gi, gf = (a.copy() for a in run.results())
     out["fused_graph"] = {"ms_per_step": round(ms, 4), "ops_per_s": round(4096 / ms * 1e3, 1),
-                          "gbps": round(nbytes / ms / 1e6, 1), "replays": replays}
+                          "gbps": round(nbytes / ms / 1e6, 1), "replays": replays, "tile_shape": run.shape}
     ms = per_step(B.MODE_FUSED, 50)
     out["fused"] = {"ms_per_step": round(ms, 4), "ops_per_s": round(4096 / ms * 1e3, 1)}
     ms = per_step(B.MODE_PLAIN, plain_steps)
@@ -263,10 +265,45 @@ def leg_batched(K, P, timer, replays=1000, plain_steps=5):
             out["plain_tuned"] = {"ms_per_step": round(ms, 3), "ops_per_s": round(4096 / ms * 1e3, 1),
                                   "tool": T.tool_stats(),
                                   "int_results_identical": bool(np.array_equal(ti[isint], pi[isint]))}
+            # the fused kernel with its tile shape chosen by the learned tuner
+            # at plan time (the default shape is the heuristic)
+            run.close()
+            run = B.BatchRun(wl)
+            ms = per_step(B.MODE_FUSED_GRAPH, replays)
+            ui, uf = run.results()
+            out["fused_graph_tuned"] = {
+                "ms_per_step": round(ms, 4), "ops_per_s": round(4096 / ms * 1e3, 1), "tile_shape": run.shape,
+                "default_tile_shape": out["fused_graph"]["tile_shape"],
+                "int_results_identical": bool(np.array_equal(ui[isint], pi[isint])),
+                "f64_max_rel_diff_vs_plain": float(np.max(np.abs(uf[~isint] - pf[~isint]) / np.maximum(ref, 1e-300)))}
+            # a batch of small ops (4096 ops of 2^4..2^12 elements): default
+            # tile shape against the tuner's choice
+            run.close()
+            small = W.batch_workload(4096, 11, 4.0, 12.0)
+            sm = {}
+            for name, shape in (("default", None), ("tuned", -1)):
+                if shape is None:
+                    K.check(P.kkmx_tools_unload(), P)
+                else:
+                    K.check(P.kkmx_tools_load(lib), P)
+                    T.load_into_tool(model)
+                run = B.BatchRun(small)
+                ms = per_step(B.MODE_FUSED_GRAPH, replays)
+                sm[name] = {"ms_per_step": round(ms, 4), "ops_per_s": round(4096 / ms * 1e3, 1), "tile_shape": run.shape}
+                if name == "default":
+                    si = run.results()[0].copy()
+                else:
+                    sm["int_results_identical"] = bool(np.array_equal(run.results()[0][small["type"] != W.T_F64],
+                                                                      si[small["type"] != W.T_F64]))
+                run.close()
+            sm["speedup_tuned_vs_default"] = round(sm["tuned"]["ops_per_s"] / sm["default"]["ops_per_s"], 3)
+            out["small_ops_fused_graph"] = sm
+            run = None
         finally:
             K.check(P.kkmx_tools_unload(), P)
     out["speedup_fused_graph_vs_plain"] = round(out["fused_graph"]["ops_per_s"] / out["plain"]["ops_per_s"], 2)
-    run.close()
+    if run is not None:
+        run.close()
     return out
 
 
@@ -444,6 +481,7 @@ def main():
             "fused_graph": bt["fused_graph"]["ops_per_s"],
             "plain": bt["plain"]["ops_per_s"],
             "plain_tuned": bt.get("plain_tuned", {}).get("ops_per_s"),
+            "fused_graph_tuned": bt.get("fused_graph_tuned", {}).get("ops_per_s"),
             "speedup_vs_plain": bt["speedup_fused_graph_vs_plain"],
         }
     print(json.dumps(line))

@@ -129,8 +129,12 @@ int kkmx_batch_add(kkmx_batch* b, int op, int type, const void* in_dev, int64_t 
 /* Tile shape of the fused kernel, before plan (lanes x consecutive elements
  * per lane and round x rounds): 0 = 256x1x16, 1 = 256x4x4, 2 = 256x4x8,
  * 3 = 512x4x4, 4 = 1024x1x16, 5 = 512x4x8 (the default), 6 = 1024x4x4,
- * 7 = 512x1x32. */
+ * 7 = 512x1x32.  Without it, the plan asks a loaded tuning tool for the
+ * shape (output variable kokkos.mi355x.batch_shape). */
 int kkmx_batch_set_shape(kkmx_batch* b, int shape);
+/* The tile shape in use: the one set, else after the plan the learned
+ * tuner's choice when a tuning tool is loaded, else the default. */
+int kkmx_batch_get_shape(kkmx_batch* b, int* shape);
 int kkmx_batch_plan(kkmx_batch* b, int64_t* tiles_out);
 int kkmx_batch_run(kkmx_batch* b, int mode, int steps);
 

@@ -112,6 +112,7 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_batch_add", [vp, C.c_int, C.c_int, vp, i64, vp, vp, f64, f64])
         _sig(lib, "kkmx_batch_plan", [vp, C.POINTER(i64)])
         _sig(lib, "kkmx_batch_set_shape", [vp, C.c_int])
+        _sig(lib, "kkmx_batch_get_shape", [vp, C.POINTER(C.c_int)])
         _sig(lib, "kkmx_set_mdrange_depth", [C.c_int])
         _sig(lib, "kkmx_batch_run", [vp, C.c_int, C.c_int])
     elif name == "kkmx_peak":

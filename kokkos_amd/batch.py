@@ -47,6 +47,9 @@ class BatchRun:
         tiles = C.c_int64()
         check(P.kkmx_batch_plan(h, C.byref(tiles)), P)
         self.tiles = tiles.value
+        sh = C.c_int()
+        check(P.kkmx_batch_get_shape(h, C.byref(sh)), P)
+        self.shape = sh.value
         check(P.kkmx_fence(), P)
 
     def run(self, mode: int, steps: int = 1) -> None:

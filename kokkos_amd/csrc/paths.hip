@@ -758,6 +758,11 @@ int kkmx_batch_set_shape(kkmx_batch* b, int shape) {
   if (!b) return reject("kkmx_batch_set_shape: null handle");
   return guarded([&] { b->batch.set_shape(shape); });
 }
+int kkmx_batch_get_shape(kkmx_batch* b, int* shape) {
+  if (!b || !shape) return reject("kkmx_batch_get_shape: null argument");
+  *shape = b->batch.shape();
+  return 0;
+}
 int kkmx_batch_plan(kkmx_batch* b, int64_t* tiles_out) {
   if (!b) return reject("kkmx_batch_plan: null handle");
   return guarded([&] {

@@ -9,6 +9,7 @@
 // on the instance stream.  Plain Kokkos user code: the candidates are applied
 // through the same launch-shape override the tuning hooks use.
 #include <Kokkos_Core.hpp>
+#include <Kokkos_Batch.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -299,7 +300,8 @@ struct InitBuffers {
 };
 
 constexpr int kRangeFamilies = 20;
-constexpr int kFamilies      = 28;
+constexpr int kFirstBatch    = 28;
+constexpr int kFamilies      = 32;
 
 // pattern, value bytes, floating, functor bytes, bytes/index, flops/index
 struct FamilyInfo {
@@ -308,7 +310,7 @@ struct FamilyInfo {
   int floating;
   int functor_bytes;
   double bytes, flops;
-  int kind;  // 0 RangePolicy, 1 TeamPolicy, 2 MDRangePolicy
+  int kind;  // 0 RangePolicy, 1 TeamPolicy, 2 MDRangePolicy, 3 batched dispatch
 };
 
 template <class F>
@@ -346,9 +348,65 @@ FamilyInfo family_info(int f) {
     case 24: return info<Md7Point>(0, 0, 0, 2);
     case 25: return info<Md5Point2D>(0, 0, 0, 2);
     case 26: return info<MdAxpy3>(0, 0, 0, 2);
-    default: return info<MdSumSq3>(1, 8, 1, 2);
+    case 27: return info<MdSumSq3>(1, 8, 1, 2);
+    // batched dispatch: the features depend on the generated op list and
+    // come from kkmx_sweep_last_features
+    default: return FamilyInfo{3, 0, 0, 0, 0.0, 1.0, 3};
   }
 }
+
+// ---- batched dispatch families (extent = element budget of the batch):
+// ops with log-uniform extents in [2^lo, 2^hi], half sums and half exclusive
+// scans, element types int32 / int64 / float64 in turn (or float64 only).
+struct BatchFamily {
+  double lo, hi;
+  bool f64_only;
+};
+BatchFamily batch_family(int f) {
+  switch (f - kFirstBatch) {
+    case 0: return {8, 20, false};
+    case 1: return {4, 12, false};
+    case 2: return {12, 22, false};
+    default: return {8, 16, true};
+  }
+}
+uint64_t sweep_mix(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+// Adds the ops of family f with element budget `budget` (<= the sweep
+// buffers): inputs are consecutive slices of buffer a, scan outputs of b,
+// results slots of c.  Returns the elements added.
+int64_t batch_ops(int f, int64_t budget, Kokkos::Experimental::BatchDispatch& bd) {
+  using namespace Kokkos::Experimental;
+  const BatchFamily bf = batch_family(f);
+  uint64_t st          = sweep_mix(uint64_t(f) * 0x100000001ull + uint64_t(budget));
+  int64_t used = 0, out = 0, nres = 0;
+  while (nres < 16384 && nres < g_buf.n) {
+    st               = sweep_mix(st);
+    const double u   = double(st >> 11) * 0x1.0p-53;
+    int64_t size     = int64_t(std::exp2(bf.lo + (bf.hi - bf.lo) * u));
+    if (used + size > budget) {
+      if (nres > 0) break;
+      size = budget;
+    }
+    const BatchType t = bf.f64_only ? BatchType::Float64 : BatchType(nres % 3);
+    const bool scan   = (st >> 7) & 1;
+    double* in        = g_buf.a + used;
+    if (scan) {
+      bd.add(BatchOp::ScanExclusive, t, in, size, g_buf.b + out, g_buf.c + nres);
+      out += size;
+    } else {
+      bd.add(BatchOp::ReduceSum, t, in, size, nullptr, g_buf.c + nres);
+    }
+    used += size;
+    ++nres;
+  }
+  return used;
+}
+double g_last_features[7] = {0, 0, 0, 0, 0, 0, 0};
 
 using P = Kokkos::RangePolicy<Kokkos::MI355X, Kokkos::IndexType<int64_t>>;
 
@@ -478,6 +536,16 @@ extern "C" {
 const char* kkmx_sweep_last_error(void) { return g_err.c_str(); }
 int kkmx_sweep_family_count(void) { return kFamilies; }
 
+// Features of the last timed batched-dispatch group: pattern, extent (total
+// elements), value bytes, functor bytes (mean op extent), floating, bytes
+// per element, flops per element -- what BatchDispatch::plan() sends the
+// tuning tool for that batch.
+int kkmx_sweep_last_features(double* out) {
+  if (!out) return -1;
+  std::copy(g_last_features, g_last_features + 7, out);
+  return 0;
+}
+
 // pattern, value bytes, floating, functor bytes, bytes/index, flops/index of a family.
 int kkmx_sweep_family_info(int f, int* pattern, int* value_bytes, int* floating, int* functor_bytes, double* bytes,
                            double* flops, int* kind) {
@@ -541,6 +609,39 @@ int kkmx_sweep_time(int f, int64_t n, int reps, int candidate, int heuristic, fl
     KKMX_SAFE_CALL(mx_event_create(&e0));
     KKMX_SAFE_CALL(mx_event_create(&e1));
     const FamilyInfo fi = family_info(f);
+    if (fi.kind == 3) {
+      // One plan per tile shape; candidates past the shape count are not
+      // valid answers and get a 1000 ms sentinel.
+      auto time_shape = [&](int shape) {
+        Kokkos::Experimental::BatchDispatch bd(ex);
+        const int64_t used = batch_ops(f, n, bd);
+        if (n_eff) *n_eff = used;
+        bd.set_shape(shape);
+        bd.plan();
+        const auto x = bd.tuning_features();
+        const double feat[7] = {double(x.pattern), double(x.extent), double(x.value_bytes), double(x.functor_bytes),
+                                x.floating ? 1.0 : 0.0, x.bytes, x.flops};
+        std::copy(feat, feat + 7, g_last_features);
+        float best = 1e30f;
+        bd.submit();
+        for (int r = 0; r < reps; ++r) {
+          KKMX_SAFE_CALL(mx_event_record(e0, ex.stream()));
+          bd.submit();
+          KKMX_SAFE_CALL(mx_event_record(e1, ex.stream()));
+          KKMX_SAFE_CALL(mx_event_sync(e1));
+          float ms = 0;
+          KKMX_SAFE_CALL(mx_event_elapsed_ms(&ms, e0, e1));
+          best = ms < best ? ms : best;
+        }
+        return best;
+      };
+      const int c0 = candidate < 0 ? 0 : candidate, c1 = candidate < 0 ? Kokkos::Impl::kRangeCandidates : candidate + 1;
+      for (int c = c0; c < c1; ++c) ms_out[c - c0] = c < Kokkos::Impl::kBatchShapes ? time_shape(c) : 1000.0f;
+      if (heuristic) ms_out[c1 - c0] = time_shape(Kokkos::Impl::kBatchDefaultShape);
+      KKMX_SAFE_CALL(mx_event_destroy(e0));
+      KKMX_SAFE_CALL(mx_event_destroy(e1));
+      return 0;
+    }
     const int pattern   = fi.pattern;
     auto& ov            = Kokkos::Impl::range_overrides();
     // cand < 0: the untuned default shape.  Range families take the

@@ -241,8 +241,10 @@ struct ToolState {
   // Declared input ids -> feature field (looked up per request, so no string
   // compares on the dispatch path).
   std::unordered_map<size_t, int> input_field;
-  size_t var_id[3]  = {0, 0, 0};  // range launch, team shape, MDRange tile
-  bool have_var[3]   = {false, false, false};
+  // range launch, team shape, MDRange tile, batch tile shape
+  size_t var_id[4]  = {0, 0, 0, 0};
+  bool have_var[4]   = {false, false, false, false};
+  int ncand[4]       = {16, 16, 16, 16};  // candidates each variable declared
   std::unordered_map<Key, int, KeyHash> cache;
   DevBuf* dev_params = nullptr;
   mx_stream_t stream = nullptr;
@@ -262,7 +264,7 @@ ToolState& tool() {
 struct Inputs {
   double extent = 1, value_bytes = 0, functor_bytes = 0, floating = 0, bytes = 0, flops = 0, cus = 256;
   int pattern   = 0;
-  int kind      = 0;  // 0 RangePolicy launch shape, 1 Team shape, 2 MDRange tile
+  int kind      = 0;  // 0 RangePolicy launch shape, 1 Team shape, 2 MDRange tile, 3 batch tile shape
 };
 void features(const Inputs& in, float* f) {
   f[0] = float(std::log2(std::max(in.extent, 1.0)) / 32.0);
@@ -279,7 +281,8 @@ void features(const Inputs& in, float* f) {
   f[11] = float(in.kind) * 0.5f;
 }
 
-int predict_one(const float* feat) {
+// Arg-max over the first `ncand` logits (the candidates the variable has).
+int predict_one(const float* feat, int ncand) {
   auto& s = tool();
   float x[F];
   for (int k = 0; k < F; ++k) x[k] = (feat[k] - s.model.mean[k]) * s.model.inv_std[k];
@@ -298,7 +301,7 @@ int predict_one(const float* feat) {
   if (mx_stream_sync(s.stream) != 0 || mx_memcpy(z, dz.p, sizeof(z)) != 0) return -1;
   ++s.inferences;
   int best = 0;
-  for (int c = 1; c < C; ++c)
+  for (int c = 1; c < std::min(ncand, C); ++c)
     if (z[c] > z[best]) best = c;
   return best;
 }
@@ -497,14 +500,17 @@ void kokkosp_declare_input_type(const char* name, const size_t id, Kokkos_Tools_
   for (const auto& nf : names)
     if (std::strcmp(name, nf.first) == 0) s.input_field[id] = nf.second;
 }
-void kokkosp_declare_output_type(const char* name, const size_t id, Kokkos_Tools_VariableInfo*) {
+void kokkosp_declare_output_type(const char* name, const size_t id, Kokkos_Tools_VariableInfo* info) {
   auto& s = tool();
   std::lock_guard<std::mutex> lk(s.mtx);
-  const char* names[3] = {"kokkos.mi355x.range_launch", "kokkos.mi355x.team_shape", "kokkos.mi355x.mdrange_tile"};
-  for (int v = 0; v < 3; ++v)
+  const char* names[4] = {"kokkos.mi355x.range_launch", "kokkos.mi355x.team_shape", "kokkos.mi355x.mdrange_tile",
+                          "kokkos.mi355x.batch_shape"};
+  for (int v = 0; v < 4; ++v)
     if (std::strcmp(name, names[v]) == 0) {
       s.var_id[v]   = id;
       s.have_var[v] = true;
+      if (info && info->valueQuantity == kokkos_value_set && info->candidates.set.size > 0)
+        s.ncand[v] = int(std::min<size_t>(info->candidates.set.size, C));
     }
 }
 void kokkosp_begin_context(const size_t) {}
@@ -516,7 +522,7 @@ void kokkosp_request_values(const size_t, const size_t n_in, const Kokkos_Tools_
   std::lock_guard<std::mutex> lk(s.mtx);
   for (size_t o = 0; o < n_out; ++o) {
     int var = -1;
-    for (int v = 0; v < 3; ++v)
+    for (int v = 0; v < 4; ++v)
       if (s.have_var[v] && out[o].type_id == s.var_id[v]) var = v;
     if (var < 0) continue;
     ++s.requests;
@@ -556,10 +562,10 @@ void kokkosp_request_values(const size_t, const size_t n_in, const Kokkos_Tools_
       xb.extent = std::exp2(double(bucket) / 4.0);
       float f[F];
       features(xb, f);
-      c            = predict_one(f);
+      c            = predict_one(f, s.ncand[var]);
       s.cache[key] = c;
     }
-    if (c >= 0 && c < C) {
+    if (c >= 0 && c < s.ncand[var]) {
       out[o].value.int_value = c;
       ++s.answered;
     }

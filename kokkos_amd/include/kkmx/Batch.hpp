@@ -74,6 +74,7 @@ constexpr int kBatchShapes = 8;
 constexpr BatchShape kBatchShape[kBatchShapes] = {{256, 1, 16}, {256, 4, 4}, {256, 4, 8},  {512, 4, 4},
                                                   {1024, 1, 16}, {512, 4, 8}, {1024, 4, 4}, {512, 1, 32}};
 constexpr int kBatchDefaultShape = 5;
+static_assert(kBatchShapes == kBatchShapeCandidates, "the batch_shape tuning variable lists every tile shape");
 constexpr int64_t kBatchDetTiles = 2048;
 
 template <class Acc>
@@ -514,6 +515,7 @@ class BatchDispatch {
   // one tile so their result (the identity) is still written.
   void plan() {
     if (m_planned) return;
+    if (!m_shape_set) m_shape = tuned_shape();
     const Impl::BatchShape shape = Impl::kBatchShape[m_shape];
     const int64_t tile_elems     = shape.tile();
     int64_t t                    = 0;
@@ -600,9 +602,31 @@ class BatchDispatch {
     if (m_planned) Impl::throw_runtime_exception("Kokkos::Experimental::BatchDispatch: set_shape() after plan()");
     if (shape < 0 || shape >= Impl::kBatchShapes)
       Impl::throw_runtime_exception("Kokkos::Experimental::BatchDispatch: bad tile shape");
-    m_shape = shape;
+    m_shape     = shape;
+    m_shape_set = true;
   }
   int shape() const { return m_shape; }
+
+  // Features the tile-shape request describes the batch with (Impl::
+  // LaunchFeatures, pattern 3): total elements, mean element bytes, mean op
+  // extent, all-float64 flag and algorithmic bytes per element.
+  Impl::LaunchFeatures tuning_features() const {
+    int64_t total = 0;
+    double bytes = 0, vbytes = 0, f64 = 0;
+    for (const auto& d : m_ops) {
+      const double esz = d.type == int(BatchType::Int32) ? 4.0 : 8.0;
+      const bool outs  = d.op == int(BatchOp::ScanExclusive) || d.op == int(BatchOp::ScanInclusive) ||
+                        d.op == int(BatchOp::Axpb);
+      total += d.n;
+      vbytes += esz * double(d.n);
+      bytes += (esz + (outs ? 8.0 : 0.0)) * double(d.n);
+      if (d.type == int(BatchType::Float64)) f64 += double(d.n);
+    }
+    const double t = total > 0 ? double(total) : 1.0;
+    const size_t m = m_ops.empty() ? 1 : m_ops.size();
+    return Impl::LaunchFeatures{3, total, int64_t(vbytes / t + 0.5), int64_t(double(total) / double(m) + 0.5),
+                                f64 >= t, bytes / t, 1.0};
+  }
 
   size_t size() const { return m_ops.size(); }
   int64_t tiles() const { return m_tiles; }
@@ -620,9 +644,17 @@ class BatchDispatch {
     (void)mx_free(m_control, nullptr);
     m_desc = nullptr;
   }
+  // The tile shape: the learned tuner's answer when a tuning tool is
+  // loaded, else the default.
+  int tuned_shape() const {
+    if (m_ops.empty() || !Tools::Experimental::have_tuning_tool()) return Impl::kBatchDefaultShape;
+    const int c = Impl::request_launch_candidate(Impl::LaunchVar::Batch, tuning_features(), Impl::kBatchDefaultShape);
+    return (c >= 0 && c < Impl::kBatchShapes) ? c : Impl::kBatchDefaultShape;
+  }
   MI355X m_exec;
   std::vector<Impl::BatchDesc> m_ops;
   bool m_planned     = false;
+  bool m_shape_set   = false;
   int m_shape        = Impl::kBatchDefaultShape;
   int64_t m_tiles    = 0;
   int64_t m_elements = 0;

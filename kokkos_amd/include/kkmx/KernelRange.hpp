@@ -712,12 +712,18 @@ struct functor_cost_hint<F, std::void_t<decltype(F::kkmx_bytes_per_index), declt
   static constexpr double bytes = F::kkmx_bytes_per_index, flops = F::kkmx_flops_per_index;
 };
 
-// The three launch-shape variables (Range launch shape, Team shape, MDRange
-// tile) share one set of declared inputs; policy_kind tells them apart.
-enum class LaunchVar : int { Range = 0, Team = 1, MDRange = 2 };
+// The launch-shape variables (Range launch shape, Team shape, MDRange tile,
+// tile shape of the fused batch kernel) share one set of declared inputs;
+// policy_kind tells them apart.
+enum class LaunchVar : int { Range = 0, Team = 1, MDRange = 2, Batch = 3 };
+constexpr int kLaunchVars          = 4;
+constexpr int kBatchShapeCandidates = 8;
+constexpr int launch_var_candidates(LaunchVar v) {
+  return v == LaunchVar::Batch ? kBatchShapeCandidates : kRangeCandidates;
+}
 
 struct LaunchTuningIds {
-  size_t out[3]  = {0, 0, 0};
+  size_t out[kLaunchVars] = {0, 0, 0, 0};
   size_t pattern = 0, extent = 0, value_bytes = 0, functor_bytes = 0, floating = 0, bytes = 0, flops = 0,
          cus = 0, kind = 0;
 };
@@ -727,13 +733,14 @@ inline const LaunchTuningIds& launch_tuning_ids() {
     LaunchTuningIds r;
     static int64_t cand[kRangeCandidates];
     for (int k = 0; k < kRangeCandidates; ++k) cand[k] = k;
-    const char* names[3] = {"kokkos.mi355x.range_launch", "kokkos.mi355x.team_shape", "kokkos.mi355x.mdrange_tile"};
-    for (int v = 0; v < 3; ++v) {
+    const char* names[kLaunchVars] = {"kokkos.mi355x.range_launch", "kokkos.mi355x.team_shape",
+                                      "kokkos.mi355x.mdrange_tile", "kokkos.mi355x.batch_shape"};
+    for (int v = 0; v < kLaunchVars; ++v) {
       VariableInfo o{};
       o.type          = kokkos_value_int64;
       o.category      = kokkos_value_categorical;
       o.valueQuantity = kokkos_value_set;
-      o.candidates    = make_candidate_set(kRangeCandidates, cand);
+      o.candidates    = make_candidate_set(size_t(launch_var_candidates(LaunchVar(v))), cand);
       r.out[v]        = declare_output_type(names[v], o);
     }
     auto in = [](const char* name, ValueType t, StatisticalCategory cat) {
@@ -757,23 +764,33 @@ inline const LaunchTuningIds& launch_tuning_ids() {
   return ids;
 }
 
-// Asks the tuning tool for one of the 16 candidates of `var`; returns the
+// Inputs of one launch-shape request.  pattern: 0 for, 1 reduce, 2 scan,
+// 3 batch; functor_bytes is sizeof(functor) (for a batch: the mean op
+// extent); bytes / flops are per index (0 when unknown).
+struct LaunchFeatures {
+  int pattern;
+  int64_t extent;
+  int64_t value_bytes;
+  int64_t functor_bytes;
+  bool floating;
+  double bytes, flops;
+};
+
+// Asks the tuning tool for one of the candidates of `var`; returns the
 // answer, or `dflt` when no tool is loaded or it leaves the default.
-// pattern: 0 for, 1 reduce, 2 scan.
-template <class F>
-int request_launch_candidate(LaunchVar var, int pattern, int64_t n, size_t value_bytes, bool floating, int dflt) {
+inline int request_launch_candidate(LaunchVar var, const LaunchFeatures& x, int dflt) {
   using namespace Tools::Experimental;
   if (!have_tuning_tool()) return dflt;
   const LaunchTuningIds& id = launch_tuning_ids();
   const size_t ctx          = get_new_context_id();
   begin_context(ctx);
-  VariableValue in[9] = {make_variable_value(id.pattern, int64_t(pattern)),
-                         make_variable_value(id.extent, int64_t(n)),
-                         make_variable_value(id.value_bytes, int64_t(value_bytes)),
-                         make_variable_value(id.functor_bytes, int64_t(sizeof(F))),
-                         make_variable_value(id.floating, int64_t(floating ? 1 : 0)),
-                         make_variable_value(id.bytes, functor_cost_hint<F>::bytes),
-                         make_variable_value(id.flops, functor_cost_hint<F>::flops),
+  VariableValue in[9] = {make_variable_value(id.pattern, int64_t(x.pattern)),
+                         make_variable_value(id.extent, x.extent),
+                         make_variable_value(id.value_bytes, x.value_bytes),
+                         make_variable_value(id.functor_bytes, x.functor_bytes),
+                         make_variable_value(id.floating, int64_t(x.floating ? 1 : 0)),
+                         make_variable_value(id.bytes, x.bytes),
+                         make_variable_value(id.flops, x.flops),
                          make_variable_value(id.cus, int64_t(MI355X::impl_cu_count())),
                          make_variable_value(id.kind, int64_t(var))};
   set_input_values(ctx, 9, in);
@@ -781,7 +798,15 @@ int request_launch_candidate(LaunchVar var, int pattern, int64_t n, size_t value
   request_output_values(ctx, 1, &out);
   end_context(ctx);
   const int64_t c = out.value.int_value;
-  return (c >= 0 && c < kRangeCandidates) ? static_cast<int>(c) : dflt;
+  return (c >= 0 && c < launch_var_candidates(var)) ? static_cast<int>(c) : dflt;
+}
+
+template <class F>
+int request_launch_candidate(LaunchVar var, int pattern, int64_t n, size_t value_bytes, bool floating, int dflt) {
+  if (!Tools::Experimental::have_tuning_tool()) return dflt;
+  const LaunchFeatures x{pattern, n, int64_t(value_bytes), int64_t(sizeof(F)), floating, functor_cost_hint<F>::bytes,
+                         functor_cost_hint<F>::flops};
+  return request_launch_candidate(var, x, dflt);
 }
 
 template <class F>

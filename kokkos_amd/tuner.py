@@ -40,6 +40,7 @@ def _libs():
     S.kkmx_sweep_setup.argtypes = [C.c_int64]
     S.kkmx_sweep_time.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, f32p, C.POINTER(C.c_int64)]
     S.kkmx_sweep_last_error.restype = C.c_char_p
+    S.kkmx_sweep_last_features.argtypes = [C.POINTER(C.c_double)]
     T.kkmx_mlp_param_count.restype = C.c_int
     T.kkmx_mlp_train.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, C.c_int,
                                  f32p, f32p]
@@ -69,12 +70,34 @@ def families():
     return out
 
 
-def raw_features(fam: dict, extent: float, cus: int) -> np.ndarray:
+# Raw request inputs of one group: pattern, extent, value bytes, functor
+# bytes, floating, bytes/index, flops/index, policy kind.
+NRAW = 8
+
+
+def family_raw(fam: dict, extent: float) -> np.ndarray:
+    return np.array([fam["pattern"], extent, fam["value_bytes"], fam["functor_bytes"], fam["floating"],
+                     fam["bytes"], fam["flops"], fam.get("kind", 0)], np.float64)
+
+
+def features_from_raw(raw: np.ndarray, cus: int) -> np.ndarray:
+    """The tool's 12 features (kkmx_tuner_features) of raw request inputs."""
     _, T = _libs()
-    f = np.zeros(NF, np.float32)
-    T.kkmx_tuner_features(fam["pattern"], float(extent), float(fam["value_bytes"]), float(fam["functor_bytes"]),
-                          fam["floating"], fam["bytes"], fam["flops"], cus, fam.get("kind", 0), _fp(f))
-    return f
+    raw = np.atleast_2d(raw)
+    out = np.zeros((len(raw), NF), np.float32)
+    for i, r in enumerate(raw):
+        T.kkmx_tuner_features(int(r[0]), float(r[1]), float(r[2]), float(r[3]), int(r[4]), float(r[5]), float(r[6]),
+                              cus, int(r[7]), _fp(out[i]))
+    return out
+
+
+def raw_features(fam: dict, extent: float, cus: int) -> np.ndarray:
+    return features_from_raw(family_raw(fam, extent), cus)[0]
+
+
+def ncand_of_kind(kind: np.ndarray) -> np.ndarray:
+    """Valid candidates per group: 8 tile shapes for batches, else 16."""
+    return np.where(np.asarray(kind) == 3, 8, NC)
 
 
 def collect(extents_per_family: int = 160, seed: int = 2024, lo: float = 8.0, hi: float = 27.0, reps: int = 2,
@@ -87,12 +110,13 @@ def collect(extents_per_family: int = 160, seed: int = 2024, lo: float = 8.0, hi
     if S.kkmx_sweep_setup(max_n) != 0:
         raise KkmxError(S.kkmx_sweep_last_error().decode())
     rng = np.random.default_rng(seed)
-    fam_ids, extents, times, feats = [], [], [], []
+    fam_ids, extents, times, feats, raws = [], [], [], [], []
     buf = np.zeros(NC + 1, np.float32)
     neff = C.c_int64()
+    lf = (C.c_double * 7)()
     # extent ranges (log2) per policy kind: Range indices, Team leagues,
-    # MDRange points
-    ranges = {0: (lo, hi), 1: (6.0, 18.0), 2: (12.0, hi)}
+    # MDRange points, batch element budgets
+    ranges = {0: (lo, hi), 1: (6.0, 18.0), 2: (12.0, hi), 3: (14.0, hi)}
     try:
         for f in range(len(fams)):
             a, b = ranges[fams[f]["kind"]]
@@ -103,13 +127,20 @@ def collect(extents_per_family: int = 160, seed: int = 2024, lo: float = 8.0, hi
                 fam_ids.append(f)
                 extents.append(int(neff.value))
                 times.append(buf.copy())
-                feats.append(raw_features(fams[f], neff.value, cus))
+                if fams[f]["kind"] == 3:
+                    S.kkmx_sweep_last_features(lf)
+                    raw = np.array(list(lf) + [3.0], np.float64)
+                else:
+                    raw = family_raw(fams[f], neff.value)
+                raws.append(raw)
+                feats.append(features_from_raw(raw, cus)[0])
             if progress:
                 progress(f"family {f} ({len(ns)} extents) done, {len(times)} groups", )
     finally:
         S.kkmx_sweep_teardown()
     return dict(family=np.array(fam_ids, np.int32), extent=np.array(extents, np.int64),
-                times=np.array(times, np.float32), feats=np.array(feats, np.float32), families=fams)
+                times=np.array(times, np.float32), feats=np.array(feats, np.float32), raw=np.array(raws),
+                families=fams)
 
 
 def soft_targets(times: np.ndarray, tau: float = 0.03) -> np.ndarray:
@@ -195,11 +226,29 @@ def tool_stats():
     return dict(requests=a.value, answered=b.value, inferences=c.value)
 
 
+def group_raw(data: dict) -> np.ndarray:
+    """Raw request inputs of every group (rebuilt from the family table for
+    sample files written before they were stored)."""
+    if "raw" in data:
+        return np.asarray(data["raw"], np.float64)
+    return np.array([family_raw(data["families"][f], e) for f, e in zip(data["family"], data["extent"])])
+
+
 def hintless_features(data: dict, cus: int = 256) -> np.ndarray:
     """Features of every group as a functor without the optional
-    kkmx_bytes_per_index / kkmx_flops_per_index hints would present them."""
-    fams = [dict(f, bytes=0.0, flops=0.0) for f in data["families"]]
-    return np.array([raw_features(fams[f], e, cus) for f, e in zip(data["family"], data["extent"])], np.float32)
+    kkmx_bytes_per_index / kkmx_flops_per_index hints would present them
+    (batch groups keep theirs: the runtime computes them)."""
+    raw = group_raw(data).copy()
+    hinted = raw[:, 7] < 3
+    raw[hinted, 5] = 0.0
+    raw[hinted, 6] = 0.0
+    return features_from_raw(raw, cus)
+
+
+def masked_argmax(Z: np.ndarray, ncand: np.ndarray) -> np.ndarray:
+    Z = Z.copy()
+    Z[np.arange(NC)[None, :] >= np.asarray(ncand)[:, None]] = -np.inf
+    return Z.argmax(axis=1)
 
 
 def fit_and_evaluate(data: dict, seed: int = 11, holdout: float = 0.2, hint_dropout: bool = True, cus: int = 256,
@@ -224,8 +273,10 @@ def fit_and_evaluate(data: dict, seed: int = 11, holdout: float = 0.2, hint_drop
         Y_tr = np.concatenate([Y_tr, Y[tr]])
     mean, inv = normaliser(X_tr)
     params, loss = train((X_tr - mean) * inv, Y_tr, **train_kw)
+    kinds = group_raw(data)[:, 7].astype(int)
+    ncand = ncand_of_kind(kinds)
     Z = predict(params, (data["feats"] - mean) * inv)
-    choice = Z.argmax(axis=1)
+    choice = masked_argmax(Z, ncand)
     r_mlp = regret(data["times"], choice)
     r_heur = regret(data["times"], np.full(G, NC))
     res = dict(
@@ -235,16 +286,15 @@ def fit_and_evaluate(data: dict, seed: int = 11, holdout: float = 0.2, hint_drop
         train=dict(mlp=summary(r_mlp[tr]), heuristic=summary(r_heur[tr])),
     )
     if hl is not None:
-        c_hl = predict(params, (hl - mean) * inv).argmax(axis=1)
+        c_hl = masked_argmax(predict(params, (hl - mean) * inv), ncand)
         res["test_without_hints"] = dict(mlp=summary(regret(data["times"], c_hl)[te]))
     # held-out regret per policy kind (RangePolicy launch shape, Team shape,
     # MDRange tile)
-    kinds = np.array([data["families"][f].get("kind", 0) for f in data["family"]])
     is_te = np.zeros(G, bool)
     is_te[te] = True
     res["test_by_kind"] = {
         name: dict(groups=int((is_te & (kinds == k)).sum()), mlp=summary(r_mlp[is_te & (kinds == k)]),
                    heuristic=summary(r_heur[is_te & (kinds == k)]))
-        for k, name in enumerate(("range", "team", "mdrange")) if (is_te & (kinds == k)).any()
+        for k, name in enumerate(("range", "team", "mdrange", "batch")) if (is_te & (kinds == k)).any()
     }
     return dict(params=params, mean=mean, inv_std=inv, loss=loss, report=res, choice=choice)

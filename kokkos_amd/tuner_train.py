@@ -40,13 +40,14 @@ def main(argv=None) -> int:
     t0 = time.time()
     if a.samples:
         z = np.load(a.samples)
-        data = {k: z[k] for k in ("family", "extent", "times", "feats")}
+        data = {k: z[k] for k in ("family", "extent", "times", "feats", "raw") if k in z}
         data["families"] = T.families()
+        data["raw"] = T.group_raw(data)
     else:
         data = T.collect(extents_per_family=a.extents, seed=a.seed, reps=a.reps, cus=cus,
                          progress=lambda m: print(f"[{time.time() - t0:6.1f}s] {m}", flush=True))
     np.savez_compressed(out / "samples.npz", family=data["family"], extent=data["extent"], times=data["times"],
-                        feats=data["feats"])
+                        feats=data["feats"], raw=data["raw"])
     print(f"collected {len(data['times'])} groups = {len(data['times']) * T.NC} samples "
           f"in {time.time() - t0:.1f}s", flush=True)
     t1 = time.time()
@@ -62,7 +63,8 @@ def main(argv=None) -> int:
     meta = dict(features=T.NF, hidden=T.NH, configs=T.NC, report=rep,
                 candidates={"range": "for/reduce: block=128<<(c/4), items=1<<(c%4); scan: block=128<<(c/4), items=8<<(c%4)",
                             "team": "(team_size, vector_length) = kTeamCandidates[c]",
-                            "mdrange": "tile: fastest dim 16<<(c%4), next 1<<(c/4), others 1"})
+                            "mdrange": "tile: fastest dim 16<<(c%4), next 1<<(c/4), others 1",
+                            "batch": "fused-kernel tile shape Impl::kBatchShape[c], c < 8"})
     T.save_model(fit["params"], fit["mean"], fit["inv_std"], meta, out / "mlp_gfx950.bin", out / "mlp_gfx950.json")
     (out / "report.json").write_text(json.dumps(rep, indent=1) + "\n")
     return 0

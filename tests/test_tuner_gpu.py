@@ -87,3 +87,41 @@ def test_tool_answers_requests_and_results_stay_exact(paths, oracle, tmp_path):
         assert after["inferences"] >= 1
     finally:
         K.check(P.kkmx_tools_unload(), P)
+
+
+def test_tool_chooses_batch_tile_shape_and_results_match(paths, oracle):
+    """With the tuner loaded, a batch planned without an explicit tile shape
+    asks the tool (kokkos.mi355x.batch_shape); the shape is a valid one and
+    the results equal those of the default shape bit for bit on integers and
+    within 1e-12 on float64."""
+    from kokkos_amd import batch as B
+    from kokkos_amd import workloads as W
+
+    P = paths
+    wl = W.batch_workload(300, 21, 4, 16)
+    ref = B.BatchRun(wl, 5)
+    ref.run(B.MODE_FUSED, 1)
+    K.fence()
+    ri, rf = (a.copy() for a in ref.results())
+    ref.close()
+    lib = str(K.lib_path("kkmx_tuner"))
+    K.check(P.kkmx_tools_load(lib.encode()), P)
+    try:
+        T.load_into_tool(T.MODEL_PATH)
+        before = T.tool_stats()
+        run = B.BatchRun(wl)
+        after = T.tool_stats()
+        print(f"tuned tile shape {run.shape}, tool {before} -> {after}")
+        assert 0 <= run.shape < 8
+        # the input fills are tuned range dispatches too; the plan adds one
+        assert after["requests"] >= before["requests"] + 1
+        run.run(B.MODE_FUSED_GRAPH, 2)
+        K.fence()
+        gi, gf = run.results()
+        isint = wl["type"] != W.T_F64
+        assert np.array_equal(gi[isint], ri[isint])
+        rel = np.abs(gf[~isint] - rf[~isint]) / np.maximum(np.abs(rf[~isint]), 1e-300)
+        assert rel.size == 0 or rel.max() <= 1e-12
+        run.close()
+    finally:
+        K.check(P.kkmx_tools_unload(), P)
