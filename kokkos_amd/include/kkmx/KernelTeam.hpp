@@ -192,13 +192,18 @@ class TeamMember<MI355X> {
     int nact         = nthr - wstart;
     if (nact > 64) nact = 64;
     T x  = (m_vector_lane == 0) ? v : identity;
-    T in = seg_inclusive(x, l, nact, join);
+    // lanes past the team are inactive: the DPP scan reads the identity there
+    T in = wave::inclusive_scan(x, identity, join);
+    const int nw = (nthr + 63) >> 6;
+    if (nw == 1) {  // one-wavefront team: no LDS, no barrier
+      v = wave::readlane(in, nact - 1);
+      return;
+    }
     T* slots = reinterpret_cast<T*>(m_coll);
     __syncthreads();
     if (l == nact - 1) slots[w] = in;
     __syncthreads();
-    const int nw = (nthr + 63) >> 6;
-    T r          = identity;
+    T r = identity;
     for (int k = 0; k < nw; ++k) join(r, slots[k]);
     __syncthreads();
     v = r;
@@ -215,12 +220,16 @@ class TeamMember<MI355X> {
     int nact         = nthr - wstart;
     if (nact > 64) nact = 64;
     T x  = (m_vector_lane == 0) ? v : identity;
-    T in = seg_inclusive(x, l, nact, join);
+    T in = wave::inclusive_scan(x, identity, join);
     // exclusive within the wavefront: inclusive value of the previous lane of
     // this thread's leader
     const int leader = (m_team_rank * m_vl) - wstart;
     T ex             = wave::shuffle(in, leader > 0 ? leader - 1 : 0);
     if (leader == 0) ex = identity;
+    if (nthr <= 64) {  // one-wavefront team: no LDS, no barrier
+      if (total) *total = wave::readlane(in, nact - 1);
+      return ex;
+    }
     T* slots = reinterpret_cast<T*>(m_coll);
     __syncthreads();
     if (l == nact - 1) slots[w] = in;
@@ -262,13 +271,17 @@ class TeamMember<MI355X> {
     const int l      = wave::lane();
     int nact         = nthr - w * 64;
     if (nact > 64) nact = 64;
-    T in     = seg_inclusive(v, l, nact, join);
+    T in         = wave::inclusive_scan(v, identity, join);
+    const int nw = (nthr + 63) >> 6;
+    if (nw == 1) {  // one-wavefront team: no LDS, no barrier
+      v = wave::readlane(in, nact - 1);
+      return;
+    }
     T* slots = reinterpret_cast<T*>(m_coll);
     __syncthreads();
     if (l == nact - 1) slots[w] = in;
     __syncthreads();
-    const int nw = (nthr + 63) >> 6;
-    T r          = identity;
+    T r = identity;
     for (int k = 0; k < nw; ++k) join(r, slots[k]);
     __syncthreads();
     v = r;
@@ -796,7 +809,9 @@ TeamLaunch team_launch_shape(const Policy& policy, const F& f, MI355XInstance* i
     int by_lds = static_cast<int>((160 * 1024) / lds);
     if (by_lds < per_cu) per_cu = by_lds > 0 ? by_lds : 1;
   }
-  int64_t g = std::min<int64_t>(p.league_size, static_cast<int64_t>(cus) * per_cu * 8);
+  // parallel_for: 8 waves of resident teams; reductions: 2, so the last
+  // workgroup's pass over the per-workgroup partials stays short.
+  int64_t g = std::min<int64_t>(p.league_size, static_cast<int64_t>(cus) * per_cu * (pattern == 1 ? 2 : 8));
   grid      = static_cast<int>(g > 0 ? g : 0);
   p.l1_bytes = policy.team_scratch_size(1) + static_cast<size_t>(ts) * policy.thread_scratch_size(1);
   p.l1_pool  = nullptr;
