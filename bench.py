@@ -184,6 +184,29 @@ def leg_md_team(K, P, timer):
     }
 
 
+def leg_deep_copy(K, P, timer):
+    """deep_copy between device Views: a contiguous 2 GiB copy (against the
+    runtime's own device-to-device memcpy), a 2 GiB fill, and a 512^3 copy
+    from LayoutLeft to LayoutRight."""
+    chk = lambda rc: K.check(rc, P)  # noqa: E731
+    rt = K.load("kkmx_rt")
+    n = 1 << 28
+    A, B = K.DeviceBuffer(n * 8), K.DeviceBuffer(n * 8)
+    chk(P.kkmx_fill_uniform_f64(A.ptr, n, 44, -1.0, 1.0))
+    cp = _timeit(K, timer, lambda: chk(P.kkmx_deep_copy_f64(B.ptr, A.ptr, n, 1, 1, 0, 0)), reps=10)
+    mc = _timeit(K, timer, lambda: K.check(rt.mx_memcpy_async(B.ptr, A.ptr, n * 8, P.kkmx_stream())), reps=10)
+    fl = _timeit(K, timer, lambda: chk(P.kkmx_deep_fill_f64(B.ptr, n, 0.5)), reps=10)
+    m = 512
+    tr = _timeit(K, timer, lambda: chk(P.kkmx_deep_copy_f64(B.ptr, A.ptr, m, m, m, 1, 0)), reps=10)
+    del A, B
+    return {
+        "copy_contiguous_2GiB": {"ms": round(cp[0], 4), "gbps": round(16 * n / cp[0] / 1e6, 1)},
+        "runtime_memcpy_2GiB": {"ms": round(mc[0], 4), "gbps": round(16 * n / mc[0] / 1e6, 1)},
+        "fill_2GiB": {"ms": round(fl[0], 4), "gbps": round(8 * n / fl[0] / 1e6, 1)},
+        "copy_left_to_right_512^3": {"ms": round(tr[0], 4), "gbps": round(16 * m**3 / tr[0] / 1e6, 1)},
+    }
+
+
 def leg_atomics(K, P, timer):
     """GUPS (2^30 int64 table, 2^28 updates), Zipf(1.1) histogram of 2^28
     keys over 65536 bins (LDS-privatised), FP64 scatter of 2^26 into 2^20."""
@@ -314,7 +337,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
-    ap.add_argument("--only", choices=["small_reduce_scan", "mdrange_team", "atomics", "batched"],
+    ap.add_argument("--only", choices=["small_reduce_scan", "mdrange_team", "atomics", "deep_copy", "batched"],
                     help="run one secondary workload alone and print its JSON (for profiling); no headline line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -332,7 +355,7 @@ def main():
     P = K.init(0)
     if args.only:
         legs = {"small_reduce_scan": leg_small, "mdrange_team": leg_md_team, "atomics": leg_atomics,
-                "batched": leg_batched}
+                "deep_copy": leg_deep_copy, "batched": leg_batched}
         print(json.dumps({args.only: legs[args.only](K, P, K.EventTimer())}), flush=True)
         return
     props = K.device_props()
@@ -417,6 +440,7 @@ def main():
         for name, fn in [("small_reduce_scan", lambda: leg_small(K, P, timer)),
                          ("mdrange_team", lambda: leg_md_team(K, P, timer)),
                          ("atomics", lambda: leg_atomics(K, P, timer)),
+                         ("deep_copy", lambda: leg_deep_copy(K, P, timer)),
                          ("batched", lambda: leg_batched(K, P, timer))]:
             t_leg = time.perf_counter()
             extra[name] = fn()

@@ -52,6 +52,16 @@ int kkmx_stream_scale(double* b_dev, const double* c_dev, int64_t n, double scal
 int kkmx_stream_add(const double* a_dev, const double* b_dev, double* c_dev, int64_t n);
 int kkmx_stream_triad(double* a_dev, const double* b_dev, const double* c_dev, int64_t n, double scalar);
 
+/* ---- deep_copy between device Views (ViewFill / ViewCopy,
+ * core/src/Kokkos_CopyViews.hpp:58-420; HIP core/src/HIP/Kokkos_HIP_DeepCopy.cpp:23-51).
+ * kkmx_deep_copy_f64: dst(i,j,k) = src(i,j,k) for rank-3 Views of
+ * n0 x n1 x n2 doubles; layout 0 = LayoutLeft, 1 = LayoutRight (same
+ * layouts: one contiguous copy; different: an index-space copy).
+ * kkmx_deep_fill_f64: deep_copy(View<double*>(dst, n), value). ---- */
+int kkmx_deep_copy_f64(double* dst_dev, const double* src_dev, int64_t n0, int64_t n1, int64_t n2, int dst_layout,
+                       int src_layout);
+int kkmx_deep_fill_f64(double* dst_dev, int64_t n, double value);
+
 /* ---- parallel_reduce into a device result ---- */
 int kkmx_reduce_sum_f64(const double* x_dev, int64_t n, double* out_dev);
 /* op: 0 = Sum, 1 = Min, 2 = Max */

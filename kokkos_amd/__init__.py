@@ -85,6 +85,8 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_fill_uniform_i32", [vp, i64, u64, i32, i32])
         _sig(lib, "kkmx_stream_set", [vp, i64, f64])
         _sig(lib, "kkmx_stream_copy", [vp, vp, i64])
+        _sig(lib, "kkmx_deep_copy_f64", [vp, vp, i64, i64, i64, C.c_int, C.c_int])
+        _sig(lib, "kkmx_deep_fill_f64", [vp, i64, f64])
         _sig(lib, "kkmx_stream_scale", [vp, vp, i64, f64])
         _sig(lib, "kkmx_stream_add", [vp, vp, vp, i64])
         _sig(lib, "kkmx_stream_triad", [vp, vp, vp, i64, f64])

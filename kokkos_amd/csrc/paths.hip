@@ -469,6 +469,28 @@ int kkmx_stream_set(double* a, int64_t n, double scalar) {
     Kokkos::parallel_for("set", StreamPolicy(0, int(n)), KOKKOS_LAMBDA(const int i) { A[i] = scalar; });
   });
 }
+int kkmx_deep_copy_f64(double* dst, const double* src, int64_t n0, int64_t n1, int64_t n2, int dst_layout,
+                       int src_layout) {
+  if (n0 < 0 || n1 < 0 || n2 < 0 || dst_layout < 0 || dst_layout > 1 || src_layout < 0 || src_layout > 1 ||
+      (n0 * n1 * n2 > 0 && (!dst || !src)))
+    return reject("kkmx_deep_copy_f64: bad arguments");
+  return guarded([&] {
+    using L = Kokkos::View<double***, Kokkos::LayoutLeft, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+    using R = Kokkos::View<double***, Kokkos::LayoutRight, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+    using CL = Kokkos::View<const double***, Kokkos::LayoutLeft, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+    using CR = Kokkos::View<const double***, Kokkos::LayoutRight, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+    const size_t a = size_t(n0), b = size_t(n1), c = size_t(n2);
+    Kokkos::MI355X ex;
+    if (dst_layout == 0 && src_layout == 0) Kokkos::deep_copy(ex, L(dst, a, b, c), CL(src, a, b, c));
+    else if (dst_layout == 0) Kokkos::deep_copy(ex, L(dst, a, b, c), CR(src, a, b, c));
+    else if (src_layout == 0) Kokkos::deep_copy(ex, R(dst, a, b, c), CL(src, a, b, c));
+    else Kokkos::deep_copy(ex, R(dst, a, b, c), CR(src, a, b, c));
+  });
+}
+int kkmx_deep_fill_f64(double* dst, int64_t n, double value) {
+  if (n < 0 || (n > 0 && !dst)) return reject("kkmx_deep_fill_f64: bad arguments");
+  return guarded([&] { Kokkos::deep_copy(Kokkos::MI355X(), DevView<double>(dst, n), value); });
+}
 int kkmx_stream_copy(const double* a, double* b, int64_t n) {
   if (n < 0 || n > 0x7fffffff) return reject("kkmx_stream_copy: n must fit IndexType<int>");
   return guarded([&] {

@@ -1498,6 +1498,103 @@ KKMX_TEST(atomic_views) {
   }
 }
 
+// deep_copy: ViewFill / ViewCopy (core/src/Kokkos_CopyViews.hpp:58-420)
+KKMX_TEST(deep_copy_fill_contiguous_and_strided) {
+  Kokkos::View<double*, Exec> a("a", 1000003);
+  Kokkos::deep_copy(a, 2.5);
+  auto ha = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), a);
+  for (size_t i = 0; i < ha.extent(0); ++i) CHECK_EQ(ha(i), 2.5);
+  // a strided 30 x 20 window of a larger allocation: only its elements change
+  Kokkos::View<double*, Exec> base("base", 64 * 40);
+  Kokkos::deep_copy(base, -1.0);
+  Kokkos::View<double**, Kokkos::LayoutStride, Exec> sv(base.data(), Kokkos::LayoutStride(30, 2, 20, 128));
+  Kokkos::deep_copy(sv, 7.0);
+  auto hb = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), base);
+  for (int k = 0; k < 64 * 40; ++k) {
+    const bool in = (k % 128) % 2 == 0 && (k % 128) / 2 < 30 && k / 128 < 20;
+    CHECK_EQ(hb(k), in ? 7.0 : -1.0);
+  }
+  Kokkos::MI355X ex;
+  Kokkos::deep_copy(ex, sv, 3.0);
+  ex.fence();
+  Kokkos::deep_copy(hb, base);
+  for (int k = 0; k < 64 * 40; ++k) {
+    const bool in = (k % 128) % 2 == 0 && (k % 128) / 2 < 30 && k / 128 < 20;
+    CHECK_EQ(hb(k), in ? 3.0 : -1.0);
+  }
+}
+struct Set2 {
+  Kokkos::View<int64_t**, Kokkos::LayoutLeft, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, const int64_t j) const { v(i, j) = i * 1000 + j; }
+};
+struct Set3 {
+  Kokkos::View<int64_t***, Kokkos::LayoutRight, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, const int64_t j, const int64_t k) const {
+    v(i, j, k) = (i * 100 + j) * 100 + k;
+  }
+};
+struct Set5 {
+  Kokkos::View<int32_t*****, Kokkos::LayoutLeft, Exec> v;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t a, const int64_t b, const int64_t c, const int64_t d,
+                                         const int64_t e) const {
+    v(a, b, c, d, e) = int32_t((((a * 10 + b) * 10 + c) * 10 + d) * 10 + e);
+  }
+};
+KKMX_TEST(deep_copy_between_layouts) {
+  using MD2 = Kokkos::MDRangePolicy<Exec, Kokkos::Rank<2>, Kokkos::IndexType<int64_t>>;
+  using MD3 = Kokkos::MDRangePolicy<Exec, Kokkos::Rank<3>, Kokkos::IndexType<int64_t>>;
+  using MD5 = Kokkos::MDRangePolicy<Exec, Kokkos::Rank<5>, Kokkos::IndexType<int64_t>>;
+  Kokkos::View<int64_t**, Kokkos::LayoutLeft, Exec> L("L", 37, 53);
+  Kokkos::parallel_for(MD2({0, 0}, {37, 53}), Set2{L});
+  Kokkos::View<int64_t**, Kokkos::LayoutRight, Exec> R("R", 37, 53);
+  Kokkos::deep_copy(R, L);
+  auto hr = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), R);
+  for (int i = 0; i < 37; ++i)
+    for (int j = 0; j < 53; ++j) CHECK_EQ(hr(i, j), int64_t(i) * 1000 + j);
+  Kokkos::View<int64_t***, Kokkos::LayoutRight, Exec> R3("R3", 13, 7, 29);
+  Kokkos::parallel_for(MD3({0, 0, 0}, {13, 7, 29}), Set3{R3});
+  Kokkos::View<int64_t***, Kokkos::LayoutLeft, Exec> L3("L3", 13, 7, 29);
+  Kokkos::MI355X ex;
+  Kokkos::deep_copy(ex, L3, R3);  // ordered on ex, no fence
+  ex.fence();
+  auto hl3 = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), L3);
+  for (int i = 0; i < 13; ++i)
+    for (int j = 0; j < 7; ++j)
+      for (int k = 0; k < 29; ++k) CHECK_EQ(hl3(i, j, k), (int64_t(i) * 100 + j) * 100 + k);
+  Kokkos::View<int32_t*****, Kokkos::LayoutLeft, Exec> L5("L5", 3, 4, 5, 6, 7);
+  Kokkos::parallel_for(MD5({0, 0, 0, 0, 0}, {3, 4, 5, 6, 7}), Set5{L5});
+  Kokkos::View<int32_t*****, Kokkos::LayoutRight, Exec> R5("R5", 3, 4, 5, 6, 7);
+  Kokkos::deep_copy(R5, L5);
+  auto h5 = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), R5);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 4; ++b)
+      for (int c = 0; c < 5; ++c)
+        for (int d = 0; d < 6; ++d)
+          for (int e = 0; e < 7; ++e) CHECK_EQ(h5(a, b, c, d, e), int32_t((((a * 10 + b) * 10 + c) * 10 + d) * 10 + e));
+}
+KKMX_TEST(deep_copy_contiguous_and_strided_sources) {
+  // contiguous, 16-byte path and element path (odd byte count)
+  Kokkos::View<double*, Exec> x("x", 4099), y("y", 4099);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, 4099), KOKKOS_LAMBDA(const int i) { x(i) = 0.5 * i; });
+  Kokkos::deep_copy(y, x);
+  auto hy = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), y);
+  for (int i = 0; i < 4099; ++i) CHECK_EQ(hy(i), 0.5 * i);
+  Kokkos::View<char*, Exec> c1("c1", 1001), c2("c2", 1001);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, 1001), KOKKOS_LAMBDA(const int i) { c1(i) = char(i * 7); });
+  Kokkos::deep_copy(c2, c1);
+  auto hc = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), c2);
+  for (int i = 0; i < 1001; ++i) CHECK_EQ(int(hc(i)), int(char(i * 7)));
+  // strided source window -> contiguous LayoutLeft destination
+  Kokkos::View<double*, Exec> base("base", 64 * 40);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, 64 * 40), KOKKOS_LAMBDA(const int k) { base(k) = k; });
+  Kokkos::View<double**, Kokkos::LayoutStride, Exec> sv(base.data(), Kokkos::LayoutStride(30, 2, 20, 128));
+  Kokkos::View<double**, Kokkos::LayoutLeft, Exec> d("d", 30, 20);
+  Kokkos::deep_copy(d, sv);
+  auto hd = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), d);
+  for (int i = 0; i < 30; ++i)
+    for (int j = 0; j < 20; ++j) CHECK_EQ(hd(i, j), double(i * 2 + j * 128));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- C-ABI

@@ -22,6 +22,36 @@ struct FillFunctor {
   KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { ptr[i] = val; }
 };
 
+// Contiguous copy of elements (8-byte words when sizes and alignment allow).
+template <class T>
+struct CopyElems {
+  const T* src;
+  T* dst;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { dst[i] = src[i]; }
+};
+
+// Index-space fill and copy, for Views whose span is not their index space
+// (LayoutStride) or whose layouts differ: MDRange over the extents (rank
+// 2-6) or a Range (rank 1); the functor receives the View indices directly.
+template <class ViewT>
+struct IndexFill {
+  ViewT dst;
+  typename ViewT::non_const_value_type val;
+  template <class... I>
+  KOKKOS_INLINE_FUNCTION void operator()(const I... i) const {
+    dst(i...) = val;
+  }
+};
+template <class DV, class SV>
+struct IndexCopy {
+  DV dst;
+  SV src;
+  template <class... I>
+  KOKKOS_INLINE_FUNCTION void operator()(const I... i) const {
+    dst(i...) = src(i...);
+  }
+};
+
 template <class DV, class SV>
 struct ElementCopy {
   DV dst;
@@ -39,8 +69,150 @@ struct ElementCopy {
     else if constexpr (DV::rank == 2) dst(idx[0], idx[1]) = src(idx[0], idx[1]);
     else if constexpr (DV::rank == 3) dst(idx[0], idx[1], idx[2]) = src(idx[0], idx[1], idx[2]);
     else if constexpr (DV::rank == 4) dst(idx[0], idx[1], idx[2], idx[3]) = src(idx[0], idx[1], idx[2], idx[3]);
+    else if constexpr (DV::rank == 5)
+      dst(idx[0], idx[1], idx[2], idx[3], idx[4]) = src(idx[0], idx[1], idx[2], idx[3], idx[4]);
+    else if constexpr (DV::rank == 6)
+      dst(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5]) = src(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5]);
+    else if constexpr (DV::rank == 7)
+      dst(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5], idx[6]) =
+          src(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5], idx[6]);
+    else
+      dst(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5], idx[6], idx[7]) =
+          src(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5], idx[6], idx[7]);
   }
 };
+
+// ViewCopy between LayoutLeft and LayoutRight (both contiguous, rank >= 2)
+// as an LDS-tiled transpose on gfx950: each 256-lane workgroup moves a
+// 64 x 64 tile of (first index, last index) for one multi-index of the
+// middle dimensions.  It reads along the source's contiguous dimension and
+// writes along the destination's, both coalesced; the tile sits in LDS with
+// a 65-element row pitch so neither pass has bank conflicts.
+//   LayoutLeft  offset of (i, mid, k) = i + e0 * (midL + M * k)
+//   LayoutRight offset of (i, mid, k) = (i * M + midR) * e_last + k
+// midL / midR: the middle indices flattened left-first / right-first, M
+// their product.
+struct TransposeShape {
+  int64_t e0, elast, M;
+  int64_t mid[4];  // extents of the middle dimensions (1 when absent)
+  int nmid;
+  int64_t tiles0, tileslast;
+};
+template <class W, bool SrcLeft>
+__global__ __launch_bounds__(256) void k_layout_transpose(W* __restrict__ dst, const W* __restrict__ src,
+                                                          const TransposeShape sh) {
+  __shared__ W tile[64][65];
+  const int64_t b  = blockIdx.x;
+  const int64_t t0 = b % sh.tiles0;
+  const int64_t tk = (b / sh.tiles0) % sh.tileslast;
+  int64_t g        = b / (sh.tiles0 * sh.tileslast);  // middle multi-index, flattened left-first
+  int64_t midL = g, midR = 0;
+  {
+    int64_t idx[4] = {0, 0, 0, 0};
+    for (int d = 0; d < sh.nmid; ++d) {
+      idx[d] = g % sh.mid[d];
+      g /= sh.mid[d];
+    }
+    for (int d = 0; d < sh.nmid; ++d) midR = midR * sh.mid[d] + idx[d];
+  }
+  const int lane = threadIdx.x & 63, row = threadIdx.x >> 6;
+  const int64_t i0 = t0 * 64, k0 = tk * 64;
+  // load along the source's contiguous dimension
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int q = row + 4 * r;
+    if (SrcLeft) {
+      const int64_t i = i0 + lane, k = k0 + q;
+      if (i < sh.e0 && k < sh.elast) tile[q][lane] = src[i + sh.e0 * (midL + sh.M * k)];
+    } else {
+      const int64_t k = k0 + lane, i = i0 + q;
+      if (i < sh.e0 && k < sh.elast) tile[q][lane] = src[(i * sh.M + midR) * sh.elast + k];
+    }
+  }
+  __syncthreads();
+  // store along the destination's contiguous dimension
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int q = row + 4 * r;
+    if (SrcLeft) {  // destination LayoutRight: k fastest; tile[k_local][i_local]
+      const int64_t k = k0 + lane, i = i0 + q;
+      if (i < sh.e0 && k < sh.elast) dst[(i * sh.M + midR) * sh.elast + k] = tile[lane][q];
+    } else {  // destination LayoutLeft: i fastest; tile[i_local][k_local]
+      const int64_t i = i0 + lane, k = k0 + q;
+      if (i < sh.e0 && k < sh.elast) dst[i + sh.e0 * (midL + sh.M * k)] = tile[lane][q];
+    }
+  }
+}
+template <class T>
+using transpose_word_t =
+    std::conditional_t<sizeof(T) == 8, uint64_t,
+                       std::conditional_t<sizeof(T) == 4, uint32_t, std::conditional_t<sizeof(T) == 2, uint16_t, uint8_t>>>;
+
+// Launches the transpose when it applies (Left <-> Right, contiguous, rank
+// 2-6, trivially copyable elements of 1-8 bytes, first and last extents of
+// at least 16); returns false otherwise.
+template <class DV, class SV>
+bool layout_transpose(const MI355X& exec, const DV& dst, const SV& src) {
+  using T            = typename DV::non_const_value_type;
+  using DL           = typename DV::array_layout;
+  using SL           = typename SV::array_layout;
+  constexpr bool lr  = std::is_same<DL, LayoutRight>::value && std::is_same<SL, LayoutLeft>::value;
+  constexpr bool rl  = std::is_same<DL, LayoutLeft>::value && std::is_same<SL, LayoutRight>::value;
+  constexpr unsigned R = DV::rank;
+  if constexpr (!(lr || rl) || R < 2 || R > 6 || !std::is_trivially_copyable<T>::value ||
+                !(sizeof(T) == 1 || sizeof(T) == 2 || sizeof(T) == 4 || sizeof(T) == 8)) {
+    return false;
+  } else {
+    if (!dst.span_is_contiguous() || !src.span_is_contiguous()) return false;
+    TransposeShape sh{};
+    sh.e0    = int64_t(dst.extent(0));
+    sh.elast = int64_t(dst.extent(R - 1));
+    if (sh.e0 < 16 || sh.elast < 16) return false;
+    sh.M    = 1;
+    sh.nmid = int(R) - 2;
+    for (int d = 0; d < 4; ++d) sh.mid[d] = 1;
+    for (int d = 0; d < sh.nmid; ++d) {
+      sh.mid[d] = int64_t(dst.extent(d + 1));
+      sh.M *= sh.mid[d];
+    }
+    if (sh.M == 0) return true;
+    sh.tiles0    = (sh.e0 + 63) / 64;
+    sh.tileslast = (sh.elast + 63) / 64;
+    const int64_t blocks = sh.tiles0 * sh.tileslast * sh.M;
+    if (blocks > 0x7fffffff) return false;
+    using W  = transpose_word_t<T>;
+    W* d     = reinterpret_cast<W*>(dst.data());
+    const W* s_ = reinterpret_cast<const W*>(src.data());
+    const void* k = lr ? reinterpret_cast<const void*>(&k_layout_transpose<W, true>)
+                       : reinterpret_cast<const void*>(&k_layout_transpose<W, false>);
+    mx_launch_kernel(k, static_cast<unsigned>(blocks), 256, 0, exec.stream(), d, s_, sh);
+    return true;
+  }
+}
+
+// Runs f over the index space of v on exec: a Range for rank 1, an MDRange
+// (iterating in v's layout order, so consecutive lanes touch consecutive
+// addresses of v) for ranks 2-6, a flat Range with index decomposition
+// beyond.
+template <class V, class F, class FlatF>
+void for_each_index(const MI355X& exec, const char* label, const V& v, const F& f, const FlatF& flat) {
+  constexpr unsigned R = V::rank;
+  if constexpr (R == 1) {
+    parallel_for(label, RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, int64_t(v.extent(0))),
+                 KOKKOS_LAMBDA(const int64_t i) { f(i); });
+  } else if constexpr (R >= 2 && R <= 6) {
+    constexpr Iterate it = std::is_same<typename V::array_layout, LayoutRight>::value ? Iterate::Right : Iterate::Left;
+    using MP             = MDRangePolicy<MI355X, Rank<R, it, it>, IndexType<int64_t>>;
+    typename MP::point_type lo{}, hi{};
+    for (unsigned r = 0; r < R; ++r) {
+      lo[r] = 0;
+      hi[r] = static_cast<int64_t>(v.extent(r));
+    }
+    parallel_for(label, MP(exec, lo, hi), f);
+  } else {
+    parallel_for(label, RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, int64_t(v.size())), flat);
+  }
+}
 
 template <class Mem>
 constexpr bool host_accessible() {
@@ -62,6 +234,90 @@ void check_shapes(const DV& dst, const SV& src) {
 }
 }  // namespace Impl
 
+namespace Impl {
+// ViewFill (reference core/src/Kokkos_CopyViews.hpp:58-260): one pass over
+// the span when it is the index space, else over the index space.
+template <class V>
+void host_fill(const V& dst, const typename V::const_value_type& value) {
+  if (dst.span_is_contiguous()) {
+    auto* p = dst.data();
+    for (size_t i = 0; i < dst.span(); ++i) p[i] = value;
+  } else {
+    using HV = View<typename V::data_type, typename V::array_layout, HostSpace, MemoryTraits<Unmanaged>>;
+    HV h(dst.data(), dst.layout());
+    IndexFill<HV> f{h, value};
+    for (size_t i = 0; i < dst.size(); ++i) {
+      size_t idx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      size_t r      = i;
+      for (int d = int(V::rank) - 1; d >= 0; --d) {
+        idx[d] = r % dst.extent(d);
+        r /= dst.extent(d);
+      }
+      if constexpr (V::rank == 1) f(idx[0]);
+      else if constexpr (V::rank == 2) f(idx[0], idx[1]);
+      else if constexpr (V::rank == 3) f(idx[0], idx[1], idx[2]);
+      else if constexpr (V::rank == 4) f(idx[0], idx[1], idx[2], idx[3]);
+      else if constexpr (V::rank == 5) f(idx[0], idx[1], idx[2], idx[3], idx[4]);
+      else if constexpr (V::rank == 6) f(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5]);
+      else if constexpr (V::rank == 7) f(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5], idx[6]);
+      else f(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5], idx[6], idx[7]);
+    }
+  }
+}
+template <class V>
+void device_fill(const MI355X& exec, const V& dst, const typename V::const_value_type& value) {
+  if (dst.span_is_contiguous()) {
+    FillFunctor<V> f{dst.data(), value};
+    parallel_for("Kokkos::ViewFill", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, int64_t(dst.span())), f);
+    return;
+  }
+  using UV = View<typename V::data_type, typename V::array_layout, MI355XSpace, MemoryTraits<Unmanaged>>;
+  UV u(dst.data(), dst.layout());
+  IndexFill<UV> f{u, value};
+  for_each_index(exec, "Kokkos::ViewFill", u, f, [=] KOKKOS_FUNCTION(const int64_t i) {
+    size_t idx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t r     = i;
+    for (int d = int(UV::rank) - 1; d >= 0; --d) {
+      idx[d] = static_cast<size_t>(r % static_cast<int64_t>(u.extent(d)));
+      r /= static_cast<int64_t>(u.extent(d));
+    }
+    if constexpr (UV::rank == 7) f(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5], idx[6]);
+    else if constexpr (UV::rank == 8) f(idx[0], idx[1], idx[2], idx[3], idx[4], idx[5], idx[6], idx[7]);
+  });
+}
+// ViewCopy between device-accessible Views (reference
+// core/src/Kokkos_CopyViews.hpp:262-420): same layout and both spans
+// contiguous -> one contiguous copy (8-byte words when sizes and alignment
+// allow); LayoutLeft <-> LayoutRight -> the LDS-tiled transpose; otherwise
+// an index-space copy in dst's layout order.
+template <class DV, class SV>
+void device_copy(const MI355X& exec, const DV& dst, const SV& src) {
+  using T                = typename DV::non_const_value_type;
+  const bool same_layout = std::is_same<typename DV::array_layout, typename SV::array_layout>::value || DV::rank <= 1;
+  if (same_layout && dst.span_is_contiguous() && src.span_is_contiguous()) {
+    const size_t bytes = dst.span() * sizeof(T);
+    if (bytes % 8 == 0 && reinterpret_cast<uintptr_t>(dst.data()) % 8 == 0 &&
+        reinterpret_cast<uintptr_t>(src.data()) % 8 == 0) {
+      parallel_for("Kokkos::ViewCopy", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, int64_t(bytes / 8)),
+                   CopyElems<uint64_t>{reinterpret_cast<const uint64_t*>(src.data()),
+                                       reinterpret_cast<uint64_t*>(dst.data())});
+    } else {
+      parallel_for("Kokkos::ViewCopy", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, int64_t(dst.span())),
+                   CopyElems<T>{src.data(), dst.data()});
+    }
+    return;
+  }
+  if (layout_transpose(exec, dst, src)) return;
+  using UD = View<typename DV::data_type, typename DV::array_layout, MI355XSpace, MemoryTraits<Unmanaged>>;
+  using US = View<typename SV::const_data_type, typename SV::array_layout, MI355XSpace, MemoryTraits<Unmanaged>>;
+  UD d(dst.data(), dst.layout());
+  US s(src.data(), src.layout());
+  IndexCopy<UD, US> f{d, s};
+  ElementCopy<UD, US> flat{d, s};
+  for_each_index(exec, "Kokkos::ViewCopy", d, f, flat);
+}
+}  // namespace Impl
+
 // View <- scalar
 template <class D, class... P>
 inline void deep_copy(const View<D, P...>& dst, const typename View<D, P...>::const_value_type& value) {
@@ -71,11 +327,9 @@ inline void deep_copy(const View<D, P...>& dst, const typename View<D, P...>::co
   if (dst.span() == 0) return;
   Kokkos::fence("Kokkos::deep_copy: scalar copy, pre view equality check");
   if constexpr (Impl::host_accessible<Mem>()) {
-    auto* p = dst.data();
-    for (size_t i = 0; i < dst.span(); ++i) p[i] = value;
+    Impl::host_fill(dst, value);
   } else {
-    Impl::FillFunctor<V> f{dst.data(), value};
-    parallel_for("Kokkos::ViewFill", RangePolicy<MI355X, IndexType<int64_t>>(0, int64_t(dst.span())), f);
+    Impl::device_fill(MI355X(), dst, value);
   }
   Kokkos::fence("Kokkos::deep_copy: scalar copy, post copy fence");
 }
@@ -88,11 +342,13 @@ inline void deep_copy(const E& exec, const View<D, P...>& dst,
   if (dst.span() == 0) return;
   if constexpr (Impl::host_accessible<Mem>()) {
     exec.fence();
-    auto* p = dst.data();
-    for (size_t i = 0; i < dst.span(); ++i) p[i] = value;
+    Impl::host_fill(dst, value);
+  } else if constexpr (std::is_same<E, MI355X>::value) {
+    Impl::device_fill(exec, dst, value);
   } else {
-    Impl::FillFunctor<V> f{dst.data(), value};
-    parallel_for("Kokkos::ViewFill", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, int64_t(dst.span())), f);
+    exec.fence();
+    Impl::device_fill(MI355X(), dst, value);
+    Kokkos::fence("Kokkos::deep_copy: fill on another space's instance");
   }
 }
 
@@ -133,18 +389,16 @@ inline void deep_copy(const View<D1, P1...>& dst, const View<D2, P2...>& src) {
   }
   Kokkos::fence("Kokkos::deep_copy: copy between views, pre view equality check");
   const bool same_layout = std::is_same<typename DV::array_layout, typename SV::array_layout>::value || DV::rank <= 1;
-  if (same_layout && dst.span_is_contiguous() && src.span_is_contiguous()) {
-    uint64_t kid = 0;
-    (void)kid;
+  constexpr bool both_device = Impl::device_accessible<typename DV::memory_space>() &&
+                               Impl::device_accessible<typename SV::memory_space>();
+  if constexpr (both_device) {
+    Impl::device_copy(MI355X(), dst, src);
+  } else if (same_layout && dst.span_is_contiguous() && src.span_is_contiguous()) {
     KKMX_SAFE_CALL(mx_memcpy(dst.data(), src.data(), dst.span() * sizeof(typename DV::value_type)));
   } else if constexpr (Impl::host_accessible<typename DV::memory_space>() &&
                        Impl::host_accessible<typename SV::memory_space>()) {
     Impl::ElementCopy<DV, SV> f{dst, src};
     for (int64_t i = 0; i < int64_t(dst.size()); ++i) f(i);
-  } else if constexpr (Impl::device_accessible<typename DV::memory_space>() &&
-                       Impl::device_accessible<typename SV::memory_space>()) {
-    Impl::ElementCopy<DV, SV> f{dst, src};
-    parallel_for("Kokkos::ViewCopy", RangePolicy<MI355X, IndexType<int64_t>>(0, int64_t(dst.size())), f);
   } else {
     // Different layouts across spaces: stage through a host copy of src in
     // its own layout, then copy element-wise on the host / device.
@@ -170,6 +424,12 @@ inline void deep_copy(const E& exec, const View<D1, P1...>& dst, const View<D2, 
   using SV = View<D2, P2...>;
   Impl::check_shapes(dst, src);
   const bool same_layout = std::is_same<typename DV::array_layout, typename SV::array_layout>::value || DV::rank <= 1;
+  if constexpr (std::is_same<E, MI355X>::value && Impl::device_accessible<typename DV::memory_space>() &&
+                Impl::device_accessible<typename SV::memory_space>()) {
+    // ordered on exec's stream, no fence
+    if (dst.data() != src.data() || dst.span() != src.span()) Impl::device_copy(exec, dst, src);
+    return;
+  }
   if (!(same_layout && dst.span_is_contiguous() && src.span_is_contiguous()))
     return deep_copy(dst, src);
   if constexpr (std::is_same<E, MI355X>::value) {

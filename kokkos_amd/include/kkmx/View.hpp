@@ -414,7 +414,11 @@ class View : public ViewTraits<DataType, Properties...> {
                 size_t n2 = KOKKOS_IMPL_NOARG, size_t n3 = KOKKOS_IMPL_NOARG, size_t n4 = KOKKOS_IMPL_NOARG,
                 size_t n5 = KOKKOS_IMPL_NOARG, size_t n6 = KOKKOS_IMPL_NOARG, size_t n7 = KOKKOS_IMPL_NOARG)
       : View(view_alloc(label), n0, n1, n2, n3, n4, n5, n6, n7) {}
-  explicit View(const char* label, size_t n0 = KOKKOS_IMPL_NOARG, size_t n1 = KOKKOS_IMPL_NOARG,
+  // String-literal labels (char arrays, as the reference's is_view_label,
+  // core/src/View/Kokkos_ViewCtor.hpp:59-68): a char pointer stays free for
+  // the unmanaged constructor of char Views.
+  template <size_t N>
+  explicit View(const char (&label)[N], size_t n0 = KOKKOS_IMPL_NOARG, size_t n1 = KOKKOS_IMPL_NOARG,
                 size_t n2 = KOKKOS_IMPL_NOARG, size_t n3 = KOKKOS_IMPL_NOARG, size_t n4 = KOKKOS_IMPL_NOARG,
                 size_t n5 = KOKKOS_IMPL_NOARG, size_t n6 = KOKKOS_IMPL_NOARG, size_t n7 = KOKKOS_IMPL_NOARG)
       : View(view_alloc(std::string(label)), n0, n1, n2, n3, n4, n5, n6, n7) {}
