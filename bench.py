@@ -207,6 +207,39 @@ def leg_deep_copy(K, P, timer):
     }
 
 
+def leg_std_algorithms(K, P, timer):
+    """Kokkos::Experimental std algorithms at 2^28 elements: inclusive_scan
+    int32, exclusive_scan int64 with an initial value, a running-maximum
+    inclusive_scan (user operator, no neutral element), reduce int64 and the
+    float64 dot product transform_reduce."""
+    import ctypes as C
+
+    chk = lambda rc: K.check(rc, P)  # noqa: E731
+    n = 1 << 28
+    A, B = K.DeviceBuffer(n * 8), K.DeviceBuffer(n * 8)
+    O = K.DeviceBuffer(n * 8)
+    out_i, out_f = C.c_int64(), C.c_double()
+    chk(P.kkmx_fill_uniform_i32(A.ptr, n, 61, 0, 7))
+    isc = _timeit(K, timer, lambda: chk(P.kkmx_std_inclusive_scan_i32(A.ptr, O.ptr, n)), reps=10)
+    msc = _timeit(K, timer, lambda: chk(P.kkmx_std_inclusive_scan_max_i32(A.ptr, O.ptr, n)), reps=10)
+    chk(P.kkmx_fill_uniform_i64(A.ptr, n, 62, -(1 << 40), 1 << 40))
+    esc = _timeit(K, timer, lambda: chk(P.kkmx_std_exclusive_scan_i64(A.ptr, O.ptr, n, 42)), reps=10)
+    red = _timeit(K, timer, lambda: chk(P.kkmx_std_reduce_i64(A.ptr, n, 0, C.byref(out_i))), reps=10)
+    chk(P.kkmx_fill_uniform_f64(A.ptr, n, 63, 0.0, 1.0))
+    chk(P.kkmx_fill_uniform_f64(B.ptr, n, 64, 0.0, 1.0))
+    dot = _timeit(K, timer, lambda: chk(P.kkmx_std_transform_reduce_dot_f64(A.ptr, B.ptr, n, 0.0, C.byref(out_f))),
+                  reps=10)
+    del A, B, O
+    g = lambda bpe, t: round(bpe * n / t[0] / 1e6, 1)  # noqa: E731
+    return {
+        "inclusive_scan_i32": {"ms": round(isc[0], 4), "gbps": g(8, isc)},
+        "inclusive_scan_max_i32": {"ms": round(msc[0], 4), "gbps": g(8, msc)},
+        "exclusive_scan_i64_init": {"ms": round(esc[0], 4), "gbps": g(16, esc)},
+        "reduce_i64": {"ms": round(red[0], 4), "gbps": g(8, red)},
+        "transform_reduce_dot_f64": {"ms": round(dot[0], 4), "gbps": g(16, dot)},
+    }
+
+
 def leg_atomics(K, P, timer):
     """GUPS (2^30 int64 table, 2^28 updates), Zipf(1.1) histogram of 2^28
     keys over 65536 bins (LDS-privatised), FP64 scatter of 2^26 into 2^20."""
@@ -337,7 +370,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
-    ap.add_argument("--only", choices=["small_reduce_scan", "mdrange_team", "atomics", "deep_copy", "batched"],
+    ap.add_argument("--only", choices=["small_reduce_scan", "mdrange_team", "atomics", "deep_copy", "std_algorithms",
+                                       "batched"],
                     help="run one secondary workload alone and print its JSON (for profiling); no headline line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -355,7 +389,7 @@ def main():
     P = K.init(0)
     if args.only:
         legs = {"small_reduce_scan": leg_small, "mdrange_team": leg_md_team, "atomics": leg_atomics,
-                "deep_copy": leg_deep_copy, "batched": leg_batched}
+                "deep_copy": leg_deep_copy, "std_algorithms": leg_std_algorithms, "batched": leg_batched}
         print(json.dumps({args.only: legs[args.only](K, P, K.EventTimer())}), flush=True)
         return
     props = K.device_props()
@@ -441,6 +475,7 @@ def main():
                          ("mdrange_team", lambda: leg_md_team(K, P, timer)),
                          ("atomics", lambda: leg_atomics(K, P, timer)),
                          ("deep_copy", lambda: leg_deep_copy(K, P, timer)),
+                         ("std_algorithms", lambda: leg_std_algorithms(K, P, timer)),
                          ("batched", lambda: leg_batched(K, P, timer))]:
             t_leg = time.perf_counter()
             extra[name] = fn()

@@ -62,6 +62,23 @@ int kkmx_deep_copy_f64(double* dst_dev, const double* src_dev, int64_t n0, int64
                        int src_layout);
 int kkmx_deep_fill_f64(double* dst_dev, int64_t n, double value);
 
+/* ---- Kokkos::Experimental std-algorithm scans and reductions over device
+ * Views (algorithms/src/std_algorithms/Kokkos_InclusiveScan.hpp,
+ * Kokkos_ExclusiveScan.hpp, Kokkos_TransformExclusiveScan.hpp,
+ * Kokkos_Reduce.hpp, Kokkos_TransformReduce.hpp).  in and out may be the
+ * same array.  Reductions return their value through a host pointer.
+ * kkmx_std_inclusive_scan_max_i32: running maximum (a user operator with no
+ *   neutral element).
+ * kkmx_std_transform_exclusive_scan_sq_f64: out[i] = init + sum_{j<i} in[j]^2. */
+int kkmx_std_inclusive_scan_i32(const int32_t* in_dev, int32_t* out_dev, int64_t n);
+int kkmx_std_exclusive_scan_i64(const int64_t* in_dev, int64_t* out_dev, int64_t n, int64_t init);
+int kkmx_std_inclusive_scan_max_i32(const int32_t* in_dev, int32_t* out_dev, int64_t n);
+int kkmx_std_transform_exclusive_scan_sq_f64(const double* in_dev, double* out_dev, int64_t n, double init);
+int kkmx_std_reduce_i64(const int64_t* in_dev, int64_t n, int64_t init, int64_t* out_host);
+int kkmx_std_reduce_max_i64(const int64_t* in_dev, int64_t n, int64_t init, int64_t* out_host);
+int kkmx_std_transform_reduce_dot_f64(const double* a_dev, const double* b_dev, int64_t n, double init,
+                                      double* out_host);
+
 /* ---- parallel_reduce into a device result ---- */
 int kkmx_reduce_sum_f64(const double* x_dev, int64_t n, double* out_dev);
 /* op: 0 = Sum, 1 = Min, 2 = Max */

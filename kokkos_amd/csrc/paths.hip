@@ -6,6 +6,7 @@
 // instance.  Nothing here calls HIP directly.
 #include <Kokkos_Core.hpp>
 #include <Kokkos_Batch.hpp>
+#include <Kokkos_StdAlgorithms.hpp>
 
 #include <cmath>
 #include <memory>
@@ -490,6 +491,66 @@ int kkmx_deep_copy_f64(double* dst, const double* src, int64_t n0, int64_t n1, i
 int kkmx_deep_fill_f64(double* dst, int64_t n, double value) {
   if (n < 0 || (n > 0 && !dst)) return reject("kkmx_deep_fill_f64: bad arguments");
   return guarded([&] { Kokkos::deep_copy(Kokkos::MI355X(), DevView<double>(dst, n), value); });
+}
+namespace {
+struct MaxI32Op {
+  KOKKOS_FUNCTION int32_t operator()(const int32_t a, const int32_t b) const { return a > b ? a : b; }
+};
+struct MaxI64Op {
+  KOKKOS_FUNCTION int64_t operator()(const int64_t a, const int64_t b) const { return a > b ? a : b; }
+};
+struct PlusF64Op {
+  KOKKOS_FUNCTION double operator()(const double a, const double b) const { return a + b; }
+};
+struct SquareF64Op {
+  KOKKOS_FUNCTION double operator()(const double a) const { return a * a; }
+};
+}  // namespace
+int kkmx_std_inclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n) {
+  if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_std_inclusive_scan_i32: bad arguments");
+  return guarded([&] {
+    namespace KE = Kokkos::Experimental;
+    KE::inclusive_scan(Kokkos::MI355X(), DevView<const int32_t>(in, n), DevView<int32_t>(out, n));
+  });
+}
+int kkmx_std_exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, int64_t init) {
+  if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_std_exclusive_scan_i64: bad arguments");
+  return guarded([&] {
+    namespace KE = Kokkos::Experimental;
+    KE::exclusive_scan(Kokkos::MI355X(), DevView<const int64_t>(in, n), DevView<int64_t>(out, n), init);
+  });
+}
+int kkmx_std_inclusive_scan_max_i32(const int32_t* in, int32_t* out, int64_t n) {
+  if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_std_inclusive_scan_max_i32: bad arguments");
+  return guarded([&] {
+    namespace KE = Kokkos::Experimental;
+    KE::inclusive_scan(Kokkos::MI355X(), DevView<const int32_t>(in, n), DevView<int32_t>(out, n), MaxI32Op{});
+  });
+}
+int kkmx_std_transform_exclusive_scan_sq_f64(const double* in, double* out, int64_t n, double init) {
+  if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_std_transform_exclusive_scan_sq_f64: bad arguments");
+  return guarded([&] {
+    namespace KE = Kokkos::Experimental;
+    KE::transform_exclusive_scan(Kokkos::MI355X(), DevView<const double>(in, n), DevView<double>(out, n), init,
+                                 PlusF64Op{}, SquareF64Op{});
+  });
+}
+int kkmx_std_reduce_i64(const int64_t* in, int64_t n, int64_t init, int64_t* out) {
+  if (n < 0 || (n > 0 && !in) || !out) return reject("kkmx_std_reduce_i64: bad arguments");
+  return guarded([&] { *out = Kokkos::Experimental::reduce(Kokkos::MI355X(), DevView<const int64_t>(in, n), init); });
+}
+int kkmx_std_reduce_max_i64(const int64_t* in, int64_t n, int64_t init, int64_t* out) {
+  if (n < 0 || (n > 0 && !in) || !out) return reject("kkmx_std_reduce_max_i64: bad arguments");
+  return guarded([&] {
+    *out = Kokkos::Experimental::reduce(Kokkos::MI355X(), DevView<const int64_t>(in, n), init, MaxI64Op{});
+  });
+}
+int kkmx_std_transform_reduce_dot_f64(const double* a, const double* b, int64_t n, double init, double* out) {
+  if (n < 0 || (n > 0 && (!a || !b)) || !out) return reject("kkmx_std_transform_reduce_dot_f64: bad arguments");
+  return guarded([&] {
+    *out = Kokkos::Experimental::transform_reduce(Kokkos::MI355X(), DevView<const double>(a, n),
+                                                  DevView<const double>(b, n), init);
+  });
 }
 int kkmx_stream_copy(const double* a, double* b, int64_t n) {
   if (n < 0 || n > 0x7fffffff) return reject("kkmx_stream_copy: n must fit IndexType<int>");

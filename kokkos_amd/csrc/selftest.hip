@@ -7,6 +7,7 @@
 // run them one by one on the GPU.  Closed forms and invariants are the ones
 // the reference's tests assert; each test names the reference test it follows.
 #include <Kokkos_Core.hpp>
+#include <Kokkos_StdAlgorithms.hpp>
 
 #include <cstdio>
 #include <cstring>
@@ -1593,6 +1594,74 @@ KKMX_TEST(deep_copy_contiguous_and_strided_sources) {
   auto hd = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), d);
   for (int i = 0; i < 30; ++i)
     for (int j = 0; j < 20; ++j) CHECK_EQ(hd(i, j), double(i * 2 + j * 128));
+}
+
+// std algorithms: iterator forms (algorithms/src/std_algorithms)
+struct MinI64Op {
+  KOKKOS_FUNCTION int64_t operator()(const int64_t a, const int64_t b) const { return a < b ? a : b; }
+};
+struct TimesTwo {
+  KOKKOS_FUNCTION int64_t operator()(const int64_t a) const { return 2 * a; }
+};
+KKMX_TEST(std_iterators_and_subranges) {
+  namespace KE = Kokkos::Experimental;
+  const int n = 1000;
+  Kokkos::View<int64_t*, Exec> x("x", n), y("y", n);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), KOKKOS_LAMBDA(const int i) { x(i) = (i * 37) % 101 - 50; });
+  Kokkos::deep_copy(y, -1);
+  CHECK_EQ(KE::distance(KE::begin(x), KE::end(x)), ptrdiff_t(n));
+  Kokkos::MI355X ex;
+  // inclusive scan of x[10, 995) into y[10, ...)
+  auto last = KE::inclusive_scan(ex, KE::cbegin(x) + 10, KE::cend(x) - 5, KE::begin(y) + 10);
+  CHECK_EQ(KE::distance(KE::begin(y), last), ptrdiff_t(n - 5));
+  auto hx = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), x);
+  auto hy = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), y);
+  int64_t s = 0;
+  for (int i = 0; i < n; ++i) {
+    if (i >= 10 && i < n - 5) {
+      s += hx(i);
+      CHECK_EQ(hy(i), s);
+    } else {
+      CHECK_EQ(hy(i), int64_t(-1));
+    }
+  }
+  // running minimum with an initial value, and a transform scan
+  KE::inclusive_scan(ex, x, y, MinI64Op{}, int64_t(-7));
+  Kokkos::deep_copy(hy, y);
+  int64_t m = -7;
+  for (int i = 0; i < n; ++i) {
+    m = hx(i) < m ? hx(i) : m;
+    CHECK_EQ(hy(i), m);
+  }
+  KE::transform_inclusive_scan(ex, x, y, KE::Impl::PlusOp{}, TimesTwo{});
+  Kokkos::deep_copy(hy, y);
+  s = 0;
+  for (int i = 0; i < n; ++i) {
+    s += 2 * hx(i);
+    CHECK_EQ(hy(i), s);
+  }
+  CHECK_EQ(KE::reduce(ex, x, int64_t(3), MinI64Op{}), int64_t(-50));
+  CHECK_EQ(KE::transform_reduce(ex, x, int64_t(0), KE::Impl::PlusOp{}, TimesTwo{}), 2 * s / 2);
+}
+KKMX_TEST(std_strided_views) {
+  namespace KE = Kokkos::Experimental;
+  // every third element of a buffer, scanned into every second of another
+  Kokkos::View<int32_t*, Exec> a("a", 300), b("b", 200);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, 300), KOKKOS_LAMBDA(const int i) { a(i) = i; });
+  Kokkos::deep_copy(b, 0);
+  Kokkos::View<int32_t*, Kokkos::LayoutStride, Exec> as(a.data(), Kokkos::LayoutStride(100, 3));
+  Kokkos::View<int32_t*, Kokkos::LayoutStride, Exec> bs(b.data(), Kokkos::LayoutStride(100, 2));
+  Kokkos::MI355X ex;
+  auto last = KE::exclusive_scan(ex, as, bs, int32_t(5));
+  CHECK_EQ(KE::distance(KE::begin(bs), last), ptrdiff_t(100));
+  auto hb = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), b);
+  int32_t s = 5;
+  for (int k = 0; k < 100; ++k) {
+    CHECK_EQ(hb(2 * k), s);
+    CHECK_EQ(hb(2 * k + 1), 0);
+    s += 3 * k;
+  }
+  CHECK_EQ(KE::reduce(ex, as), int32_t(3 * 99 * 100 / 2));
 }
 
 }  // namespace
