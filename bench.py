@@ -242,7 +242,8 @@ def leg_std_algorithms(K, P, timer):
 
 def leg_atomics(K, P, timer):
     """GUPS (2^30 int64 table, 2^28 updates), Zipf(1.1) histogram of 2^28
-    keys over 65536 bins (LDS-privatised), FP64 scatter of 2^26 into 2^20."""
+    keys over 65536 bins (LDS-privatised, and through ScatterView), FP64
+    scatter of 2^26 into 2^20 (atomic_add, and through ScatterView)."""
     chk = lambda rc: K.check(rc, P)  # noqa: E731
     rt = K.load("kkmx_rt")
     T, U = 1 << 30, 1 << 28
@@ -255,18 +256,28 @@ def leg_atomics(K, P, timer):
     Kb, Bb = K.DeviceBuffer(nk * 4), K.DeviceBuffer(nb * 4)
     chk(P.kkmx_zipf_keys(Kb.ptr, nk, 1.1, nb, 8))
     h = _timeit(K, timer, lambda: chk(P.kkmx_histogram(Kb.ptr, nk, Bb.ptr, nb, 1, 32768)), reps=5)
+    hsv = _timeit(K, timer, lambda: chk(P.kkmx_scatter_view_histogram(Kb.ptr, nk, Bb.ptr, nb, 0)), reps=2, warm=1)
+    hdup = _timeit(K, timer, lambda: chk(P.kkmx_scatter_view_histogram(Kb.ptr, nk, Bb.ptr, nb, 1)), reps=5)
     del Kb, Bb
     ns, nbins = 1 << 26, 1 << 20
     Vb, Ib, Bb = K.DeviceBuffer(ns * 8), K.DeviceBuffer(ns * 4), K.DeviceBuffer(nbins * 8)
     chk(P.kkmx_fill_uniform_f64(Vb.ptr, ns, 9, 0.0, 1.0))
     chk(P.kkmx_fill_uniform_i32(Ib.ptr, ns, 9 ^ 0xABCDEF, 0, nbins - 1))
     sc = _timeit(K, timer, lambda: chk(P.kkmx_scatter_add_f64(Vb.ptr, Ib.ptr, ns, Bb.ptr, nbins)), reps=5)
+    ssv = _timeit(K, timer, lambda: chk(P.kkmx_scatter_view_f64(Vb.ptr, Ib.ptr, ns, Bb.ptr, nbins, 0)), reps=5)
+    del Vb, Ib, Bb
     return {
         "gups_2^30_table_2^28_updates": {"ms": round(g[0], 3), "gups": round(U / g[0] / 1e6, 3)},
         "histogram_zipf_2^28_keys": {"ms": round(h[0], 3), "gbps": round(4 * nk / h[0] / 1e6, 1),
                                      "keys_per_s": round(nk / h[0] * 1e3, 1)},
         "scatter_f64_2^26_into_2^20": {"ms": round(sc[0], 3), "updates_per_s": round(ns / sc[0] * 1e3, 1),
                                        "gbps": round(12 * ns / sc[0] / 1e6, 1)},
+        "scatter_view_f64_2^26_into_2^20": {"ms": round(ssv[0], 3), "updates_per_s": round(ns / ssv[0] * 1e3, 1)},
+        "histogram_scatter_view_atomic_2^28_keys": {"ms": round(hsv[0], 3),
+                                                    "keys_per_s": round(nk / hsv[0] * 1e3, 1)},
+        "histogram_scatter_view_duplicated_2^28_keys": {"ms": round(hdup[0], 3),
+                                                        "keys_per_s": round(nk / hdup[0] * 1e3, 1),
+                                                        "gbps": round(4 * nk / hdup[0] / 1e6, 1)},
     }
 
 

@@ -131,6 +131,16 @@ int kkmx_xorshift64_stream(uint64_t* out_dev, int64_t n, uint64_t seed);
 int kkmx_gups(int64_t* table_dev, int64_t table_size, const uint64_t* stream_dev, int64_t nupdates);
 int kkmx_zipf_keys(int32_t* keys_dev, int64_t n, double s, int32_t nbins, uint64_t seed);
 int kkmx_histogram(const int32_t* keys_dev, int64_t n, int32_t* bins_dev, int32_t nbins, int mode, int lds_bins);
+/* The same scatter and histogram through Kokkos::Experimental::ScatterView
+ * (containers/src/Kokkos_ScatterView.hpp): create_scatter_view(bins),
+ * access()(idx) += value in the loop, contribute(bins, sv).  op for the
+ * float64 form: 0 = ScatterSum, 1 = ScatterMin, 2 = ScatterMax. */
+int kkmx_scatter_view_f64(const double* vals_dev, const int32_t* idx_dev, int64_t n, double* bins_dev,
+                          int32_t nbins, int op);
+/* duplicated = 0: ScatterNonDuplicated (one atomic copy); 1: ScatterDuplicated
+ * (replicas folded by contribute). */
+int kkmx_scatter_view_histogram(const int32_t* keys_dev, int64_t n, int32_t* bins_dev, int32_t nbins,
+                                int duplicated);
 int kkmx_scatter_add_f64(const double* vals_dev, const int32_t* idx_dev, int64_t n, double* bins_dev, int32_t nbins);
 
 /* ---- batched uneven dispatch (config 5) ----

@@ -7,6 +7,7 @@
 #include <Kokkos_Core.hpp>
 #include <Kokkos_Batch.hpp>
 #include <Kokkos_StdAlgorithms.hpp>
+#include <Kokkos_ScatterView.hpp>
 
 #include <cmath>
 #include <memory>
@@ -413,6 +414,35 @@ int reduce_int(const T* x, int64_t n, int op, int64_t* out) {
 }
 }  // namespace
 
+namespace {
+template <class SV>
+struct ScatterViewF64 {
+  DevView<const double> vals;
+  DevView<const int32_t> idx;
+  SV sv;
+  int op;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const {
+    auto a = sv.access();
+    if constexpr (std::is_same<typename SV::original_value_type, double>::value) a(idx(i)).update(vals(i));
+  }
+};
+template <class SV>
+struct ScatterViewHist {
+  DevView<const int32_t> keys;
+  SV sv;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { sv.access()(keys(i)) += 1; }
+};
+template <class Op>
+void scatter_view_f64(const double* vals, const int32_t* idx, int64_t n, double* bins, int32_t nbins) {
+  namespace KE = Kokkos::Experimental;
+  DevView<double> B(bins, nbins);
+  auto sv = KE::create_scatter_view(Op{}, B);
+  Kokkos::parallel_for("kkmx::scatter_view_f64", Range64(0, n),
+                       ScatterViewF64<decltype(sv)>{DevView<const double>(vals, n), DevView<const int32_t>(idx, n), sv, 0});
+  KE::contribute(B, sv);
+}
+}  // namespace
+
 extern "C" {
 
 int kkmx_init(int device) {
@@ -780,6 +810,30 @@ int kkmx_histogram(const int32_t* keys, int64_t n, int32_t* bins, int32_t nbins,
     const int cus = Kokkos::MI355X::impl_cu_count();
     auto pol = Kokkos::TeamPolicy<Kokkos::MI355X>(cus, 1024).set_scratch_size(0, Kokkos::PerTeam(size_t(lb) * 4));
     Kokkos::parallel_for("kkmx::hist_lds", pol, HistPrivatised{K, B, n, lb});
+  });
+}
+int kkmx_scatter_view_f64(const double* vals, const int32_t* idx, int64_t n, double* bins, int32_t nbins, int op) {
+  if (n < 0 || nbins < 1 || !bins || (n && (!vals || !idx)) || op < 0 || op > 2)
+    return reject("kkmx_scatter_view_f64: bad arguments");
+  return guarded([&] {
+    namespace KE = Kokkos::Experimental;
+    if (op == 0) scatter_view_f64<KE::ScatterSum>(vals, idx, n, bins, nbins);
+    else if (op == 1) scatter_view_f64<KE::ScatterMin>(vals, idx, n, bins, nbins);
+    else scatter_view_f64<KE::ScatterMax>(vals, idx, n, bins, nbins);
+  });
+}
+int kkmx_scatter_view_histogram(const int32_t* keys, int64_t n, int32_t* bins, int32_t nbins, int duplicated) {
+  if (n < 0 || nbins < 1 || !bins || (n && !keys)) return reject("kkmx_scatter_view_histogram: bad arguments");
+  return guarded([&] {
+    namespace KE = Kokkos::Experimental;
+    DevView<int32_t> B(bins, nbins);
+    auto run = [&](auto sv) {
+      Kokkos::parallel_for("kkmx::scatter_view_histogram", Range64(0, n),
+                           ScatterViewHist<decltype(sv)>{DevView<const int32_t>(keys, n), sv});
+      KE::contribute(B, sv);
+    };
+    if (duplicated) run(KE::create_scatter_view(KE::ScatterSum{}, KE::ScatterDuplicated{}, KE::ScatterAtomic{}, B));
+    else run(KE::create_scatter_view(B));
   });
 }
 int kkmx_scatter_add_f64(const double* vals, const int32_t* idx, int64_t n, double* bins, int32_t nbins) {

@@ -8,6 +8,7 @@
 // the reference's tests assert; each test names the reference test it follows.
 #include <Kokkos_Core.hpp>
 #include <Kokkos_StdAlgorithms.hpp>
+#include <Kokkos_ScatterView.hpp>
 
 #include <cstdio>
 #include <cstring>
@@ -1662,6 +1663,115 @@ KKMX_TEST(std_strided_views) {
     s += 3 * k;
   }
   CHECK_EQ(KE::reduce(ex, as), int32_t(3 * 99 * 100 / 2));
+}
+
+// ScatterView (containers/src/Kokkos_ScatterView.hpp; restates the sum /
+// prod / min / max checks of containers/unit_tests/TestScatterView.hpp)
+template <class SV>
+struct ScatterSumF {
+  SV sv;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
+    auto a = sv.access();
+    a(i % 37) += int64_t(i);
+    a(i % 37 + 37) -= 1;
+    a[74] += 1;
+  }
+};
+KKMX_TEST(scatter_view_sum_into_original_and_contribute) {
+  namespace KE = Kokkos::Experimental;
+  const int n = 100000;
+  Kokkos::View<int64_t*, Exec> bins("bins", 75);
+  Kokkos::deep_copy(bins, 0);
+  auto sv = KE::create_scatter_view(bins);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), ScatterSumF<decltype(sv)>{sv});
+  KE::contribute(bins, sv);  // same data: nothing to do
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), bins);
+  for (int b = 0; b < 37; ++b) {
+    int64_t want = 0, cnt = 0;
+    for (int i = b; i < n; i += 37) {
+      want += i;
+      ++cnt;
+    }
+    CHECK_EQ(h(b), want);
+    CHECK_EQ(h(b + 37), -cnt);
+  }
+  CHECK_EQ(h(74), int64_t(n));
+  // a separate ScatterView, reset, then contributed into an existing View
+  KE::ScatterView<int64_t*, Kokkos::LayoutRight, Exec> sv2("sv2", 75);
+  sv2.reset();
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), ScatterSumF<decltype(sv2)>{sv2});
+  KE::contribute(bins, sv2);
+  Kokkos::deep_copy(h, bins);
+  CHECK_EQ(h(74), int64_t(2 * n));
+  CHECK_EQ(h(0), 2 * ((n - 1) / 37) * ((n - 1) / 37 + 1) / 2 * 37);
+}
+template <class SMax, class SMin, class SProd>
+struct ScatterMinMaxProdF {
+  SMax mx;
+  SMin mn;
+  SProd pr;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
+    const double v = double((i * 7919) % 10007) - 5000.0;
+    mx.access()(i % 16).update(v);
+    mn.access()(i % 16).update(v);
+    if (i < 64) pr.access()(i % 4) *= int64_t(2);
+  }
+};
+KKMX_TEST(scatter_view_min_max_prod) {
+  namespace KE = Kokkos::Experimental;
+  const int n = 50000;
+  Kokkos::View<double*, Exec> vmax("vmax", 16), vmin("vmin", 16);
+  Kokkos::View<int64_t*, Exec> vprod("vprod", 4);
+  auto smax  = KE::create_scatter_view(KE::ScatterMax{}, vmax);
+  auto smin  = KE::create_scatter_view(KE::ScatterMin{}, vmin);
+  auto sprod = KE::create_scatter_view(KE::ScatterProd{}, vprod);
+  smax.reset();
+  smin.reset();
+  sprod.reset();
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n),
+                       ScatterMinMaxProdF<decltype(smax), decltype(smin), decltype(sprod)>{smax, smin, sprod});
+  auto hmax = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), vmax);
+  auto hmin = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), vmin);
+  auto hpr  = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), vprod);
+  for (int b = 0; b < 16; ++b) {
+    double wmax = -1e300, wmin = 1e300;
+    for (int i = b; i < n; i += 16) {
+      const double v = double((i * 7919) % 10007) - 5000.0;
+      wmax = v > wmax ? v : wmax;
+      wmin = v < wmin ? v : wmin;
+    }
+    CHECK_EQ(hmax(b), wmax);
+    CHECK_EQ(hmin(b), wmin);
+  }
+  for (int b = 0; b < 4; ++b) CHECK_EQ(hpr(b), int64_t(1) << 16);
+}
+
+KKMX_TEST(scatter_view_duplicated_replicas) {
+  namespace KE = Kokkos::Experimental;
+  const int n = 300000;
+  Kokkos::View<int64_t*, Exec> bins("bins", 75);
+  Kokkos::deep_copy(bins, 5);  // contributions add to what is there
+  auto sv = KE::create_scatter_view(KE::ScatterSum{}, KE::ScatterDuplicated{}, KE::ScatterAtomic{}, bins);
+  CHECK(sv.replicas() > 1);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), ScatterSumF<decltype(sv)>{sv});
+  KE::contribute(bins, sv);
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), bins);
+  for (int b = 0; b < 37; ++b) {
+    int64_t want = 5, cnt = 0;
+    for (int i = b; i < n; i += 37) {
+      want += i;
+      ++cnt;
+    }
+    CHECK_EQ(h(b), want);
+    CHECK_EQ(h(b + 37), 5 - cnt);
+  }
+  CHECK_EQ(h(74), int64_t(n) + 5);
+  // reset_except, a second pass: the replicas start from zero again
+  sv.reset_except(bins);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), ScatterSumF<decltype(sv)>{sv});
+  KE::contribute(bins, sv);
+  Kokkos::deep_copy(h, bins);
+  CHECK_EQ(h(74), int64_t(2 * n) + 5);
 }
 
 }  // namespace
