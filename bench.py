@@ -240,6 +240,45 @@ def leg_std_algorithms(K, P, timer):
     }
 
 
+def leg_sort(K, P, timer):
+    """Kokkos::sort (radix) of 2^28 int32 and 2^27 float64 keys, and BinSort
+    of 2^24 float64 keys over 2^20 bins with the within-bin sort."""
+    chk = lambda rc: K.check(rc, P)  # noqa: E731
+    n32, n64 = 1 << 28, 1 << 27
+    A = K.DeviceBuffer(n32 * 4)
+    fill32 = lambda: chk(P.kkmx_fill_uniform_i32(A.ptr, n32, 91, -(1 << 30), 1 << 30))  # noqa: E731
+    fill32()
+    t32 = []
+    for _ in range(3):
+        fill32()
+        K.fence()
+        timer.start()
+        chk(P.kkmx_sort(A.ptr, n32, 0))
+        t32.append(timer.stop())
+    del A
+    B = K.DeviceBuffer(n64 * 8)
+    t64 = []
+    for _ in range(3):
+        chk(P.kkmx_fill_uniform_f64(B.ptr, n64, 92, -1.0, 1.0))
+        K.fence()
+        timer.start()
+        chk(P.kkmx_sort(B.ptr, n64, 3))
+        t64.append(timer.stop())
+    del B
+    nb, bins = 1 << 24, 1 << 20
+    Kd, Vd = K.DeviceBuffer(nb * 8), K.DeviceBuffer(nb * 8)
+    chk(P.kkmx_fill_uniform_f64(Kd.ptr, nb, 93, 0.0, 1.0))
+    chk(P.kkmx_fill_uniform_i64(Vd.ptr, nb, 94, 0, 1000))
+    bs = _timeit(K, timer, lambda: chk(P.kkmx_binsort_f64(Kd.ptr, Vd.ptr, nb, bins, 0.0, 1.0, 1, None)), reps=3,
+                 warm=1)
+    del Kd, Vd
+    return {
+        "sort_int32_2^28": {"ms": round(min(t32), 3), "keys_per_s": round(n32 / min(t32) * 1e3, 1)},
+        "sort_f64_2^27": {"ms": round(min(t64), 3), "keys_per_s": round(n64 / min(t64) * 1e3, 1)},
+        "binsort_f64_2^24_keys_2^20_bins": {"ms": round(bs[0], 3), "keys_per_s": round(nb / bs[0] * 1e3, 1)},
+    }
+
+
 def leg_atomics(K, P, timer):
     """GUPS (2^30 int64 table, 2^28 updates), Zipf(1.1) histogram of 2^28
     keys over 65536 bins (LDS-privatised, and through ScatterView), FP64
@@ -382,7 +421,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="only the headline workload")
     ap.add_argument("--only", choices=["small_reduce_scan", "mdrange_team", "atomics", "deep_copy", "std_algorithms",
-                                       "batched"],
+                                       "sort", "batched"],
                     help="run one secondary workload alone and print its JSON (for profiling); no headline line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -400,7 +439,8 @@ def main():
     P = K.init(0)
     if args.only:
         legs = {"small_reduce_scan": leg_small, "mdrange_team": leg_md_team, "atomics": leg_atomics,
-                "deep_copy": leg_deep_copy, "std_algorithms": leg_std_algorithms, "batched": leg_batched}
+                "deep_copy": leg_deep_copy, "std_algorithms": leg_std_algorithms, "sort": leg_sort,
+                "batched": leg_batched}
         print(json.dumps({args.only: legs[args.only](K, P, K.EventTimer())}), flush=True)
         return
     props = K.device_props()
@@ -487,6 +527,7 @@ def main():
                          ("atomics", lambda: leg_atomics(K, P, timer)),
                          ("deep_copy", lambda: leg_deep_copy(K, P, timer)),
                          ("std_algorithms", lambda: leg_std_algorithms(K, P, timer)),
+                         ("sort", lambda: leg_sort(K, P, timer)),
                          ("batched", lambda: leg_batched(K, P, timer))]:
             t_leg = time.perf_counter()
             extra[name] = fn()

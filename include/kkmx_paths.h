@@ -79,6 +79,17 @@ int kkmx_std_reduce_max_i64(const int64_t* in_dev, int64_t n, int64_t init, int6
 int kkmx_std_transform_reduce_dot_f64(const double* a_dev, const double* b_dev, int64_t n, double init,
                                       double* out_host);
 
+/* ---- sorting (algorithms/src/sorting/Kokkos_SortPublicAPI.hpp,
+ * Kokkos_BinSortPublicAPI.hpp, Kokkos_BinOpsPublicAPI.hpp).
+ * kkmx_sort: Kokkos::sort of n keys in place, ascending; type 0 = int32,
+ *   1 = int64, 2 = float, 3 = double, 4 = uint32, 5 = uint64.
+ * kkmx_binsort_f64: BinSort(keys, BinOp1D(nbins, kmin, kmax),
+ *   sort_within_bins).create_permute_vector(), then sort(values); perm_out
+ *   (n ints, may be null) receives the permutation vector. ---- */
+int kkmx_sort(void* keys_dev, int64_t n, int type);
+int kkmx_binsort_f64(const double* keys_dev, int64_t* values_dev, int64_t n, int nbins, double kmin, double kmax,
+                     int sort_within_bins, int32_t* perm_out_dev);
+
 /* ---- parallel_reduce into a device result ---- */
 int kkmx_reduce_sum_f64(const double* x_dev, int64_t n, double* out_dev);
 /* op: 0 = Sum, 1 = Min, 2 = Max */

@@ -8,6 +8,7 @@
 #include <Kokkos_Batch.hpp>
 #include <Kokkos_StdAlgorithms.hpp>
 #include <Kokkos_ScatterView.hpp>
+#include <Kokkos_Sort.hpp>
 
 #include <cmath>
 #include <memory>
@@ -536,6 +537,36 @@ struct SquareF64Op {
   KOKKOS_FUNCTION double operator()(const double a) const { return a * a; }
 };
 }  // namespace
+int kkmx_sort(void* keys, int64_t n, int type) {
+  if (n < 0 || (n > 0 && !keys) || type < 0 || type > 5) return reject("kkmx_sort: bad arguments");
+  return guarded([&] {
+    Kokkos::MI355X ex;
+    switch (type) {
+      case 0: Kokkos::sort(ex, DevView<int32_t>(static_cast<int32_t*>(keys), n)); break;
+      case 1: Kokkos::sort(ex, DevView<int64_t>(static_cast<int64_t*>(keys), n)); break;
+      case 2: Kokkos::sort(ex, DevView<float>(static_cast<float*>(keys), n)); break;
+      case 3: Kokkos::sort(ex, DevView<double>(static_cast<double*>(keys), n)); break;
+      case 4: Kokkos::sort(ex, DevView<uint32_t>(static_cast<uint32_t*>(keys), n)); break;
+      default: Kokkos::sort(ex, DevView<uint64_t>(static_cast<uint64_t*>(keys), n)); break;
+    }
+  });
+}
+int kkmx_binsort_f64(const double* keys, int64_t* values, int64_t n, int nbins, double kmin, double kmax,
+                     int sort_within_bins, int32_t* perm_out) {
+  if (n < 0 || n > 0x7fffffff || nbins < 1 || (n > 0 && (!keys || !values)) || !(kmax > kmin))
+    return reject("kkmx_binsort_f64: bad arguments");
+  return guarded([&] {
+    using KV = DevView<const double>;
+    KV K(keys, n);
+    Kokkos::MI355X ex;
+    Kokkos::BinSort<KV, Kokkos::BinOp1D<KV>> bs(ex, K, Kokkos::BinOp1D<KV>(nbins, kmin, kmax), sort_within_bins != 0);
+    bs.create_permute_vector(ex);
+    bs.sort(ex, DevView<int64_t>(values, n));
+    if (perm_out && n > 0)
+      KKMX_SAFE_CALL(mx_memcpy_async(perm_out, bs.get_permute_vector().data(), size_t(n) * 4, ex.stream()));
+    ex.fence();
+  });
+}
 int kkmx_std_inclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n) {
   if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_std_inclusive_scan_i32: bad arguments");
   return guarded([&] {

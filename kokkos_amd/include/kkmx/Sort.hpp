@@ -1,0 +1,564 @@
+// Sorting for the MI355X execution space: Kokkos::BinSort with BinOp1D /
+// BinOp3D, and Kokkos::sort.
+//
+// Reference: algorithms/src/sorting/Kokkos_BinSortPublicAPI.hpp (BinSort:
+// count keys per bin, exclusive scan of the counts into bin offsets, place
+// every index at its bin's next slot, optionally sort within bins, then
+// permute values through the permutation vector),
+// algorithms/src/sorting/Kokkos_BinOpsPublicAPI.hpp (BinOp1D / BinOp3D:
+// bin(keys, i), max_bins(), and the in-bin comparison), and
+// algorithms/src/sorting/Kokkos_SortPublicAPI.hpp (sort(view),
+// sort(exec, view), sort(view, comparator), sort(view, begin, end)).
+//
+// Kokkos::sort of a contiguous rank-1 device View of 4- or 8-byte
+// arithmetic keys is an LSD radix sort written for gfx950: 8-bit digits,
+// one histogram pass and one scatter pass per digit over 4096-key tiles of
+// 256 lanes.  The scatter ranks keys with equal digits inside each
+// wavefront from eight ballots, combines the wavefronts through LDS into a
+// digit-sorted copy of the tile in LDS, and writes that copy out so each
+// digit's keys leave as one contiguous run, at offsets from a digit-major
+// exclusive scan of the per-tile histograms: every pass reads its keys
+// twice and writes them once, stably.  Signed and floating-point keys are mapped to unsigned order on the
+// first pass and back on the last.  Other element types, comparators and
+// host Views sort on the host with std::sort (results identical to an
+// ascending sort).
+#ifndef KKMX_SORT_HPP
+#define KKMX_SORT_HPP
+
+#include <Kokkos_Core.hpp>
+
+#include <algorithm>
+#include <vector>
+
+namespace Kokkos {
+
+// ------------------------------------------------------------------ bin ops
+template <class KeyViewType>
+struct BinOp1D {
+  int max_bins_ = 0;
+  double mul_   = 0;
+  double min_   = 0;
+  BinOp1D() = delete;
+  // max_bins bins over [min, max] (one extra bin holds max itself); integer
+  // keys whose range is at most max_bins get one value per bin.
+  BinOp1D(int max_bins, typename KeyViewType::const_value_type min, typename KeyViewType::const_value_type max)
+      : max_bins_(max_bins + 1),
+        mul_(static_cast<double>(max_bins) / (static_cast<double>(max) - static_cast<double>(min))),
+        min_(static_cast<double>(min)) {
+    if (std::is_integral<typename KeyViewType::non_const_value_type>::value &&
+        static_cast<double>(max) - static_cast<double>(min) <= static_cast<double>(max_bins))
+      mul_ = 1.0;
+  }
+  template <class ViewType>
+  KOKKOS_INLINE_FUNCTION int bin(ViewType& keys, const int& i) const {
+    return static_cast<int>(mul_ * (static_cast<double>(keys(i)) - min_));
+  }
+  KOKKOS_INLINE_FUNCTION int max_bins() const { return max_bins_; }
+  // true: key i1 goes before key i2 inside a bin
+  template <class ViewType, class I1, class I2>
+  KOKKOS_INLINE_FUNCTION bool operator()(ViewType& keys, I1& i1, I2& i2) const {
+    return keys(i1) < keys(i2);
+  }
+};
+
+template <class KeyViewType>
+struct BinOp3D {
+  int max_bins_[3] = {0, 0, 0};
+  double mul_[3]   = {0, 0, 0};
+  double min_[3]   = {0, 0, 0};
+  BinOp3D() = delete;
+  BinOp3D(int max_bins[], typename KeyViewType::const_value_type min[],
+          typename KeyViewType::const_value_type max[]) {
+    for (int d = 0; d < 3; ++d) {
+      max_bins_[d] = max_bins[d];
+      mul_[d]      = static_cast<double>(max_bins[d]) / (static_cast<double>(max[d]) - static_cast<double>(min[d]));
+      min_[d]      = static_cast<double>(min[d]);
+    }
+  }
+  template <class ViewType>
+  KOKKOS_INLINE_FUNCTION int bin(ViewType& keys, const int& i) const {
+    const int b0 = int(mul_[0] * (keys(i, 0) - min_[0]));
+    const int b1 = int(mul_[1] * (keys(i, 1) - min_[1]));
+    const int b2 = int(mul_[2] * (keys(i, 2) - min_[2]));
+    return (b0 * max_bins_[1] + b1) * max_bins_[2] + b2;
+  }
+  KOKKOS_INLINE_FUNCTION int max_bins() const { return max_bins_[0] * max_bins_[1] * max_bins_[2]; }
+  // lexicographically greater first (the reference's in-bin order)
+  template <class ViewType, class I1, class I2>
+  KOKKOS_INLINE_FUNCTION bool operator()(ViewType& keys, I1& i1, I2& i2) const {
+    for (int d = 0; d < 3; ++d) {
+      if (keys(i1, d) > keys(i2, d)) return true;
+      if (keys(i1, d) != keys(i2, d)) return false;
+    }
+    return false;
+  }
+};
+
+namespace Impl {
+
+template <class Keys, class Op>
+struct BinCountF {
+  Keys keys;
+  Op op;
+  View<int*, MI355XSpace, MemoryTraits<Unmanaged>> count;
+  int begin;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const { atomic_add(&count(op.bin(keys, begin + i)), 1); }
+};
+struct BinOffsetF {
+  View<const int*, MI355XSpace, MemoryTraits<Unmanaged>> count;
+  View<int*, MI355XSpace, MemoryTraits<Unmanaged>> offsets;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, int& upd, const bool final) const {
+    if (final) offsets(i) = upd;
+    upd += count(i);
+  }
+};
+template <class Keys, class Op>
+struct BinPlaceF {
+  Keys keys;
+  Op op;
+  View<int*, MI355XSpace, MemoryTraits<Unmanaged>> fill;
+  View<const int*, MI355XSpace, MemoryTraits<Unmanaged>> offsets;
+  View<int*, MI355XSpace, MemoryTraits<Unmanaged>> order;
+  int begin;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
+    const int j = begin + i;
+    const int b = op.bin(keys, j);
+    order(offsets(b) + atomic_fetch_add(&fill(b), 1)) = j;
+  }
+};
+// Insertion sort of one bin's indices by one thread, as the reference does on
+// devices (quadratic in the bin size: meant for the small bins BinSort is
+// used with); equal keys keep index order, so the result does not depend on
+// the order the binning atomics ran in.
+template <class Keys, class Op>
+struct BinSortWithinF {
+  Keys keys;
+  Op op;
+  View<const int*, MI355XSpace, MemoryTraits<Unmanaged>> count;
+  View<const int*, MI355XSpace, MemoryTraits<Unmanaged>> offsets;
+  View<int*, MI355XSpace, MemoryTraits<Unmanaged>> order;
+  KOKKOS_INLINE_FUNCTION void operator()(const int b) const {
+    const int lo = offsets(b), hi = lo + count(b);
+    for (int k = lo + 1; k < hi; ++k) {
+      int cur = order(k);
+      int j   = k - 1;
+      while (j >= lo) {
+        int prev       = order(j);
+        const bool before = op(keys, cur, prev) || (!op(keys, prev, cur) && cur < prev);
+        if (!before) break;
+        order(j + 1) = prev;
+        --j;
+      }
+      order(j + 1) = cur;
+    }
+  }
+};
+template <class Dst, class Src>
+struct PermuteRowsF {
+  Dst dst;  // rows [0, len)
+  Src src;
+  View<const int*, MI355XSpace, MemoryTraits<Unmanaged>> order;
+  int shift;  // values index = order(i) + shift
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
+    const int s = order(i) + shift;
+    if constexpr (Dst::rank == 1) {
+      dst(i) = src(s);
+    } else {
+      for (size_t c = 0; c < dst.extent(1); ++c) dst(i, c) = src(s, c);
+    }
+  }
+};
+template <class Dst, class Src>
+struct CopyRowsF {
+  Dst dst;
+  Src src;
+  int begin;  // dst rows [begin, begin + len)
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const {
+    if constexpr (Dst::rank == 1) {
+      dst(begin + i) = src(i);
+    } else {
+      for (size_t c = 0; c < dst.extent(1); ++c) dst(begin + i, c) = src(i, c);
+    }
+  }
+};
+
+}  // namespace Impl
+
+// ------------------------------------------------------------------ BinSort
+template <class KeyViewType, class BinSortOp, class Space = typename KeyViewType::device_type,
+          class SizeType = typename KeyViewType::size_type>
+class BinSort {
+ public:
+  using exec_space       = MI355X;
+  using offset_type      = View<int*, MI355XSpace>;
+  using bin_count_type   = View<const int*, MI355XSpace>;
+  using const_key_view_type = typename KeyViewType::const_type;
+
+  BinSort() = delete;
+  template <class ExecutionSpace>
+  BinSort(const ExecutionSpace& exec, const_key_view_type keys_, int range_begin_, int range_end_, BinSortOp bin_op_,
+          bool sort_within_bins_ = false)
+      : keys(keys_),
+        range_begin(range_begin_),
+        range_end(range_end_),
+        bin_op(bin_op_),
+        sort_within_bins(sort_within_bins_) {
+    static_assert(std::is_same<ExecutionSpace, MI355X>::value, "BinSort runs on the MI355X execution space");
+    if (range_begin > range_end) Kokkos::abort("BinSort: range_begin > range_end");
+    sort_order  = offset_type("Kokkos::BinSort::sort_order", size_t(range_end - range_begin));
+    bin_count   = View<int*, MI355XSpace>("Kokkos::BinSort::bin_count", size_t(bin_op.max_bins()));
+    bin_fill    = View<int*, MI355XSpace>("Kokkos::BinSort::bin_fill", size_t(bin_op.max_bins()));
+    bin_offsets = offset_type("Kokkos::BinSort::bin_offsets", size_t(bin_op.max_bins()));
+    (void)exec;
+  }
+  BinSort(const_key_view_type keys_, int range_begin_, int range_end_, BinSortOp bin_op_, bool sort_within_bins_ = false)
+      : BinSort(exec_space{}, keys_, range_begin_, range_end_, bin_op_, sort_within_bins_) {}
+  template <class ExecutionSpace>
+  BinSort(const ExecutionSpace& exec, const_key_view_type keys_, BinSortOp bin_op_, bool sort_within_bins_ = false)
+      : BinSort(exec, keys_, 0, int(keys_.extent(0)), bin_op_, sort_within_bins_) {}
+  BinSort(const_key_view_type keys_, BinSortOp bin_op_, bool sort_within_bins_ = false)
+      : BinSort(exec_space{}, keys_, bin_op_, sort_within_bins_) {}
+
+  // Counts, offsets and the permutation; call again when the keys change.
+  template <class ExecutionSpace>
+  void create_permute_vector(const ExecutionSpace& exec) {
+    using U        = View<int*, MI355XSpace, MemoryTraits<Unmanaged>>;
+    using CU       = View<const int*, MI355XSpace, MemoryTraits<Unmanaged>>;
+    const int len  = range_end - range_begin;
+    const int bins = bin_op.max_bins();
+    deep_copy(exec, bin_count, 0);
+    deep_copy(exec, bin_fill, 0);
+    U cnt(bin_count.data(), size_t(bins)), fill(bin_fill.data(), size_t(bins));
+    U off(bin_offsets.data(), size_t(bins)), order(sort_order.data(), size_t(len));
+    parallel_for("Kokkos::Sort::BinCount", RangePolicy<MI355X>(exec, 0, len),
+                 Impl::BinCountF<const_key_view_type, BinSortOp>{keys, bin_op, cnt, range_begin});
+    parallel_scan("Kokkos::Sort::BinOffset", RangePolicy<MI355X>(exec, 0, bins), Impl::BinOffsetF{CU(cnt), off});
+    parallel_for("Kokkos::Sort::BinBinning", RangePolicy<MI355X>(exec, 0, len),
+                 Impl::BinPlaceF<const_key_view_type, BinSortOp>{keys, bin_op, fill, CU(off), order, range_begin});
+    if (sort_within_bins)
+      parallel_for("Kokkos::Sort::BinSort", RangePolicy<MI355X>(exec, 0, bins),
+                   Impl::BinSortWithinF<const_key_view_type, BinSortOp>{keys, bin_op, CU(cnt), CU(off), order});
+  }
+  void create_permute_vector() {
+    Kokkos::fence("Kokkos::BinSort::create_permute_vector: before");
+    exec_space e{};
+    create_permute_vector(e);
+    e.fence("Kokkos::BinSort::create_permute_vector: after");
+  }
+
+  // values(range_begin + i) = values(sort_order(i) + values_range_begin - range_begin)
+  template <class ExecutionSpace, class ValuesViewType>
+  void sort(const ExecutionSpace& exec, const ValuesViewType& values, int values_range_begin,
+            int values_range_end) const {
+    if (values.extent(0) == 0) return;
+    const int len = range_end - range_begin;
+    if (values_range_end - values_range_begin != len)
+      Kokkos::abort("BinSort::sort: values range length != permutation vector length");
+    using Tmp = View<typename ValuesViewType::non_const_data_type, typename ValuesViewType::array_layout, MI355XSpace>;
+    Tmp tmp(std::string("Kokkos::SortImpl::BinSortFunctor::sorted_values"), size_t(len),
+            ValuesViewType::rank > 1 ? values.extent(1) : ~size_t(0));  // ~0: no extent
+    View<const int*, MI355XSpace, MemoryTraits<Unmanaged>> order(sort_order.data(), size_t(len));
+    parallel_for("Kokkos::Sort::CopyPermute", RangePolicy<MI355X>(exec, 0, len),
+                 Impl::PermuteRowsF<Tmp, ValuesViewType>{tmp, values, order, values_range_begin - range_begin});
+    parallel_for("Kokkos::Sort::Copy", RangePolicy<MI355X>(exec, 0, len),
+                 Impl::CopyRowsF<ValuesViewType, Tmp>{values, tmp, range_begin});
+  }
+  template <class ValuesViewType>
+  void sort(const ValuesViewType& values, int values_range_begin, int values_range_end) const {
+    Kokkos::fence("Kokkos::BinSort::sort: before");
+    exec_space exec;
+    sort(exec, values, values_range_begin, values_range_end);
+    exec.fence("Kokkos::BinSort::sort: after");
+  }
+  template <class ExecutionSpace, class ValuesViewType, class = std::enable_if_t<is_execution_space<ExecutionSpace>::value>>
+  void sort(const ExecutionSpace& exec, const ValuesViewType& values) const {
+    sort(exec, values, 0, range_end - range_begin);
+  }
+  template <class ValuesViewType>
+  void sort(const ValuesViewType& values) const {
+    sort(values, 0, range_end - range_begin);
+  }
+
+  offset_type get_permute_vector() const { return sort_order; }
+  offset_type get_bin_offsets() const { return bin_offsets; }
+  bin_count_type get_bin_count() const { return bin_count; }
+
+ private:
+  const_key_view_type keys;
+  int range_begin = 0, range_end = 0;
+  BinSortOp bin_op;
+  bool sort_within_bins = false;
+  offset_type sort_order;
+  View<int*, MI355XSpace> bin_count, bin_fill;
+  offset_type bin_offsets;
+};
+
+// ------------------------------------------------------------------ radix sort
+namespace Impl {
+
+template <class T>
+struct RadixKey {
+  using U = std::conditional_t<sizeof(T) == 4, uint32_t, uint64_t>;
+  static constexpr U top = U(1) << (8 * sizeof(T) - 1);
+  KOKKOS_FORCEINLINE_FUNCTION static U to(T v) {
+    U u;
+    __builtin_memcpy(&u, &v, sizeof(T));
+    if constexpr (std::is_floating_point<T>::value) return (u & top) ? ~u : (u | top);
+    else if constexpr (std::is_signed<T>::value) return u ^ top;
+    else return u;
+  }
+  KOKKOS_FORCEINLINE_FUNCTION static T from(U u) {
+    if constexpr (std::is_floating_point<T>::value) u = (u & top) ? (u ^ top) : ~u;
+    else if constexpr (std::is_signed<T>::value) u ^= top;
+    T v;
+    __builtin_memcpy(&v, &u, sizeof(T));
+    return v;
+  }
+};
+
+constexpr int kRadixLanes = 256;
+constexpr int kRadixItems = 16;
+constexpr int kRadixTile  = kRadixLanes * kRadixItems;
+
+// Per-tile counts of digit (key >> shift) & 255, stored digit-major:
+// hist[d * ntiles + tile].
+template <class T, bool First>
+__global__ __launch_bounds__(256) void k_radix_hist(const void* in, const int64_t n, const int shift, uint32_t* hist,
+                                                    const int64_t ntiles) {
+  using R = RadixKey<T>;
+  using U = typename R::U;
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = int64_t(blockIdx.x) * kRadixTile + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kRadixItems; ++r) {
+    const int64_t i = base + int64_t(r) * kRadixLanes;
+    if (i < n) {
+      const U u = First ? R::to(static_cast<const T*>(in)[i]) : static_cast<const U*>(in)[i];
+      atomicAdd(&h[unsigned(u >> shift) & 255u], 1u);
+    }
+  }
+  __syncthreads();
+  hist[int64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable placement of one tile's keys at their digit's offsets.  The keys
+// are first ranked into a digit-sorted copy of the tile in LDS, then written
+// out in LDS order, so each digit's keys of the tile leave as one contiguous
+// run (coalesced writes instead of one scattered word per lane).
+template <class T, bool First, bool Last>
+__global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out, const int64_t n, const int shift,
+                                                       const uint32_t* offsets, const uint32_t* hist,
+                                                       const int64_t ntiles) {
+  using R = RadixKey<T>;
+  using U = typename R::U;
+  constexpr int NW = kRadixLanes / 64;
+  __shared__ uint32_t lstart[256];  // first slot of each digit in the tile
+  __shared__ uint32_t lrun[256];    // next free slot of each digit
+  __shared__ uint32_t gbase[256];   // global position of the tile's first key of each digit
+  __shared__ uint32_t wsum[NW];
+  __shared__ uint32_t wpos[NW][256];
+  __shared__ U tile[kRadixTile];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  {  // tile-local exclusive scan of the digit counts
+    const uint32_t c   = hist[int64_t(t) * ntiles + blockIdx.x];
+    const auto plus    = [](uint32_t& a, const uint32_t& b) { a += b; };
+    const uint32_t inc = wave::inclusive_scan(c, 0u, plus);
+    if (l == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+    lstart[t] = before + inc - c;
+    lrun[t]   = before + inc - c;
+    gbase[t]  = offsets[int64_t(t) * ntiles + blockIdx.x];
+  }
+  const uint64_t lt    = (uint64_t(1) << l) - 1;
+  const int64_t tfirst = int64_t(blockIdx.x) * kRadixTile;
+  for (int r = 0; r < kRadixItems; ++r) {
+    const int64_t i  = tfirst + int64_t(r) * kRadixLanes + t;
+    const bool valid = i < n;
+    U u              = 0;
+    if (valid) u = First ? R::to(static_cast<const T*>(in)[i]) : static_cast<const U*>(in)[i];
+    const unsigned d = unsigned(u >> shift) & 255u;
+    // lanes of this wavefront holding the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = __popcll(peers & lt);
+#pragma unroll
+    for (int k = 0; k < NW; ++k) wpos[k][t] = 0;
+    __syncthreads();
+    if (valid && rank == 0) wpos[w][d] = __popcll(peers);
+    __syncthreads();
+    {  // counts -> tile slots: digit t, wavefronts in order
+      uint32_t run = lrun[t];
+#pragma unroll
+      for (int k = 0; k < NW; ++k) {
+        const uint32_t c = wpos[k][t];
+        wpos[k][t]       = run;
+        run += c;
+      }
+      lrun[t] = run;
+    }
+    __syncthreads();
+    if (valid) tile[wpos[w][d] + rank] = u;
+    __syncthreads();  // wpos is cleared by the next round
+  }
+  const int64_t left = n - tfirst;
+  const int count    = left < kRadixTile ? int(left) : kRadixTile;
+  for (int j = t; j < count; j += kRadixLanes) {
+    const U u          = tile[j];
+    const unsigned d   = unsigned(u >> shift) & 255u;
+    const uint32_t pos = gbase[d] + (uint32_t(j) - lstart[d]);
+    if (Last) static_cast<T*>(out)[pos] = R::from(u);
+    else static_cast<U*>(out)[pos] = u;
+  }
+}
+
+struct ExclusiveScanU32 {
+  const uint32_t* in;
+  uint32_t* out;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, uint32_t& upd, const bool final) const {
+    const uint32_t x = in[i];
+    if (final) out[i] = upd;
+    upd += x;
+  }
+};
+
+// Ascending LSD radix sort of n keys in place (device memory), on exec.
+template <class T>
+void radix_sort(const MI355X& exec, T* keys, int64_t n) {
+  if (n < 2) return;
+  if (n > int64_t(0x7fffffff)) throw_runtime_exception("Kokkos::sort: more than 2^31 - 1 keys");
+  using U              = typename RadixKey<T>::U;
+  const int64_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+  const int passes     = int(sizeof(T));
+  View<U*, MI355XSpace> tmp(view_alloc(WithoutInitializing, std::string("Kokkos::sort::radix_tmp")), size_t(n));
+  View<uint32_t*, MI355XSpace> hist(view_alloc(WithoutInitializing, std::string("Kokkos::sort::radix_hist")),
+                                    size_t(256 * ntiles));
+  View<uint32_t*, MI355XSpace> offs(view_alloc(WithoutInitializing, std::string("Kokkos::sort::radix_offsets")),
+                                    size_t(256 * ntiles));
+  void* buf[2] = {static_cast<void*>(keys), static_cast<void*>(tmp.data())};
+  uint32_t* hp = hist.data();
+  uint32_t* op = offs.data();
+  for (int p = 0; p < passes; ++p) {
+    const void* in = buf[p & 1];
+    void* out      = buf[(p + 1) & 1];
+    int shift      = 8 * p;
+    const bool first = p == 0, last = p == passes - 1;
+    const void* kh = first ? reinterpret_cast<const void*>(&k_radix_hist<T, true>)
+                           : reinterpret_cast<const void*>(&k_radix_hist<T, false>);
+    int64_t nt = ntiles;
+    mx_launch_kernel(kh, unsigned(ntiles), kRadixLanes, 0, exec.stream(), in, n, shift, hp, nt);
+    parallel_scan("Kokkos::sort::radix_offsets", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, 256 * ntiles),
+                  ExclusiveScanU32{hp, op});
+    const void* ks = first ? (last ? reinterpret_cast<const void*>(&k_radix_scatter<T, true, true>)
+                                   : reinterpret_cast<const void*>(&k_radix_scatter<T, true, false>))
+                           : (last ? reinterpret_cast<const void*>(&k_radix_scatter<T, false, true>)
+                                   : reinterpret_cast<const void*>(&k_radix_scatter<T, false, false>));
+    const uint32_t* cop = op;
+    const uint32_t* chp = hp;
+    mx_launch_kernel(ks, unsigned(ntiles), kRadixLanes, 0, exec.stream(), in, out, n, shift, cop, chp, nt);
+  }
+  // an even number of passes leaves the keys in place; keep tmp and the
+  // tables alive until the stream has run them
+  exec.fence("Kokkos::sort: radix passes");
+}
+
+template <class V>
+constexpr bool radix_sortable() {
+  using T = typename V::non_const_value_type;
+  return V::rank == 1 && std::is_arithmetic<T>::value && !std::is_same<T, bool>::value &&
+         (sizeof(T) == 4 || sizeof(T) == 8) && exec_can_access<MI355X, typename V::memory_space>::value;
+}
+
+template <class V, class Comp>
+void host_sort(const V& view, size_t begin, size_t end, Comp comp) {
+  using T = typename V::non_const_value_type;
+  const size_t n = end - begin;
+  std::vector<T> h(n);
+  if constexpr (exec_can_access<Serial, typename V::memory_space>::value) {
+    for (size_t i = 0; i < n; ++i) h[i] = view(begin + i);
+    std::sort(h.begin(), h.end(), comp);
+    for (size_t i = 0; i < n; ++i) view(begin + i) = h[i];
+  } else {
+    static_assert(V::rank == 1, "Kokkos::sort sorts rank-1 Views");
+    Kokkos::fence("Kokkos::sort: host fallback");
+    if (view.span_is_contiguous()) {
+      KKMX_SAFE_CALL(mx_memcpy(h.data(), view.data() + begin, n * sizeof(T)));
+      std::sort(h.begin(), h.end(), comp);
+      KKMX_SAFE_CALL(mx_memcpy(view.data() + begin, h.data(), n * sizeof(T)));
+    } else {
+      auto m = create_mirror_view_and_copy(HostSpace(), view);
+      for (size_t i = 0; i < n; ++i) h[i] = m(begin + i);
+      std::sort(h.begin(), h.end(), comp);
+      for (size_t i = 0; i < n; ++i) m(begin + i) = h[i];
+      deep_copy(view, m);
+    }
+  }
+}
+
+}  // namespace Impl
+
+// ------------------------------------------------------------------ sort
+template <class ExecutionSpace, class DataType, class... Properties,
+          class = std::enable_if_t<is_execution_space<ExecutionSpace>::value>>
+void sort(const ExecutionSpace& exec, const View<DataType, Properties...>& view) {
+  using V = View<DataType, Properties...>;
+  using T = typename V::non_const_value_type;
+  static_assert(V::rank == 1, "Kokkos::sort sorts rank-1 Views");
+  if (view.extent(0) < 2) return;
+  if constexpr (Impl::radix_sortable<V>() && std::is_same<ExecutionSpace, MI355X>::value) {
+    if (view.span_is_contiguous()) {
+      Impl::radix_sort<T>(exec, view.data(), int64_t(view.extent(0)));
+      return;
+    }
+  }
+  exec.fence();
+  Impl::host_sort(view, 0, view.extent(0), std::less<T>());
+}
+template <class DataType, class... Properties>
+void sort(const View<DataType, Properties...>& view) {
+  Kokkos::fence("Kokkos::sort: before");
+  sort(MI355X(), view);
+  Kokkos::fence("Kokkos::sort: after");
+}
+template <class ExecutionSpace, class Comparator, class DataType, class... Properties,
+          class = std::enable_if_t<is_execution_space<ExecutionSpace>::value>>
+void sort(const ExecutionSpace& exec, const View<DataType, Properties...>& view, const Comparator& comp) {
+  exec.fence();
+  Impl::host_sort(view, 0, view.extent(0), comp);
+}
+template <class Comparator, class DataType, class... Properties,
+          class = std::enable_if_t<!is_execution_space<Comparator>::value>>
+void sort(const View<DataType, Properties...>& view, const Comparator& comp) {
+  Kokkos::fence("Kokkos::sort: before");
+  Impl::host_sort(view, 0, view.extent(0), comp);
+}
+template <class ExecutionSpace, class ViewType, class = std::enable_if_t<is_execution_space<ExecutionSpace>::value>>
+void sort(const ExecutionSpace& exec, ViewType view, size_t begin, size_t end) {
+  using T = typename ViewType::non_const_value_type;
+  if (end <= begin + 1) return;
+  if constexpr (Impl::radix_sortable<ViewType>() && std::is_same<ExecutionSpace, MI355X>::value) {
+    if (view.span_is_contiguous()) {
+      Impl::radix_sort<T>(exec, view.data() + begin, int64_t(end - begin));
+      return;
+    }
+  }
+  exec.fence();
+  Impl::host_sort(view, begin, end, std::less<T>());
+}
+template <class ViewType>
+void sort(ViewType view, size_t begin, size_t end) {
+  Kokkos::fence("Kokkos::sort: before");
+  sort(MI355X(), view, begin, end);
+  Kokkos::fence("Kokkos::sort: after");
+}
+
+}  // namespace Kokkos
+
+#endif
