@@ -13,12 +13,13 @@
 // Kokkos::sort of a contiguous rank-1 device View of 4- or 8-byte
 // arithmetic keys is an LSD radix sort written for gfx950: 8-bit digits,
 // one histogram pass and one scatter pass per digit over 4096-key tiles of
-// 256 lanes.  The scatter ranks keys with equal digits inside each
-// wavefront from eight ballots, combines the wavefronts through LDS into a
-// digit-sorted copy of the tile in LDS, and writes that copy out so each
-// digit's keys leave as one contiguous run, at offsets from a digit-major
-// exclusive scan of the per-tile histograms: every pass reads its keys
-// twice and writes them once, stably.  Signed and floating-point keys are mapped to unsigned order on the
+// 256 lanes, grouped into one segment of consecutive tiles per block (about
+// four blocks per CU).  The scatter ranks keys with equal digits inside
+// each wavefront from eight ballots, combines the wavefronts through LDS
+// into a digit-sorted copy of the tile in LDS, and writes that copy out so
+// each digit's keys leave as one contiguous run, at offsets from a
+// digit-major exclusive scan of the per-segment histograms: every pass
+// reads its keys twice and writes them once, stably.  Signed and floating-point keys are mapped to unsigned order on the
 // first pass and back on the last.  Other element types, comparators and
 // host Views sort on the host with std::sort (results identical to an
 // ascending sort).
@@ -28,6 +29,7 @@
 #include <Kokkos_Core.hpp>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace Kokkos {
@@ -320,102 +322,143 @@ constexpr int kRadixLanes = 256;
 constexpr int kRadixItems = 16;
 constexpr int kRadixTile  = kRadixLanes * kRadixItems;
 
-// Per-tile counts of digit (key >> shift) & 255, stored digit-major:
-// hist[d * ntiles + tile].
+// Each block owns a segment of `tpb` consecutive tiles of kRadixTile keys
+// (the last segment may be short).  k_radix_hist counts digit
+// (key >> shift) & 255 over its segment, in one LDS histogram per wavefront,
+// and stores the counts digit-major: hist[d * nblocks + block].
 template <class T, bool First>
 __global__ __launch_bounds__(256) void k_radix_hist(const void* in, const int64_t n, const int shift, uint32_t* hist,
-                                                    const int64_t ntiles) {
-  using R = RadixKey<T>;
-  using U = typename R::U;
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t base = int64_t(blockIdx.x) * kRadixTile + threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < kRadixItems; ++r) {
-    const int64_t i = base + int64_t(r) * kRadixLanes;
-    if (i < n) {
-      const U u = First ? R::to(static_cast<const T*>(in)[i]) : static_cast<const U*>(in)[i];
-      atomicAdd(&h[unsigned(u >> shift) & 255u], 1u);
-    }
-  }
-  __syncthreads();
-  hist[int64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// Stable placement of one tile's keys at their digit's offsets.  The keys
-// are first ranked into a digit-sorted copy of the tile in LDS, then written
-// out in LDS order, so each digit's keys of the tile leave as one contiguous
-// run (coalesced writes instead of one scattered word per lane).
-template <class T, bool First, bool Last>
-__global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out, const int64_t n, const int shift,
-                                                       const uint32_t* offsets, const uint32_t* hist,
-                                                       const int64_t ntiles) {
+                                                    const int64_t nblocks, const int64_t tpb) {
   using R = RadixKey<T>;
   using U = typename R::U;
   constexpr int NW = kRadixLanes / 64;
+  __shared__ uint32_t h[NW][256];
+  const int t = threadIdx.x, w = t >> 6;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) h[k][t] = 0;
+  __syncthreads();
+  const int64_t first = int64_t(blockIdx.x) * tpb * kRadixTile;
+  const int64_t last  = first + tpb * kRadixTile < n ? first + tpb * kRadixTile : n;
+  for (int64_t tb = first; tb < last; tb += kRadixTile) {
+    U u[kRadixItems];
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+      const int64_t i = tb + int64_t(r) * kRadixLanes + t;
+      u[r]            = 0;
+      if (i < last) u[r] = First ? R::to(static_cast<const T*>(in)[i]) : static_cast<const U*>(in)[i];
+    }
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r)
+      if (tb + int64_t(r) * kRadixLanes + t < last) atomicAdd(&h[w][unsigned(u[r] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k) c += h[k][t];
+  hist[int64_t(t) * nblocks + blockIdx.x] = c;
+}
+
+// Stable placement of a segment's keys at their digit's offsets, one tile at
+// a time.  Wavefront w ranks keys [w * 1024, (w + 1) * 1024) of the tile with
+// eight ballots per 64 keys and its own LDS digit counters (no block barrier
+// per round), the block turns the per-wavefront counts into slots of a
+// digit-sorted copy of the tile in LDS, and writes that copy out so each
+// digit's keys leave as one contiguous run.  gbase carries each digit's next
+// output position from tile to tile.
+template <class T, bool First, bool Last>
+__global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out, const int64_t n, const int shift,
+                                                       const uint32_t* offsets, const int64_t nblocks,
+                                                       const int64_t tpb) {
+  using R = RadixKey<T>;
+  using U = typename R::U;
+  constexpr int NW   = kRadixLanes / 64;
+  constexpr int WKEY = kRadixTile / NW;  // keys per wavefront per tile
+  static_assert(WKEY == 64 * kRadixItems, "each lane ranks kRadixItems keys per tile");
+  __shared__ uint32_t gbase[256];   // next output position of each digit
   __shared__ uint32_t lstart[256];  // first slot of each digit in the tile
-  __shared__ uint32_t lrun[256];    // next free slot of each digit
-  __shared__ uint32_t gbase[256];   // global position of the tile's first key of each digit
+  __shared__ uint32_t wcnt[NW][256];
   __shared__ uint32_t wsum[NW];
-  __shared__ uint32_t wpos[NW][256];
   __shared__ U tile[kRadixTile];
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
-  {  // tile-local exclusive scan of the digit counts
-    const uint32_t c   = hist[int64_t(t) * ntiles + blockIdx.x];
-    const auto plus    = [](uint32_t& a, const uint32_t& b) { a += b; };
-    const uint32_t inc = wave::inclusive_scan(c, 0u, plus);
-    if (l == 63) wsum[w] = inc;
-    __syncthreads();
-    uint32_t before = 0;
-    for (int k = 0; k < w; ++k) before += wsum[k];
-    lstart[t] = before + inc - c;
-    lrun[t]   = before + inc - c;
-    gbase[t]  = offsets[int64_t(t) * ntiles + blockIdx.x];
-  }
-  const uint64_t lt    = (uint64_t(1) << l) - 1;
-  const int64_t tfirst = int64_t(blockIdx.x) * kRadixTile;
-  for (int r = 0; r < kRadixItems; ++r) {
-    const int64_t i  = tfirst + int64_t(r) * kRadixLanes + t;
-    const bool valid = i < n;
-    U u              = 0;
-    if (valid) u = First ? R::to(static_cast<const T*>(in)[i]) : static_cast<const U*>(in)[i];
-    const unsigned d = unsigned(u >> shift) & 255u;
-    // lanes of this wavefront holding the same digit
-    uint64_t peers = __ballot(valid);
+  const uint64_t lt = (uint64_t(1) << l) - 1;
+  gbase[t]          = offsets[int64_t(t) * nblocks + blockIdx.x];
+  const int64_t seg = int64_t(blockIdx.x) * tpb * kRadixTile;
+  for (int64_t k = 0; k < tpb; ++k) {
+    const int64_t tfirst = seg + k * kRadixTile;
+    if (tfirst >= n) break;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t m = __ballot((d >> b) & 1u);
-      peers &= ((d >> b) & 1u) ? m : ~m;
+    for (int q = 0; q < NW; ++q) wcnt[q][t] = 0;
+    __syncthreads();  // counters cleared; the previous tile is fully written out
+    const int64_t wfirst = tfirst + int64_t(w) * WKEY;
+    U u[kRadixItems];
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+      const int64_t i = wfirst + int64_t(r) * 64 + l;
+      u[r]            = 0;
+      if (i < n) u[r] = First ? R::to(static_cast<const T*>(in)[i]) : static_cast<const U*>(in)[i];
     }
-    const uint32_t rank = __popcll(peers & lt);
+    uint32_t rk[kRadixItems];
 #pragma unroll
-    for (int k = 0; k < NW; ++k) wpos[k][t] = 0;
-    __syncthreads();
-    if (valid && rank == 0) wpos[w][d] = __popcll(peers);
-    __syncthreads();
-    {  // counts -> tile slots: digit t, wavefronts in order
-      uint32_t run = lrun[t];
+    for (int r = 0; r < kRadixItems; ++r) {
+      const bool valid = wfirst + int64_t(r) * 64 + l < n;
+      const unsigned d = unsigned(u[r] >> shift) & 255u;
+      // lanes differing from this lane in some digit bit; the ballot mask of
+      // a bit doubles as the per-lane select between it and its complement
+      const uint64_t vm = __builtin_amdgcn_ballot_w64(valid);
+      uint32_t dlo = ~uint32_t(vm), dhi = ~uint32_t(vm >> 32);
 #pragma unroll
-      for (int k = 0; k < NW; ++k) {
-        const uint32_t c = wpos[k][t];
-        wpos[k][t]       = run;
-        run += c;
+      for (int b = 0; b < 8; ++b) {
+        const int32_t xb = int32_t(d << (31 - b)) >> 31;  // 0 or all ones
+        const uint64_t m = __builtin_amdgcn_ballot_w64(xb != 0);
+        dlo |= uint32_t(m) ^ uint32_t(xb);
+        dhi |= uint32_t(m >> 32) ^ uint32_t(xb);
       }
-      lrun[t] = run;
+      const uint64_t peers = ~((uint64_t(dhi) << 32) | dlo);
+      const uint32_t below = __popcll(peers & lt);
+      const uint32_t prior = wcnt[w][d];
+      rk[r]                = prior + below;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (valid && below == 0) wcnt[w][d] = prior + uint32_t(__popcll(peers));
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    __syncthreads();
-    if (valid) tile[wpos[w][d] + rank] = u;
-    __syncthreads();  // wpos is cleared by the next round
-  }
-  const int64_t left = n - tfirst;
-  const int count    = left < kRadixTile ? int(left) : kRadixTile;
-  for (int j = t; j < count; j += kRadixLanes) {
-    const U u          = tile[j];
-    const unsigned d   = unsigned(u >> shift) & 255u;
-    const uint32_t pos = gbase[d] + (uint32_t(j) - lstart[d]);
-    if (Last) static_cast<T*>(out)[pos] = R::from(u);
-    else static_cast<U*>(out)[pos] = u;
+    __syncthreads();  // per-wavefront counts complete
+    uint32_t total = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      const uint32_t c = wcnt[q][t];
+      wcnt[q][t]       = total;
+      total += c;
+    }
+    {
+      const auto plus    = [](uint32_t& a, const uint32_t& b) { a += b; };
+      const uint32_t inc = wave::inclusive_scan(total, 0u, plus);
+      if (l == 63) wsum[w] = inc;
+      __syncthreads();
+      uint32_t before = 0;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) before += q < w ? wsum[q] : 0u;
+      lstart[t] = before + inc - total;
+    }
+    __syncthreads();  // lstart and per-wavefront prefixes visible
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+      if (wfirst + int64_t(r) * 64 + l < n) {
+        const unsigned d = unsigned(u[r] >> shift) & 255u;
+        tile[lstart[d] + wcnt[w][d] + rk[r]] = u[r];
+      }
+    }
+    __syncthreads();  // tile sorted by digit
+    const int64_t left = n - tfirst;
+    const int count    = left < kRadixTile ? int(left) : kRadixTile;
+    for (int j = t; j < count; j += kRadixLanes) {
+      const U v          = tile[j];
+      const unsigned d   = unsigned(v >> shift) & 255u;
+      const uint32_t pos = gbase[d] + (uint32_t(j) - lstart[d]);
+      if (Last) static_cast<T*>(out)[pos] = R::from(v);
+      else static_cast<U*>(out)[pos] = v;
+    }
+    __syncthreads();  // every read of gbase for this tile is done
+    gbase[t] += total;
   }
 }
 
@@ -429,6 +472,32 @@ struct ExclusiveScanU32 {
   }
 };
 
+// Device workspace of the radix sort (ping-pong keys and the two digit
+// tables), kept between calls and grown on demand: a fresh allocation per
+// call costs more than the smaller sorts themselves and slows the kernels
+// right after it.  Calls hold the mutex until their passes have run, so
+// sorts on different instances take turns; Kokkos::finalize frees it.
+struct RadixWorkspace {
+  std::mutex mtx;
+  void* buf    = nullptr;
+  size_t bytes = 0;
+  bool hooked  = false;
+};
+
+inline RadixWorkspace& radix_workspace() {
+  static RadixWorkspace w;
+  return w;
+}
+
+inline void radix_workspace_release() {
+  RadixWorkspace& w = radix_workspace();
+  std::lock_guard<std::mutex> lock(w.mtx);
+  if (w.buf) KKMX_SAFE_CALL(mx_free(w.buf, nullptr));
+  w.buf    = nullptr;
+  w.bytes  = 0;
+  w.hooked = false;
+}
+
 // Ascending LSD radix sort of n keys in place (device memory), on exec.
 template <class T>
 void radix_sort(const MI355X& exec, T* keys, int64_t n) {
@@ -436,36 +505,51 @@ void radix_sort(const MI355X& exec, T* keys, int64_t n) {
   if (n > int64_t(0x7fffffff)) throw_runtime_exception("Kokkos::sort: more than 2^31 - 1 keys");
   using U              = typename RadixKey<T>::U;
   const int64_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+  // segments of whole tiles, about four blocks per CU
+  const int cus        = MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256;
+  const int64_t want   = int64_t(cus) * 4;
+  const int64_t tpb    = (ntiles + want - 1) / want;
+  const int64_t nblk   = (ntiles + tpb - 1) / tpb;
   const int passes     = int(sizeof(T));
-  View<U*, MI355XSpace> tmp(view_alloc(WithoutInitializing, std::string("Kokkos::sort::radix_tmp")), size_t(n));
-  View<uint32_t*, MI355XSpace> hist(view_alloc(WithoutInitializing, std::string("Kokkos::sort::radix_hist")),
-                                    size_t(256 * ntiles));
-  View<uint32_t*, MI355XSpace> offs(view_alloc(WithoutInitializing, std::string("Kokkos::sort::radix_offsets")),
-                                    size_t(256 * ntiles));
-  void* buf[2] = {static_cast<void*>(keys), static_cast<void*>(tmp.data())};
-  uint32_t* hp = hist.data();
-  uint32_t* op = offs.data();
+  const size_t key_bytes   = (size_t(n) * sizeof(U) + 255) & ~size_t(255);
+  const size_t table_bytes = size_t(256 * nblk) * sizeof(uint32_t);
+  RadixWorkspace& ws       = radix_workspace();
+  std::lock_guard<std::mutex> lock(ws.mtx);
+  if (ws.bytes < key_bytes + 2 * table_bytes) {
+    if (ws.buf) KKMX_SAFE_CALL(mx_free(ws.buf, nullptr));
+    ws.buf   = nullptr;
+    ws.bytes = 0;
+    KKMX_SAFE_CALL(mx_malloc(&ws.buf, key_bytes + 2 * table_bytes, nullptr));
+    ws.bytes = key_bytes + 2 * table_bytes;
+  }
+  if (!ws.hooked) {
+    push_finalize_hook(&radix_workspace_release);
+    ws.hooked = true;
+  }
+  char* base   = static_cast<char*>(ws.buf);
+  void* buf[2] = {static_cast<void*>(keys), static_cast<void*>(base)};
+  uint32_t* hp = reinterpret_cast<uint32_t*>(base + key_bytes);
+  uint32_t* op = reinterpret_cast<uint32_t*>(base + key_bytes + table_bytes);
   for (int p = 0; p < passes; ++p) {
-    const void* in = buf[p & 1];
-    void* out      = buf[(p + 1) & 1];
-    int shift      = 8 * p;
+    const void* in   = buf[p & 1];
+    void* out        = buf[(p + 1) & 1];
+    int shift        = 8 * p;
     const bool first = p == 0, last = p == passes - 1;
-    const void* kh = first ? reinterpret_cast<const void*>(&k_radix_hist<T, true>)
-                           : reinterpret_cast<const void*>(&k_radix_hist<T, false>);
-    int64_t nt = ntiles;
-    mx_launch_kernel(kh, unsigned(ntiles), kRadixLanes, 0, exec.stream(), in, n, shift, hp, nt);
-    parallel_scan("Kokkos::sort::radix_offsets", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, 256 * ntiles),
+    const void* kh   = first ? reinterpret_cast<const void*>(&k_radix_hist<T, true>)
+                             : reinterpret_cast<const void*>(&k_radix_hist<T, false>);
+    int64_t nb = nblk, tp = tpb;
+    mx_launch_kernel(kh, unsigned(nblk), kRadixLanes, 0, exec.stream(), in, n, shift, hp, nb, tp);
+    parallel_scan("Kokkos::sort::radix_offsets", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, 256 * nblk),
                   ExclusiveScanU32{hp, op});
     const void* ks = first ? (last ? reinterpret_cast<const void*>(&k_radix_scatter<T, true, true>)
                                    : reinterpret_cast<const void*>(&k_radix_scatter<T, true, false>))
                            : (last ? reinterpret_cast<const void*>(&k_radix_scatter<T, false, true>)
                                    : reinterpret_cast<const void*>(&k_radix_scatter<T, false, false>));
     const uint32_t* cop = op;
-    const uint32_t* chp = hp;
-    mx_launch_kernel(ks, unsigned(ntiles), kRadixLanes, 0, exec.stream(), in, out, n, shift, cop, chp, nt);
+    mx_launch_kernel(ks, unsigned(nblk), kRadixLanes, 0, exec.stream(), in, out, n, shift, cop, nb, tp);
   }
-  // an even number of passes leaves the keys in place; keep tmp and the
-  // tables alive until the stream has run them
+  // an even number of passes leaves the keys in place; the workspace is
+  // free for the next call once the stream has run them
   exec.fence("Kokkos::sort: radix passes");
 }
 
