@@ -19,10 +19,14 @@
 // into a digit-sorted copy of the tile in LDS, and writes that copy out so
 // each digit's keys leave as one contiguous run, at offsets from a
 // digit-major exclusive scan of the per-segment histograms: every pass
-// reads its keys twice and writes them once, stably.  Signed and floating-point keys are mapped to unsigned order on the
-// first pass and back on the last.  Other element types, comparators and
-// host Views sort on the host with std::sort (results identical to an
-// ascending sort).
+// reads its keys twice and writes them once, stably.  Signed and
+// floating-point keys are mapped to unsigned order on the first pass and
+// back on the last.  As in the reference's HIP path (thrust::sort, a radix
+// sort for arithmetic keys: algorithms/src/sorting/impl/
+// Kokkos_SortImpl.hpp:222-240), -0.0 and +0.0 compare equal and keep their
+// input order, and NaNs go below -inf (sign bit set) or above +inf.  Other
+// element types, comparators and host Views sort on the host with
+// std::sort (results identical to an ascending sort).
 #ifndef KKMX_SORT_HPP
 #define KKMX_SORT_HPP
 
@@ -309,6 +313,13 @@ struct RadixKey {
     else if constexpr (std::is_signed<T>::value) return u ^ top;
     else return u;
   }
+  // Digit of an encoded key.  -0.0 (encoded ~top) takes the digits of +0.0
+  // (encoded top), so the two compare equal and keep their input order, as
+  // under operator<; the key itself keeps its bits.
+  KOKKOS_FORCEINLINE_FUNCTION static unsigned digit(U u, int shift) {
+    if constexpr (std::is_floating_point<T>::value) u = u == U(~top) ? top : u;
+    return unsigned(u >> shift) & 255u;
+  }
   KOKKOS_FORCEINLINE_FUNCTION static T from(U u) {
     if constexpr (std::is_floating_point<T>::value) u = (u & top) ? (u ^ top) : ~u;
     else if constexpr (std::is_signed<T>::value) u ^= top;
@@ -324,7 +335,7 @@ constexpr int kRadixTile  = kRadixLanes * kRadixItems;
 
 // Each block owns a segment of `tpb` consecutive tiles of kRadixTile keys
 // (the last segment may be short).  k_radix_hist counts digit
-// (key >> shift) & 255 over its segment, in one LDS histogram per wavefront,
+// R::digit(key, shift) over its segment, in one LDS histogram per wavefront,
 // and stores the counts digit-major: hist[d * nblocks + block].
 template <class T, bool First>
 __global__ __launch_bounds__(256) void k_radix_hist(const void* in, const int64_t n, const int shift, uint32_t* hist,
@@ -349,7 +360,7 @@ __global__ __launch_bounds__(256) void k_radix_hist(const void* in, const int64_
     }
 #pragma unroll
     for (int r = 0; r < kRadixItems; ++r)
-      if (tb + int64_t(r) * kRadixLanes + t < last) atomicAdd(&h[w][unsigned(u[r] >> shift) & 255u], 1u);
+      if (tb + int64_t(r) * kRadixLanes + t < last) atomicAdd(&h[w][R::digit(u[r], shift)], 1u);
   }
   __syncthreads();
   uint32_t c = 0;
@@ -401,7 +412,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out
 #pragma unroll
     for (int r = 0; r < kRadixItems; ++r) {
       const bool valid = wfirst + int64_t(r) * 64 + l < n;
-      const unsigned d = unsigned(u[r] >> shift) & 255u;
+      const unsigned d = R::digit(u[r], shift);
       // lanes differing from this lane in some digit bit; the ballot mask of
       // a bit doubles as the per-lane select between it and its complement
       const uint64_t vm = __builtin_amdgcn_ballot_w64(valid);
@@ -443,7 +454,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out
 #pragma unroll
     for (int r = 0; r < kRadixItems; ++r) {
       if (wfirst + int64_t(r) * 64 + l < n) {
-        const unsigned d = unsigned(u[r] >> shift) & 255u;
+        const unsigned d = R::digit(u[r], shift);
         tile[lstart[d] + wcnt[w][d] + rk[r]] = u[r];
       }
     }
@@ -452,7 +463,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out
     const int count    = left < kRadixTile ? int(left) : kRadixTile;
     for (int j = t; j < count; j += kRadixLanes) {
       const U v          = tile[j];
-      const unsigned d   = unsigned(v >> shift) & 255u;
+      const unsigned d   = R::digit(v, shift);
       const uint32_t pos = gbase[d] + (uint32_t(j) - lstart[d]);
       if (Last) static_cast<T*>(out)[pos] = R::from(v);
       else static_cast<U*>(out)[pos] = v;

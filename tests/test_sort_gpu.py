@@ -23,6 +23,10 @@ def keys_for(oracle, code, n, seed):
         x = oracle.fill_uniform_f64(n, seed, -1e6, 1e6)
         x[::7] = np.round(x[::7])  # ties
         x[::11] *= -1e-30          # tiny magnitudes of both signs
+        x[3::13] = 0.0             # signed zeros, interleaved: they compare
+        x[5::26] = -0.0            # equal and keep their input order
+        x[::97] = np.inf
+        x[1::97] = -np.inf
         return x.astype(dt)
     r = oracle.fill_uniform_i64(n, seed, -(1 << 62), 1 << 62)
     if code in (0, 4):
@@ -40,7 +44,9 @@ def test_sort_matches_numpy(paths, oracle, code, n):
     D = K.DeviceBuffer.from_array(x)
     run(P, P.kkmx_sort(D.ptr, n, code))
     got = D.download(TYPES[code], n)
-    assert np.array_equal(got, np.sort(x, kind="stable"))
+    want = np.sort(x, kind="stable")
+    bits = {4: np.uint32, 8: np.uint64}[x.dtype.itemsize]
+    assert np.array_equal(got.view(bits), want.view(bits))  # bitwise: -0.0 != +0.0 here
 
 
 def test_sort_large_int32(paths, oracle):
