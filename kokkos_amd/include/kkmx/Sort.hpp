@@ -374,8 +374,8 @@ __global__ __launch_bounds__(256) void k_radix_hist(const void* in, const int64_
 // eight ballots per 64 keys and its own LDS digit counters (no block barrier
 // per round), the block turns the per-wavefront counts into slots of a
 // digit-sorted copy of the tile in LDS, and writes that copy out so each
-// digit's keys leave as one contiguous run.  gbase carries each digit's next
-// output position from tile to tile.
+// digit's keys leave as one contiguous run.  Lane t carries digit t's next
+// output position from tile to tile in a register.
 template <class T, bool First, bool Last>
 __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out, const int64_t n, const int shift,
                                                        const uint32_t* offsets, const int64_t nblocks,
@@ -385,14 +385,13 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out
   constexpr int NW   = kRadixLanes / 64;
   constexpr int WKEY = kRadixTile / NW;  // keys per wavefront per tile
   static_assert(WKEY == 64 * kRadixItems, "each lane ranks kRadixItems keys per tile");
-  __shared__ uint32_t gbase[256];   // next output position of each digit
-  __shared__ uint32_t lstart[256];  // first slot of each digit in the tile
+  __shared__ uint32_t gshift[256];  // output position of tile slot j of digit d: gshift[d] + j
   __shared__ uint32_t wcnt[NW][256];
   __shared__ uint32_t wsum[NW];
   __shared__ U tile[kRadixTile];
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const uint64_t lt = (uint64_t(1) << l) - 1;
-  gbase[t]          = offsets[int64_t(t) * nblocks + blockIdx.x];
+  uint32_t gnext    = offsets[int64_t(t) * nblocks + blockIdx.x];  // digit t's next output position
   const int64_t seg = int64_t(blockIdx.x) * tpb * kRadixTile;
   for (int64_t k = 0; k < tpb; ++k) {
     const int64_t tfirst = seg + k * kRadixTile;
@@ -448,28 +447,31 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out
       uint32_t before = 0;
 #pragma unroll
       for (int q = 0; q < NW; ++q) before += q < w ? wsum[q] : 0u;
-      lstart[t] = before + inc - total;
+      // digit t's first slot in the tile; fold it into the per-wavefront
+      // prefixes and into the output shift, so placement and write-out
+      // read one LDS word per key each
+      const uint32_t first = before + inc - total;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) wcnt[q][t] += first;
+      gshift[t] = gnext - first;
+      gnext += total;
     }
-    __syncthreads();  // lstart and per-wavefront prefixes visible
+    __syncthreads();  // slot bases and output shifts visible
 #pragma unroll
     for (int r = 0; r < kRadixItems; ++r) {
-      if (wfirst + int64_t(r) * 64 + l < n) {
-        const unsigned d = R::digit(u[r], shift);
-        tile[lstart[d] + wcnt[w][d] + rk[r]] = u[r];
-      }
+      if (wfirst + int64_t(r) * 64 + l < n) tile[wcnt[w][R::digit(u[r], shift)] + rk[r]] = u[r];
     }
     __syncthreads();  // tile sorted by digit
     const int64_t left = n - tfirst;
     const int count    = left < kRadixTile ? int(left) : kRadixTile;
     for (int j = t; j < count; j += kRadixLanes) {
       const U v          = tile[j];
-      const unsigned d   = R::digit(v, shift);
-      const uint32_t pos = gbase[d] + (uint32_t(j) - lstart[d]);
+      const uint32_t pos = gshift[R::digit(v, shift)] + uint32_t(j);
       if (Last) static_cast<T*>(out)[pos] = R::from(v);
       else static_cast<U*>(out)[pos] = v;
     }
-    __syncthreads();  // every read of gbase for this tile is done
-    gbase[t] += total;
+    // the next tile writes tile, wcnt and gshift only after its first
+    // barrier, which every lane reaches after this write-out
   }
 }
 
