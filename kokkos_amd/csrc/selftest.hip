@@ -15,6 +15,7 @@
 #include <functional>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -1772,6 +1773,133 @@ KKMX_TEST(scatter_view_duplicated_replicas) {
   KE::contribute(bins, sv);
   Kokkos::deep_copy(h, bins);
   CHECK_EQ(h(74), int64_t(2 * n) + 5);
+}
+
+// =========================================================================
+// Dispatch from two host threads onto one instance
+// (core/unit_test/TestExecSpaceThreadSafety.hpp:56-380).  Kernels submitted
+// to the same instance run one after another in submission order, so a
+// kernel that counts a shared device counter from 0 to N always reads N
+// back; if two ran at once, one would see too many increments.
+// =========================================================================
+constexpr int kTsN = 100000;  // increments per kernel
+constexpr int kTsM = 10;      // kernels per host thread
+
+struct TsCounter {
+  Kokkos::View<int, Exec> view;
+  KOKKOS_FUNCTION bool clash() const {
+    Kokkos::atomic_store(view.data(), 0);
+    for (int i = 0; i < kTsN; ++i) Kokkos::atomic_inc(view.data());
+    return Kokkos::atomic_load(view.data()) != kTsN;
+  }
+};
+struct TsRangeFor {
+  TsCounter c;
+  Kokkos::View<int, Exec> error;
+  KOKKOS_FUNCTION void operator()(int) const {
+    if (c.clash()) Kokkos::atomic_store(error.data(), 1);
+  }
+};
+struct TsMDFor {
+  TsCounter c;
+  Kokkos::View<int, Exec> error;
+  KOKKOS_FUNCTION void operator()(int, int) const {
+    if (c.clash()) Kokkos::atomic_store(error.data(), 1);
+  }
+};
+struct TsTeamFor {
+  TsCounter c;
+  Kokkos::View<int, Exec> error;
+  KOKKOS_FUNCTION void operator()(const Kokkos::TeamPolicy<Exec>::member_type& m) const {
+    Kokkos::single(Kokkos::PerTeam(m), [&]() {
+      if (c.clash()) Kokkos::atomic_store(error.data(), 1);
+    });
+  }
+};
+struct TsRangeReduce {
+  using value_type = int;
+  TsCounter c;
+  KOKKOS_FUNCTION void operator()(int, int& update) const {
+    if (c.clash()) ++update;
+  }
+};
+struct TsMDReduce {
+  using value_type = int;
+  TsCounter c;
+  KOKKOS_FUNCTION void operator()(int, int, int& update) const {
+    if (c.clash()) ++update;
+  }
+};
+struct TsTeamReduce {
+  using value_type = int;
+  TsCounter c;
+  KOKKOS_FUNCTION void operator()(const Kokkos::TeamPolicy<Exec>::member_type& m, int& update) const {
+    Kokkos::single(Kokkos::PerTeam(m), [&]() {
+      if (c.clash()) ++update;
+    });
+  }
+};
+
+// Runs body(exec) kTsM times on each of two host threads, both on the
+// default instance, then checks the error flag.
+template <class Body>
+void two_threads_one_instance(const Body& body, const Kokkos::View<int, Exec>& error) {
+  auto work = [&]() {
+    Exec exec;
+    for (int j = 0; j < kTsM; ++j) body(exec);
+    exec.fence();
+  };
+  std::thread t1(work), t2(work);
+  t1.join();
+  t2.join();
+  CHECK_EQ(read0(Exec(), error), 0);
+}
+
+KKMX_TEST(thread_safety_range_for) {
+  Kokkos::View<int, Exec> view("view"), error("error");
+  two_threads_one_instance(
+      [&](const Exec& ex) { Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(ex, 0, 1), TsRangeFor{{view}, error}); },
+      error);
+}
+
+KKMX_TEST(thread_safety_mdrange_for) {
+  Kokkos::View<int, Exec> view("view"), error("error");
+  two_threads_one_instance(
+      [&](const Exec& ex) {
+        Kokkos::parallel_for(Kokkos::MDRangePolicy<Exec, Kokkos::Rank<2>>(ex, {0, 0}, {1, 1}), TsMDFor{{view}, error});
+      },
+      error);
+}
+
+KKMX_TEST(thread_safety_team_for) {
+  Kokkos::View<int, Exec> view("view"), error("error");
+  two_threads_one_instance(
+      [&](const Exec& ex) { Kokkos::parallel_for(Kokkos::TeamPolicy<Exec>(ex, 1, 1, 1), TsTeamFor{{view}, error}); },
+      error);
+}
+
+KKMX_TEST(thread_safety_range_reduce) {
+  Kokkos::View<int, Exec> view("view"), error("error");
+  two_threads_one_instance(
+      [&](const Exec& ex) { Kokkos::parallel_reduce(Kokkos::RangePolicy<Exec>(ex, 0, 1), TsRangeReduce{{view}}, error); },
+      error);
+}
+
+KKMX_TEST(thread_safety_mdrange_reduce) {
+  Kokkos::View<int, Exec> view("view"), error("error");
+  two_threads_one_instance(
+      [&](const Exec& ex) {
+        Kokkos::parallel_reduce(Kokkos::MDRangePolicy<Exec, Kokkos::Rank<2>>(ex, {0, 0}, {1, 1}), TsMDReduce{{view}},
+                                error);
+      },
+      error);
+}
+
+KKMX_TEST(thread_safety_team_reduce) {
+  Kokkos::View<int, Exec> view("view"), error("error");
+  two_threads_one_instance(
+      [&](const Exec& ex) { Kokkos::parallel_reduce(Kokkos::TeamPolicy<Exec>(ex, 1, 1, 1), TsTeamReduce{{view}}, error); },
+      error);
 }
 
 }  // namespace
