@@ -241,8 +241,9 @@ def leg_std_algorithms(K, P, timer):
 
 
 def leg_sort(K, P, timer):
-    """Kokkos::sort (radix) of 2^28 int32 and 2^27 float64 keys, and BinSort
-    of 2^24 float64 keys over 2^20 bins with the within-bin sort."""
+    """Kokkos::sort (radix) of 2^28 int32 and 2^27 float64 keys,
+    sort_by_key of 2^28 (int32, uint32) pairs, and BinSort of 2^24 float64
+    keys over 2^20 bins with the within-bin sort."""
     chk = lambda rc: K.check(rc, P)  # noqa: E731
     n32, n64 = 1 << 28, 1 << 27
     A = K.DeviceBuffer(n32 * 4)
@@ -255,7 +256,16 @@ def leg_sort(K, P, timer):
         timer.start()
         chk(P.kkmx_sort(A.ptr, n32, 0))
         t32.append(timer.stop())
-    del A
+    V = K.DeviceBuffer(n32 * 4)
+    tkv = []
+    for _ in range(3):
+        fill32()
+        chk(P.kkmx_fill_uniform_i32(V.ptr, n32, 96, 0, 1 << 30))
+        K.fence()
+        timer.start()
+        chk(P.kkmx_sort_by_key(A.ptr, V.ptr, n32, 0, 4, 0))
+        tkv.append(timer.stop())
+    del A, V
     B = K.DeviceBuffer(n64 * 8)
     t64 = []
     for _ in range(3):
@@ -274,6 +284,7 @@ def leg_sort(K, P, timer):
     del Kd, Vd
     return {
         "sort_int32_2^28": {"ms": round(min(t32), 3), "keys_per_s": round(n32 / min(t32) * 1e3, 1)},
+        "sort_by_key_int32_u32_2^28": {"ms": round(min(tkv), 3), "pairs_per_s": round(n32 / min(tkv) * 1e3, 1)},
         "sort_f64_2^27": {"ms": round(min(t64), 3), "keys_per_s": round(n64 / min(t64) * 1e3, 1)},
         "binsort_f64_2^24_keys_2^20_bins": {"ms": round(bs[0], 3), "keys_per_s": round(nb / bs[0] * 1e3, 1)},
     }

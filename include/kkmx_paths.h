@@ -87,6 +87,12 @@ int kkmx_std_transform_reduce_dot_f64(const double* a_dev, const double* b_dev, 
  *   sort_within_bins).create_permute_vector(), then sort(values); perm_out
  *   (n ints, may be null) receives the permutation vector. ---- */
 int kkmx_sort(void* keys_dev, int64_t n, int type);
+/* kkmx_sort_by_key: Kokkos::Experimental::sort_by_key(exec, keys, values)
+ *   (Kokkos_SortByKeyPublicAPI.hpp:27-64); keys as kkmx_sort's type codes,
+ *   values are n words of value_bytes (4 or 8) moved with their keys.
+ *   descending = 1 passes a greater-than comparator (the comparator
+ *   overload, :67-108). Equal keys keep their input order. */
+int kkmx_sort_by_key(void* keys_dev, void* values_dev, int64_t n, int key_type, int value_bytes, int descending);
 int kkmx_binsort_f64(const double* keys_dev, int64_t* values_dev, int64_t n, int nbins, double kmin, double kmax,
                      int sort_within_bins, int32_t* perm_out_dev);
 

@@ -90,6 +90,7 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_scatter_view_f64", [vp, vp, i64, vp, i32, C.c_int])
         _sig(lib, "kkmx_scatter_view_histogram", [vp, i64, vp, i32, C.c_int])
         _sig(lib, "kkmx_sort", [vp, i64, C.c_int])
+        _sig(lib, "kkmx_sort_by_key", [vp, vp, i64, C.c_int, C.c_int, C.c_int])
         _sig(lib, "kkmx_binsort_f64", [vp, vp, i64, C.c_int, f64, f64, C.c_int, vp])
         _sig(lib, "kkmx_std_inclusive_scan_i32", [vp, vp, i64])
         _sig(lib, "kkmx_std_exclusive_scan_i64", [vp, vp, i64, i64])

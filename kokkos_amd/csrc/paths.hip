@@ -444,6 +444,27 @@ void scatter_view_f64(const double* vals, const int32_t* idx, int64_t n, double*
 }
 }  // namespace
 
+namespace {
+// Kokkos::Experimental::sort_by_key over raw device words (C ABI below).
+template <class T>
+struct GreaterOp {
+  KOKKOS_FUNCTION bool operator()(const T& a, const T& b) const { return a > b; }
+};
+template <class K, class V>
+void sort_by_key_values(K* keys, V* values, int64_t n, int descending) {
+  namespace KE = Kokkos::Experimental;
+  Kokkos::MI355X ex;
+  if (descending) KE::sort_by_key(ex, DevView<K>(keys, n), DevView<V>(values, n), GreaterOp<K>{});
+  else KE::sort_by_key(ex, DevView<K>(keys, n), DevView<V>(values, n));
+  ex.fence();
+}
+template <class K>
+void sort_by_key_typed(void* keys, void* values, int64_t n, int value_bytes, int descending) {
+  if (value_bytes == 4) sort_by_key_values(static_cast<K*>(keys), static_cast<uint32_t*>(values), n, descending);
+  else sort_by_key_values(static_cast<K*>(keys), static_cast<uint64_t*>(values), n, descending);
+}
+}  // namespace
+
 extern "C" {
 
 int kkmx_init(int device) {
@@ -548,6 +569,21 @@ int kkmx_sort(void* keys, int64_t n, int type) {
       case 3: Kokkos::sort(ex, DevView<double>(static_cast<double*>(keys), n)); break;
       case 4: Kokkos::sort(ex, DevView<uint32_t>(static_cast<uint32_t*>(keys), n)); break;
       default: Kokkos::sort(ex, DevView<uint64_t>(static_cast<uint64_t*>(keys), n)); break;
+    }
+  });
+}
+int kkmx_sort_by_key(void* keys, void* values, int64_t n, int key_type, int value_bytes, int descending) {
+  if (n < 0 || (n > 0 && (!keys || !values)) || key_type < 0 || key_type > 5 ||
+      (value_bytes != 4 && value_bytes != 8) || descending < 0 || descending > 1)
+    return reject("kkmx_sort_by_key: bad arguments");
+  return guarded([&] {
+    switch (key_type) {
+      case 0: sort_by_key_typed<int32_t>(keys, values, n, value_bytes, descending); break;
+      case 1: sort_by_key_typed<int64_t>(keys, values, n, value_bytes, descending); break;
+      case 2: sort_by_key_typed<float>(keys, values, n, value_bytes, descending); break;
+      case 3: sort_by_key_typed<double>(keys, values, n, value_bytes, descending); break;
+      case 4: sort_by_key_typed<uint32_t>(keys, values, n, value_bytes, descending); break;
+      default: sort_by_key_typed<uint64_t>(keys, values, n, value_bytes, descending); break;
     }
   });
 }

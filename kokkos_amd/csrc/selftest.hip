@@ -9,6 +9,7 @@
 #include <Kokkos_Core.hpp>
 #include <Kokkos_StdAlgorithms.hpp>
 #include <Kokkos_ScatterView.hpp>
+#include <Kokkos_Sort.hpp>
 
 #include <cstdio>
 #include <cstring>
@@ -1773,6 +1774,99 @@ KKMX_TEST(scatter_view_duplicated_replicas) {
   KE::contribute(bins, sv);
   Kokkos::deep_copy(h, bins);
   CHECK_EQ(h(74), int64_t(2 * n) + 5);
+}
+
+// =========================================================================
+// sort_by_key through the permutation path (values that are not contiguous
+// 4- or 8-byte words): a LayoutStride column of a rank-2 View and 2-byte
+// values (algorithms/unit_tests/TestSortByKey.hpp checks keys and values
+// against a host sort of the pairs).
+// =========================================================================
+// A comparator that is not trivially copyable (TestSortByKey.hpp:33-60).
+struct GreaterHoldingView {
+  Kokkos::View<int*, Exec> dummy;
+  template <class T>
+  KOKKOS_INLINE_FUNCTION bool operator()(const T& a, const T& b) const {
+    return a > b;
+  }
+};
+
+// Empty Views on the device and on the host, and the comparator overload
+// (TestSortByKey.hpp:105-200).
+KKMX_TEST(sort_by_key_empty_and_comparator) {
+  Kokkos::View<int*, Exec> k0("k0", 0);
+  Kokkos::View<float*, Exec> v0("v0", 0);
+  Kokkos::Experimental::sort_by_key(Exec(), k0, v0);
+  Kokkos::View<int*, Kokkos::HostSpace> hk0("hk0", 0);
+  Kokkos::View<float*, Kokkos::HostSpace> hv0("hv0", 0);
+  Kokkos::Experimental::sort_by_key(Kokkos::Serial(), hk0, hv0);
+
+  const int n = 4099;
+  Kokkos::View<double*, Exec> keys("keys", n);
+  Kokkos::View<int*, Exec> vals("vals", n);
+  auto hk = Kokkos::create_mirror_view(keys);
+  auto hv = Kokkos::create_mirror_view(vals);
+  for (int i = 0; i < n; ++i) {
+    hk(i) = double((i * 37) % 101);
+    hv(i) = i;
+  }
+  Kokkos::deep_copy(keys, hk);
+  Kokkos::deep_copy(vals, hv);
+  Kokkos::Experimental::sort_by_key(Exec(), keys, vals, GreaterHoldingView{Kokkos::View<int*, Exec>("dummy", 1)});
+  auto gk = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), keys);
+  auto gv = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), vals);
+  for (int i = 0; i < n; ++i) {
+    CHECK_EQ(gk(i), hk(gv(i)));                     // values moved with their keys
+    if (i + 1 < n) CHECK(!(gk(i + 1) > gk(i)));     // descending
+    if (i + 1 < n && gk(i + 1) == gk(i)) CHECK(gv(i) < gv(i + 1));  // ties in input order
+  }
+}
+
+KKMX_TEST(sort_by_key_strided_and_short_values) {
+  const int n = 100003;
+  std::vector<int> hk(n);
+  for (int i = 0; i < n; ++i) hk[i] = int((int64_t(i) * 7919) % 1000) - 500;  // many ties
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hk[a] < hk[b]; });
+
+  Kokkos::View<int*, Exec> keys("keys", n);
+  Kokkos::View<double**, Kokkos::LayoutRight, Exec> table("table", n, 3);
+  auto hkv = Kokkos::create_mirror_view(keys);
+  auto ht  = Kokkos::create_mirror_view(table);
+  for (int i = 0; i < n; ++i) {
+    hkv(i) = hk[i];
+    for (int c = 0; c < 3; ++c) ht(i, c) = 10.0 * i + c;
+  }
+  Kokkos::deep_copy(keys, hkv);
+  Kokkos::deep_copy(table, ht);
+  // column 1 of the table as a LayoutStride View (stride 3)
+  Kokkos::View<double*, Kokkos::LayoutStride, Exec> column(table.data() + 1, Kokkos::LayoutStride(n, 3));
+  Kokkos::Experimental::sort_by_key(Exec(), keys, column);
+  Kokkos::deep_copy(hkv, keys);
+  Kokkos::deep_copy(ht, table);
+  for (int i = 0; i < n; ++i) {
+    CHECK_EQ(hkv(i), hk[order[i]]);
+    CHECK_EQ(ht(i, 1), 10.0 * order[i] + 1);
+    CHECK_EQ(ht(i, 0), 10.0 * i);  // the other columns stay put
+  }
+
+  Kokkos::View<int*, Exec> keys2("keys2", n);
+  Kokkos::View<int16_t*, Exec> small("small", n);
+  auto hs = Kokkos::create_mirror_view(small);
+  for (int i = 0; i < n; ++i) {
+    hkv(i) = hk[i];
+    hs(i)  = int16_t(i % 30000);
+  }
+  Kokkos::deep_copy(keys2, hkv);
+  Kokkos::deep_copy(small, hs);
+  Kokkos::Experimental::sort_by_key(Exec(), keys2, small);
+  Kokkos::deep_copy(hkv, keys2);
+  Kokkos::deep_copy(hs, small);
+  for (int i = 0; i < n; ++i) {
+    CHECK_EQ(hkv(i), hk[order[i]]);
+    CHECK_EQ(int(hs(i)), order[i] % 30000);
+  }
 }
 
 // =========================================================================

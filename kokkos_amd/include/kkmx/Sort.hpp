@@ -376,10 +376,18 @@ __global__ __launch_bounds__(256) void k_radix_hist(const void* in, const int64_
 // digit-sorted copy of the tile in LDS, and writes that copy out so each
 // digit's keys leave as one contiguous run.  Lane t carries digit t's next
 // output position from tile to tile in a register.
-template <class T, bool First, bool Last>
-__global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out, const int64_t n, const int shift,
-                                                       const uint32_t* offsets, const int64_t nblocks,
-                                                       const int64_t tpb) {
+//
+// A payload word PT (4 or 8 bytes) can travel with each key through the same
+// slots: the values of a sort by key, or with Iota the key's input position,
+// generated on the first pass, so the last pass leaves the sorting
+// permutation.  NoPayload sorts keys alone.
+struct NoPayload {};
+
+template <class T, bool First, bool Last, class PT, bool Iota>
+__global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out, const PT* in_p, PT* out_p,
+                                                       const int64_t n, const int shift, const uint32_t* offsets,
+                                                       const int64_t nblocks, const int64_t tpb) {
+  constexpr bool Pairs = !std::is_same<PT, NoPayload>::value;
   using R = RadixKey<T>;
   using U = typename R::U;
   constexpr int NW   = kRadixLanes / 64;
@@ -389,6 +397,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out
   __shared__ uint32_t wcnt[NW][256];
   __shared__ uint32_t wsum[NW];
   __shared__ U tile[kRadixTile];
+  __shared__ PT ptile[Pairs ? kRadixTile : 1];
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const uint64_t lt = (uint64_t(1) << l) - 1;
   uint32_t gnext    = offsets[int64_t(t) * nblocks + blockIdx.x];  // digit t's next output position
@@ -401,11 +410,16 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out
     __syncthreads();  // counters cleared; the previous tile is fully written out
     const int64_t wfirst = tfirst + int64_t(w) * WKEY;
     U u[kRadixItems];
+    PT ix[Pairs ? kRadixItems : 1];
 #pragma unroll
     for (int r = 0; r < kRadixItems; ++r) {
       const int64_t i = wfirst + int64_t(r) * 64 + l;
       u[r]            = 0;
       if (i < n) u[r] = First ? R::to(static_cast<const T*>(in)[i]) : static_cast<const U*>(in)[i];
+      if constexpr (Pairs) {
+        if constexpr (First && Iota) ix[r] = PT(i);
+        else if (i < n) ix[r] = in_p[i];
+      }
     }
     uint32_t rk[kRadixItems];
 #pragma unroll
@@ -459,7 +473,11 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out
     __syncthreads();  // slot bases and output shifts visible
 #pragma unroll
     for (int r = 0; r < kRadixItems; ++r) {
-      if (wfirst + int64_t(r) * 64 + l < n) tile[wcnt[w][R::digit(u[r], shift)] + rk[r]] = u[r];
+      if (wfirst + int64_t(r) * 64 + l < n) {
+        const uint32_t slot = wcnt[w][R::digit(u[r], shift)] + rk[r];
+        tile[slot]          = u[r];
+        if constexpr (Pairs) ptile[slot] = ix[r];
+      }
     }
     __syncthreads();  // tile sorted by digit
     const int64_t left = n - tfirst;
@@ -469,6 +487,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const void* in, void* out
       const uint32_t pos = gshift[R::digit(v, shift)] + uint32_t(j);
       if (Last) static_cast<T*>(out)[pos] = R::from(v);
       else static_cast<U*>(out)[pos] = v;
+      if constexpr (Pairs) out_p[pos] = ptile[j];
     }
     // the next tile writes tile, wcnt and gshift only after its first
     // barrier, which every lane reaches after this write-out
@@ -511,11 +530,35 @@ inline void radix_workspace_release() {
   w.hooked = false;
 }
 
+template <class T, class PT, bool Iota>
+const void* radix_scatter_kernel(bool first, bool last) {
+  if (first) return last ? reinterpret_cast<const void*>(&k_radix_scatter<T, true, true, PT, Iota>)
+                         : reinterpret_cast<const void*>(&k_radix_scatter<T, true, false, PT, Iota>);
+  return last ? reinterpret_cast<const void*>(&k_radix_scatter<T, false, true, PT, Iota>)
+              : reinterpret_cast<const void*>(&k_radix_scatter<T, false, false, PT, Iota>);
+}
+
 // Ascending LSD radix sort of n keys in place (device memory), on exec.
-template <class T>
-void radix_sort(const MI355X& exec, T* keys, int64_t n) {
-  if (n < 2) return;
+// With a payload (n device words of PT), the sort is of (key, payload)
+// pairs and the payload is permuted with the keys, in place; with iota the
+// payload's input is ignored and it receives the sorting permutation
+// (sorted key i came from position payload[i]).
+template <class T, class PT = NoPayload, bool Iota = false>
+void radix_sort(const MI355X& exec, T* keys, int64_t n, PT* payload = nullptr) {
+  static_assert(std::is_same<PT, NoPayload>::value || sizeof(PT) == 4 || sizeof(PT) == 8,
+                "radix sort payloads are 4- or 8-byte words");
+  static_assert(!Iota || std::is_same<PT, uint32_t>::value, "a permutation payload is 32-bit");
   if (n > int64_t(0x7fffffff)) throw_runtime_exception("Kokkos::sort: more than 2^31 - 1 keys");
+  if (n < 2) {
+    if constexpr (Iota) {
+      if (n == 1) {
+        const uint32_t zero = 0;
+        KKMX_SAFE_CALL(mx_memcpy_async(payload, &zero, sizeof(zero), exec.stream()));
+        exec.fence("Kokkos::sort: one key");
+      }
+    }
+    return;
+  }
   using U              = typename RadixKey<T>::U;
   const int64_t ntiles = (n + kRadixTile - 1) / kRadixTile;
   // segments of whole tiles, about four blocks per CU
@@ -526,14 +569,17 @@ void radix_sort(const MI355X& exec, T* keys, int64_t n) {
   const int passes     = int(sizeof(T));
   const size_t key_bytes   = (size_t(n) * sizeof(U) + 255) & ~size_t(255);
   const size_t table_bytes = size_t(256 * nblk) * sizeof(uint32_t);
+  constexpr bool pairs     = !std::is_same<PT, NoPayload>::value;
+  const size_t idx_bytes   = pairs ? (size_t(n) * sizeof(PT) + 255) & ~size_t(255) : 0;
+  const size_t need        = key_bytes + 2 * table_bytes + idx_bytes;
   RadixWorkspace& ws       = radix_workspace();
   std::lock_guard<std::mutex> lock(ws.mtx);
-  if (ws.bytes < key_bytes + 2 * table_bytes) {
+  if (ws.bytes < need) {
     if (ws.buf) KKMX_SAFE_CALL(mx_free(ws.buf, nullptr));
     ws.buf   = nullptr;
     ws.bytes = 0;
-    KKMX_SAFE_CALL(mx_malloc(&ws.buf, key_bytes + 2 * table_bytes, nullptr));
-    ws.bytes = key_bytes + 2 * table_bytes;
+    KKMX_SAFE_CALL(mx_malloc(&ws.buf, need, nullptr));
+    ws.bytes = need;
   }
   if (!ws.hooked) {
     push_finalize_hook(&radix_workspace_release);
@@ -543,6 +589,7 @@ void radix_sort(const MI355X& exec, T* keys, int64_t n) {
   void* buf[2] = {static_cast<void*>(keys), static_cast<void*>(base)};
   uint32_t* hp = reinterpret_cast<uint32_t*>(base + key_bytes);
   uint32_t* op = reinterpret_cast<uint32_t*>(base + key_bytes + table_bytes);
+  PT* ibuf[2]   = {payload, pairs ? reinterpret_cast<PT*>(base + key_bytes + 2 * table_bytes) : nullptr};
   for (int p = 0; p < passes; ++p) {
     const void* in   = buf[p & 1];
     void* out        = buf[(p + 1) & 1];
@@ -554,15 +601,15 @@ void radix_sort(const MI355X& exec, T* keys, int64_t n) {
     mx_launch_kernel(kh, unsigned(nblk), kRadixLanes, 0, exec.stream(), in, n, shift, hp, nb, tp);
     parallel_scan("Kokkos::sort::radix_offsets", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, 256 * nblk),
                   ExclusiveScanU32{hp, op});
-    const void* ks = first ? (last ? reinterpret_cast<const void*>(&k_radix_scatter<T, true, true>)
-                                   : reinterpret_cast<const void*>(&k_radix_scatter<T, true, false>))
-                           : (last ? reinterpret_cast<const void*>(&k_radix_scatter<T, false, true>)
-                                   : reinterpret_cast<const void*>(&k_radix_scatter<T, false, false>));
+    const void* ks      = radix_scatter_kernel<T, PT, Iota>(first, last);
     const uint32_t* cop = op;
-    mx_launch_kernel(ks, unsigned(nblk), kRadixLanes, 0, exec.stream(), in, out, n, shift, cop, nb, tp);
+    const PT* iin       = ibuf[p & 1];
+    PT* iout            = ibuf[(p + 1) & 1];
+    mx_launch_kernel(ks, unsigned(nblk), kRadixLanes, 0, exec.stream(), in, out, iin, iout, n, shift, cop, nb, tp);
   }
-  // an even number of passes leaves the keys in place; the workspace is
-  // free for the next call once the stream has run them
+  // an even number of passes leaves the keys (and the permutation) in
+  // place; the workspace is free for the next call once the stream has run
+  // them
   exec.fence("Kokkos::sort: radix passes");
 }
 
@@ -655,6 +702,140 @@ void sort(ViewType view, size_t begin, size_t end) {
   sort(MI355X(), view, begin, end);
   Kokkos::fence("Kokkos::sort: after");
 }
+
+// ------------------------------------------------------------------ sort_by_key
+// algorithms/src/sorting/Kokkos_SortByKeyPublicAPI.hpp:27-108 and
+// impl/Kokkos_SortByKeyImpl.hpp:98-109 (admissible Views), 262-324
+// (permutation fallback).  Keys that the radix sort takes are sorted on the
+// device with the values riding along as radix payload (contiguous values of
+// 4 or 8 bytes) or as (key, position) pairs whose permutation then gathers
+// the values; either way equal keys keep their input order, as with the
+// rocThrust path (impl/Kokkos_SortByKeyImpl.hpp:145-158).  A comparator, or keys
+// the radix sort does not take, sort a permutation on the host with
+// std::stable_sort.
+namespace Impl {
+
+template <class V>
+constexpr bool sort_by_key_admissible() {
+  using L = typename V::array_layout;
+  return is_view<V>::value && V::rank == 1 &&
+         (std::is_same<L, LayoutLeft>::value || std::is_same<L, LayoutRight>::value ||
+          std::is_same<L, LayoutStride>::value);
+}
+
+template <class Dst, class Src>
+struct SortByKeyCopyF {
+  Dst dst;
+  Src src;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { dst(i) = src(i); }
+};
+template <class Dst, class Src, class P>
+struct SortByKeyGatherF {
+  Dst dst;
+  Src src;
+  P perm;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { dst(i) = src(perm(i)); }
+};
+
+// view(i) = view(perm(i)) for every i, through a device copy of view.
+template <class VV, class P>
+void apply_permutation(const MI355X& exec, const P& perm, const VV& view) {
+  using T         = typename VV::non_const_value_type;
+  const int64_t n = int64_t(view.extent(0));
+  View<T*, MI355XSpace> copy(view_alloc(WithoutInitializing, std::string("Kokkos::sort_by_key::permute_copy")),
+                             size_t(n));
+  parallel_for("Kokkos::sort_by_key::copy", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, n),
+               SortByKeyCopyF<decltype(copy), VV>{copy, view});
+  parallel_for("Kokkos::sort_by_key::gather", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, n),
+               SortByKeyGatherF<VV, decltype(copy), P>{view, copy, perm});
+  exec.fence("Kokkos::sort_by_key: permuted");
+}
+
+template <class KV, class VV, class Comp>
+void sort_by_key_host(const KV& keys, const VV& values, Comp comp) {
+  const size_t n = keys.extent(0);
+  auto hk        = create_mirror_view_and_copy(HostSpace(), keys);
+  auto hv        = create_mirror_view_and_copy(HostSpace(), values);
+  std::vector<uint32_t> idx(n);
+  for (size_t i = 0; i < n; ++i) idx[i] = uint32_t(i);
+  std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return comp(hk(a), hk(b)); });
+  std::vector<typename KV::non_const_value_type> k(n);
+  std::vector<typename VV::non_const_value_type> v(n);
+  for (size_t i = 0; i < n; ++i) {
+    k[i] = hk(idx[i]);
+    v[i] = hv(idx[i]);
+  }
+  for (size_t i = 0; i < n; ++i) {
+    hk(i) = k[i];
+    hv(i) = v[i];
+  }
+  deep_copy(keys, hk);
+  deep_copy(values, hv);
+}
+
+template <class ExecutionSpace, class KV, class VV>
+void check_sort_by_key(const KV& keys, const VV& values) {
+  static_assert(sort_by_key_admissible<KV>() && sort_by_key_admissible<VV>(),
+                "Kokkos::sort_by_key only accepts 1D values View with LayoutRight, LayoutLeft or LayoutStride.");
+  static_assert(SpaceAccessibility<ExecutionSpace, typename KV::memory_space>::accessible,
+                "Kokkos::sort: execution space instance is not able to access the memory space of the keys View "
+                "argument!");
+  static_assert(SpaceAccessibility<ExecutionSpace, typename VV::memory_space>::accessible,
+                "Kokkos::sort: execution space instance is not able to access the memory space of the values View "
+                "argument!");
+  if (values.size() != keys.size())
+    Kokkos::abort((std::string("values and keys extents must be the same. The values extent is ") +
+                   std::to_string(values.size()) + ", and the keys extent is " + std::to_string(keys.size()) + ".")
+                      .c_str());
+}
+
+}  // namespace Impl
+
+namespace Experimental {
+
+template <class ExecutionSpace, class KeysDataType, class... KeysProperties, class ValuesDataType,
+          class... ValuesProperties>
+void sort_by_key(const ExecutionSpace& exec, const View<KeysDataType, KeysProperties...>& keys,
+                 const View<ValuesDataType, ValuesProperties...>& values) {
+  using KV = View<KeysDataType, KeysProperties...>;
+  using VV = View<ValuesDataType, ValuesProperties...>;
+  using T  = typename KV::non_const_value_type;
+  Kokkos::Impl::check_sort_by_key<ExecutionSpace>(keys, values);
+  if (keys.extent(0) <= 1) return;
+  if constexpr (Kokkos::Impl::radix_sortable<KV>() && std::is_same<ExecutionSpace, MI355X>::value) {
+    if (keys.span_is_contiguous()) {
+      using W = typename VV::non_const_value_type;
+      if constexpr ((sizeof(W) == 4 || sizeof(W) == 8) && std::is_trivially_copyable<W>::value) {
+        if (values.span_is_contiguous()) {
+          // the values ride through the radix passes with their keys
+          using PT = std::conditional_t<sizeof(W) == 4, uint32_t, uint64_t>;
+          Kokkos::Impl::radix_sort<T, PT>(exec, keys.data(), int64_t(keys.extent(0)),
+                                          reinterpret_cast<PT*>(values.data()));
+          return;
+        }
+      }
+      View<uint32_t*, MI355XSpace> perm(view_alloc(WithoutInitializing, std::string("Kokkos::sort_by_key::permute")),
+                                        keys.extent(0));
+      Kokkos::Impl::radix_sort<T, uint32_t, true>(exec, keys.data(), int64_t(keys.extent(0)), perm.data());
+      Kokkos::Impl::apply_permutation(exec, perm, values);
+      return;
+    }
+  }
+  exec.fence("Kokkos::sort_by_key: before host sort");
+  Kokkos::Impl::sort_by_key_host(keys, values, std::less<T>());
+}
+
+template <class ExecutionSpace, class ComparatorType, class KeysDataType, class... KeysProperties,
+          class ValuesDataType, class... ValuesProperties>
+void sort_by_key(const ExecutionSpace& exec, const View<KeysDataType, KeysProperties...>& keys,
+                 const View<ValuesDataType, ValuesProperties...>& values, const ComparatorType& comparator) {
+  Kokkos::Impl::check_sort_by_key<ExecutionSpace>(keys, values);
+  if (keys.extent(0) <= 1) return;
+  exec.fence("Kokkos::sort_by_key: before host sort");
+  Kokkos::Impl::sort_by_key_host(keys, values, comparator);
+}
+
+}  // namespace Experimental
 
 }  // namespace Kokkos
 

@@ -87,3 +87,38 @@ def test_binsort_without_bin_sort_is_bin_monotone(paths, oracle):
     bins = (nbins / 10.0 * (keys[perm] + 5.0)).astype(np.int64)
     assert np.all(np.diff(bins) >= 0)
     assert np.array_equal(Vd.download(np.int64, n), vals[perm])
+
+
+@pytest.mark.parametrize("code", sorted(TYPES))
+@pytest.mark.parametrize("n", [1, 2, 4097, 1 << 20])
+@pytest.mark.parametrize("vbytes", [4, 8])
+def test_sort_by_key_is_stable(paths, oracle, code, n, vbytes):
+    """Kokkos::Experimental::sort_by_key (Kokkos_SortByKeyPublicAPI.hpp:27-64):
+    keys as numpy's stable sort (bitwise), values moved with their keys, and
+    equal keys keeping their input order (values = input positions here)."""
+    P = paths
+    x = keys_for(oracle, code, n, 90 + code)
+    vdt = np.uint32 if vbytes == 4 else np.uint64
+    vals = (np.arange(n, dtype=np.uint64) * 2654435761 % (1 << 32)).astype(vdt)
+    Dk, Dv = K.DeviceBuffer.from_array(x), K.DeviceBuffer.from_array(vals)
+    run(P, P.kkmx_sort_by_key(Dk.ptr, Dv.ptr, n, code, vbytes, 0))
+    order = np.argsort(x, kind="stable")
+    bits = {4: np.uint32, 8: np.uint64}[x.dtype.itemsize]
+    assert np.array_equal(Dk.download(TYPES[code], n).view(bits), x[order].view(bits))
+    assert np.array_equal(Dv.download(vdt, n), vals[order])
+
+
+@pytest.mark.parametrize("code", [0, 3])
+def test_sort_by_key_with_comparator(paths, oracle, code):
+    """The comparator overload (Kokkos_SortByKeyPublicAPI.hpp:67-108) with a
+    greater-than comparator: descending keys, ties in input order."""
+    P = paths
+    n = 5000
+    x = keys_for(oracle, code, n, 95)
+    vals = np.arange(n, dtype=np.uint64)
+    Dk, Dv = K.DeviceBuffer.from_array(x), K.DeviceBuffer.from_array(vals)
+    run(P, P.kkmx_sort_by_key(Dk.ptr, Dv.ptr, n, code, 8, 1))
+    # stable descending: sort by negated key; ties keep input order
+    order = sorted(range(n), key=lambda i: -float(x[i]))
+    assert np.array_equal(Dk.download(TYPES[code], n), x[order])
+    assert np.array_equal(Dv.download(np.uint64, n), vals[order])
