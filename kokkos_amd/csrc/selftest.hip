@@ -1777,6 +1777,100 @@ KKMX_TEST(scatter_view_duplicated_replicas) {
 }
 
 // =========================================================================
+// BinSort over a sub-range of the keys (algorithms/unit_tests/
+// TestBinSortB.hpp:75-185): distinct unsigned keys binned one per bin,
+// values in a LayoutStride View; and double keys with many per bin sorted
+// within bins, whose permutation must be the stable order of the sub-range.
+// =========================================================================
+KKMX_TEST(binsort_subrange_and_strided_values) {
+  const int n = 6000, b = 1000, e = 4500;
+  std::vector<unsigned> hk(n);
+  for (int i = 0; i < n; ++i) hk[i] = unsigned((int64_t(i) * 2654435761u) % 1000003u);  // distinct
+  Kokkos::View<unsigned*, Exec> keys("keys", n);
+  auto hkv = Kokkos::create_mirror_view(keys);
+  for (int i = 0; i < n; ++i) hkv(i) = hk[i];
+  Kokkos::deep_copy(keys, hkv);
+  unsigned lo = hk[b], hi = hk[b];
+  for (int i = b; i < e; ++i) {
+    lo = hk[i] < lo ? hk[i] : lo;
+    hi = hk[i] > hi ? hk[i] : hi;
+  }
+  using KV = Kokkos::View<unsigned*, Exec>;
+  Kokkos::BinOp1D<KV> binner(e - b + 5, lo, hi);
+  Kokkos::BinSort<KV, Kokkos::BinOp1D<KV>> sorter(keys, b, e, binner, true);
+  sorter.create_permute_vector(Exec());
+  // values: column 0 of an n x 2 row-major table, LayoutStride with stride 2
+  Kokkos::View<double*, Exec> store("store", 2 * n);
+  Kokkos::View<double*, Kokkos::LayoutStride, Exec> vals(store.data(), Kokkos::LayoutStride(n, 2));
+  auto hs = Kokkos::create_mirror_view(store);
+  for (int i = 0; i < 2 * n; ++i) hs(i) = 0.5 * i;
+  Kokkos::deep_copy(store, hs);
+  sorter.sort(Exec(), keys, b, e);
+  sorter.sort(Exec(), vals, b, e);
+  Exec().fence();
+  auto gk = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), keys);
+  auto gs = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), store);
+  std::vector<unsigned> want(hk.begin() + b, hk.begin() + e);
+  std::sort(want.begin(), want.end());
+  std::vector<int> row_of(1000003, -1);
+  for (int i = 0; i < n; ++i) row_of[hk[i]] = i;
+  for (int i = 0; i < n; ++i) {
+    if (i < b || i >= e) {
+      CHECK_EQ(gk(i), hk[i]);
+      CHECK_EQ(gs(2 * i), 0.5 * (2 * i));
+    } else {
+      CHECK_EQ(gk(i), want[i - b]);
+      CHECK_EQ(gs(2 * i), 0.5 * (2 * row_of[gk(i)]));  // the value moved with its key
+    }
+    CHECK_EQ(gs(2 * i + 1), 0.5 * (2 * i + 1));  // the other column stays put
+  }
+
+  // within-bin sort of a sub-range: 16 bins for 3500 keys on a coarse grid
+  std::vector<double> hd(n);
+  for (int i = 0; i < n; ++i) hd[i] = double((int64_t(i) * 7919) % 257) / 256.0;
+  Kokkos::View<double*, Exec> dkeys("dkeys", n);
+  auto hdv = Kokkos::create_mirror_view(dkeys);
+  for (int i = 0; i < n; ++i) hdv(i) = hd[i];
+  Kokkos::deep_copy(dkeys, hdv);
+  using DV = Kokkos::View<double*, Exec>;
+  Kokkos::BinSort<DV, Kokkos::BinOp1D<DV>> dsorter(dkeys, b, e, Kokkos::BinOp1D<DV>(16, 0.0, 1.0), true);
+  dsorter.create_permute_vector(Exec());
+  auto perm = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), dsorter.get_permute_vector());
+  std::vector<int> order(e - b);
+  for (int i = 0; i < e - b; ++i) order[i] = b + i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return hd[x] < hd[y]; });
+  for (int i = 0; i < e - b; ++i) CHECK_EQ(perm(i), order[i]);
+  auto cnt = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), dsorter.get_bin_count());
+  auto off = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), dsorter.get_bin_offsets());
+  int total = 0;
+  for (int k = 0; k < 17; ++k) {
+    CHECK_EQ(off(k), total);
+    total += cnt(k);
+  }
+  CHECK_EQ(total, e - b);
+
+  // without the within-bin sort (atomic placement): bins non-decreasing
+  // over the sub-range, the same keys, the rest untouched
+  Kokkos::deep_copy(keys, hkv);
+  Kokkos::BinSort<KV, Kokkos::BinOp1D<KV>> usorter(keys, b, e, binner, false);
+  usorter.create_permute_vector(Exec());
+  usorter.sort(Exec(), keys, b, e);
+  Exec().fence();
+  Kokkos::deep_copy(gk, keys);
+  std::vector<unsigned> got;
+  for (int i = 0; i < n; ++i) {
+    if (i < b || i >= e) {
+      CHECK_EQ(gk(i), hk[i]);
+      continue;
+    }
+    got.push_back(gk(i));
+    if (i > b) CHECK(binner.bin(gk, i - 1) <= binner.bin(gk, i));
+  }
+  std::sort(got.begin(), got.end());
+  CHECK(got == want);
+}
+
+// =========================================================================
 // sort_by_key through the permutation path (values that are not contiguous
 // 4- or 8-byte words): a LayoutStride column of a rank-2 View and 2-byte
 // values (algorithms/unit_tests/TestSortByKey.hpp checks keys and values

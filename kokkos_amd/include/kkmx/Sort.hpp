@@ -132,6 +132,24 @@ struct BinPlaceF {
     order(offsets(b) + atomic_fetch_add(&fill(b), 1)) = j;
   }
 };
+template <class T>
+struct is_bin_op_1d : std::false_type {};
+template <class K>
+struct is_bin_op_1d<BinOp1D<K>> : std::true_type {};
+
+template <class Dst, class Keys>
+struct BinKeyCopyF {
+  Dst dst;
+  Keys keys;
+  int begin;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const { dst(i) = keys(begin + i); }
+};
+struct BinShiftF {
+  View<int*, MI355XSpace, MemoryTraits<Unmanaged>> order;
+  int begin;
+  KOKKOS_INLINE_FUNCTION void operator()(const int i) const { order(i) += begin; }
+};
+
 // Insertion sort of one bin's indices by one thread, as the reference does on
 // devices (quadratic in the bin size: meant for the small bins BinSort is
 // used with); equal keys keep index order, so the result does not depend on
@@ -189,115 +207,6 @@ struct CopyRowsF {
 };
 
 }  // namespace Impl
-
-// ------------------------------------------------------------------ BinSort
-template <class KeyViewType, class BinSortOp, class Space = typename KeyViewType::device_type,
-          class SizeType = typename KeyViewType::size_type>
-class BinSort {
- public:
-  using exec_space       = MI355X;
-  using offset_type      = View<int*, MI355XSpace>;
-  using bin_count_type   = View<const int*, MI355XSpace>;
-  using const_key_view_type = typename KeyViewType::const_type;
-
-  BinSort() = delete;
-  template <class ExecutionSpace>
-  BinSort(const ExecutionSpace& exec, const_key_view_type keys_, int range_begin_, int range_end_, BinSortOp bin_op_,
-          bool sort_within_bins_ = false)
-      : keys(keys_),
-        range_begin(range_begin_),
-        range_end(range_end_),
-        bin_op(bin_op_),
-        sort_within_bins(sort_within_bins_) {
-    static_assert(std::is_same<ExecutionSpace, MI355X>::value, "BinSort runs on the MI355X execution space");
-    if (range_begin > range_end) Kokkos::abort("BinSort: range_begin > range_end");
-    sort_order  = offset_type("Kokkos::BinSort::sort_order", size_t(range_end - range_begin));
-    bin_count   = View<int*, MI355XSpace>("Kokkos::BinSort::bin_count", size_t(bin_op.max_bins()));
-    bin_fill    = View<int*, MI355XSpace>("Kokkos::BinSort::bin_fill", size_t(bin_op.max_bins()));
-    bin_offsets = offset_type("Kokkos::BinSort::bin_offsets", size_t(bin_op.max_bins()));
-    (void)exec;
-  }
-  BinSort(const_key_view_type keys_, int range_begin_, int range_end_, BinSortOp bin_op_, bool sort_within_bins_ = false)
-      : BinSort(exec_space{}, keys_, range_begin_, range_end_, bin_op_, sort_within_bins_) {}
-  template <class ExecutionSpace>
-  BinSort(const ExecutionSpace& exec, const_key_view_type keys_, BinSortOp bin_op_, bool sort_within_bins_ = false)
-      : BinSort(exec, keys_, 0, int(keys_.extent(0)), bin_op_, sort_within_bins_) {}
-  BinSort(const_key_view_type keys_, BinSortOp bin_op_, bool sort_within_bins_ = false)
-      : BinSort(exec_space{}, keys_, bin_op_, sort_within_bins_) {}
-
-  // Counts, offsets and the permutation; call again when the keys change.
-  template <class ExecutionSpace>
-  void create_permute_vector(const ExecutionSpace& exec) {
-    using U        = View<int*, MI355XSpace, MemoryTraits<Unmanaged>>;
-    using CU       = View<const int*, MI355XSpace, MemoryTraits<Unmanaged>>;
-    const int len  = range_end - range_begin;
-    const int bins = bin_op.max_bins();
-    deep_copy(exec, bin_count, 0);
-    deep_copy(exec, bin_fill, 0);
-    U cnt(bin_count.data(), size_t(bins)), fill(bin_fill.data(), size_t(bins));
-    U off(bin_offsets.data(), size_t(bins)), order(sort_order.data(), size_t(len));
-    parallel_for("Kokkos::Sort::BinCount", RangePolicy<MI355X>(exec, 0, len),
-                 Impl::BinCountF<const_key_view_type, BinSortOp>{keys, bin_op, cnt, range_begin});
-    parallel_scan("Kokkos::Sort::BinOffset", RangePolicy<MI355X>(exec, 0, bins), Impl::BinOffsetF{CU(cnt), off});
-    parallel_for("Kokkos::Sort::BinBinning", RangePolicy<MI355X>(exec, 0, len),
-                 Impl::BinPlaceF<const_key_view_type, BinSortOp>{keys, bin_op, fill, CU(off), order, range_begin});
-    if (sort_within_bins)
-      parallel_for("Kokkos::Sort::BinSort", RangePolicy<MI355X>(exec, 0, bins),
-                   Impl::BinSortWithinF<const_key_view_type, BinSortOp>{keys, bin_op, CU(cnt), CU(off), order});
-  }
-  void create_permute_vector() {
-    Kokkos::fence("Kokkos::BinSort::create_permute_vector: before");
-    exec_space e{};
-    create_permute_vector(e);
-    e.fence("Kokkos::BinSort::create_permute_vector: after");
-  }
-
-  // values(range_begin + i) = values(sort_order(i) + values_range_begin - range_begin)
-  template <class ExecutionSpace, class ValuesViewType>
-  void sort(const ExecutionSpace& exec, const ValuesViewType& values, int values_range_begin,
-            int values_range_end) const {
-    if (values.extent(0) == 0) return;
-    const int len = range_end - range_begin;
-    if (values_range_end - values_range_begin != len)
-      Kokkos::abort("BinSort::sort: values range length != permutation vector length");
-    using Tmp = View<typename ValuesViewType::non_const_data_type, typename ValuesViewType::array_layout, MI355XSpace>;
-    Tmp tmp(std::string("Kokkos::SortImpl::BinSortFunctor::sorted_values"), size_t(len),
-            ValuesViewType::rank > 1 ? values.extent(1) : ~size_t(0));  // ~0: no extent
-    View<const int*, MI355XSpace, MemoryTraits<Unmanaged>> order(sort_order.data(), size_t(len));
-    parallel_for("Kokkos::Sort::CopyPermute", RangePolicy<MI355X>(exec, 0, len),
-                 Impl::PermuteRowsF<Tmp, ValuesViewType>{tmp, values, order, values_range_begin - range_begin});
-    parallel_for("Kokkos::Sort::Copy", RangePolicy<MI355X>(exec, 0, len),
-                 Impl::CopyRowsF<ValuesViewType, Tmp>{values, tmp, range_begin});
-  }
-  template <class ValuesViewType>
-  void sort(const ValuesViewType& values, int values_range_begin, int values_range_end) const {
-    Kokkos::fence("Kokkos::BinSort::sort: before");
-    exec_space exec;
-    sort(exec, values, values_range_begin, values_range_end);
-    exec.fence("Kokkos::BinSort::sort: after");
-  }
-  template <class ExecutionSpace, class ValuesViewType, class = std::enable_if_t<is_execution_space<ExecutionSpace>::value>>
-  void sort(const ExecutionSpace& exec, const ValuesViewType& values) const {
-    sort(exec, values, 0, range_end - range_begin);
-  }
-  template <class ValuesViewType>
-  void sort(const ValuesViewType& values) const {
-    sort(values, 0, range_end - range_begin);
-  }
-
-  offset_type get_permute_vector() const { return sort_order; }
-  offset_type get_bin_offsets() const { return bin_offsets; }
-  bin_count_type get_bin_count() const { return bin_count; }
-
- private:
-  const_key_view_type keys;
-  int range_begin = 0, range_end = 0;
-  BinSortOp bin_op;
-  bool sort_within_bins = false;
-  offset_type sort_order;
-  View<int*, MI355XSpace> bin_count, bin_fill;
-  offset_type bin_offsets;
-};
 
 // ------------------------------------------------------------------ radix sort
 namespace Impl {
@@ -647,6 +556,133 @@ void host_sort(const V& view, size_t begin, size_t end, Comp comp) {
 }
 
 }  // namespace Impl
+
+// ------------------------------------------------------------------ BinSort
+template <class KeyViewType, class BinSortOp, class Space = typename KeyViewType::device_type,
+          class SizeType = typename KeyViewType::size_type>
+class BinSort {
+ public:
+  using exec_space       = MI355X;
+  using offset_type      = View<int*, MI355XSpace>;
+  using bin_count_type   = View<const int*, MI355XSpace>;
+  using const_key_view_type = typename KeyViewType::const_type;
+
+  BinSort() = delete;
+  template <class ExecutionSpace>
+  BinSort(const ExecutionSpace& exec, const_key_view_type keys_, int range_begin_, int range_end_, BinSortOp bin_op_,
+          bool sort_within_bins_ = false)
+      : keys(keys_),
+        range_begin(range_begin_),
+        range_end(range_end_),
+        bin_op(bin_op_),
+        sort_within_bins(sort_within_bins_) {
+    static_assert(std::is_same<ExecutionSpace, MI355X>::value, "BinSort runs on the MI355X execution space");
+    if (range_begin > range_end) Kokkos::abort("BinSort: range_begin > range_end");
+    sort_order  = offset_type("Kokkos::BinSort::sort_order", size_t(range_end - range_begin));
+    bin_count   = View<int*, MI355XSpace>("Kokkos::BinSort::bin_count", size_t(bin_op.max_bins()));
+    bin_fill    = View<int*, MI355XSpace>("Kokkos::BinSort::bin_fill", size_t(bin_op.max_bins()));
+    bin_offsets = offset_type("Kokkos::BinSort::bin_offsets", size_t(bin_op.max_bins()));
+    (void)exec;
+  }
+  BinSort(const_key_view_type keys_, int range_begin_, int range_end_, BinSortOp bin_op_, bool sort_within_bins_ = false)
+      : BinSort(exec_space{}, keys_, range_begin_, range_end_, bin_op_, sort_within_bins_) {}
+  template <class ExecutionSpace>
+  BinSort(const ExecutionSpace& exec, const_key_view_type keys_, BinSortOp bin_op_, bool sort_within_bins_ = false)
+      : BinSort(exec, keys_, 0, int(keys_.extent(0)), bin_op_, sort_within_bins_) {}
+  BinSort(const_key_view_type keys_, BinSortOp bin_op_, bool sort_within_bins_ = false)
+      : BinSort(exec_space{}, keys_, bin_op_, sort_within_bins_) {}
+
+  // Counts, offsets and the permutation; call again when the keys change.
+  template <class ExecutionSpace>
+  void create_permute_vector(const ExecutionSpace& exec) {
+    using U        = View<int*, MI355XSpace, MemoryTraits<Unmanaged>>;
+    using CU       = View<const int*, MI355XSpace, MemoryTraits<Unmanaged>>;
+    const int len  = range_end - range_begin;
+    const int bins = bin_op.max_bins();
+    deep_copy(exec, bin_count, 0);
+    deep_copy(exec, bin_fill, 0);
+    U cnt(bin_count.data(), size_t(bins)), fill(bin_fill.data(), size_t(bins));
+    U off(bin_offsets.data(), size_t(bins)), order(sort_order.data(), size_t(len));
+    parallel_for("Kokkos::Sort::BinCount", RangePolicy<MI355X>(exec, 0, len),
+                 Impl::BinCountF<const_key_view_type, BinSortOp>{keys, bin_op, cnt, range_begin});
+    parallel_scan("Kokkos::Sort::BinOffset", RangePolicy<MI355X>(exec, 0, bins), Impl::BinOffsetF{CU(cnt), off});
+    if constexpr (Impl::is_bin_op_1d<BinSortOp>::value && KeyViewType::rank == 1 &&
+                  Impl::radix_sortable<KeyViewType>()) {
+      // BinOp1D's bin never decreases as the key grows, so "by bin, then by
+      // key, equal keys in index order" is the stable order of the keys: the
+      // permutation of a radix sort of (key, position) pairs
+      if (sort_within_bins && len > 0) {
+        using T = typename KeyViewType::non_const_value_type;
+        View<T*, MI355XSpace> copy(view_alloc(WithoutInitializing, std::string("Kokkos::BinSort::key_copy")),
+                                   size_t(len));
+        parallel_for("Kokkos::Sort::BinKeyCopy", RangePolicy<MI355X>(exec, 0, len),
+                     Impl::BinKeyCopyF<View<T*, MI355XSpace>, const_key_view_type>{copy, keys, range_begin});
+        Impl::radix_sort<T, uint32_t, true>(exec, copy.data(), int64_t(len),
+                                            reinterpret_cast<uint32_t*>(sort_order.data()));
+        if (range_begin != 0)
+          parallel_for("Kokkos::Sort::BinShift", RangePolicy<MI355X>(exec, 0, len), Impl::BinShiftF{order, range_begin});
+        return;
+      }
+    }
+    parallel_for("Kokkos::Sort::BinBinning", RangePolicy<MI355X>(exec, 0, len),
+                 Impl::BinPlaceF<const_key_view_type, BinSortOp>{keys, bin_op, fill, CU(off), order, range_begin});
+    if (sort_within_bins)
+      parallel_for("Kokkos::Sort::BinSort", RangePolicy<MI355X>(exec, 0, bins),
+                   Impl::BinSortWithinF<const_key_view_type, BinSortOp>{keys, bin_op, CU(cnt), CU(off), order});
+  }
+  void create_permute_vector() {
+    Kokkos::fence("Kokkos::BinSort::create_permute_vector: before");
+    exec_space e{};
+    create_permute_vector(e);
+    e.fence("Kokkos::BinSort::create_permute_vector: after");
+  }
+
+  // values(range_begin + i) = values(sort_order(i) + values_range_begin - range_begin)
+  template <class ExecutionSpace, class ValuesViewType>
+  void sort(const ExecutionSpace& exec, const ValuesViewType& values, int values_range_begin,
+            int values_range_end) const {
+    if (values.extent(0) == 0) return;
+    const int len = range_end - range_begin;
+    if (values_range_end - values_range_begin != len)
+      Kokkos::abort("BinSort::sort: values range length != permutation vector length");
+    using Tmp = View<typename ValuesViewType::non_const_data_type, typename ValuesViewType::array_layout, MI355XSpace>;
+    Tmp tmp(std::string("Kokkos::SortImpl::BinSortFunctor::sorted_values"), size_t(len),
+            ValuesViewType::rank > 1 ? values.extent(1) : ~size_t(0));  // ~0: no extent
+    View<const int*, MI355XSpace, MemoryTraits<Unmanaged>> order(sort_order.data(), size_t(len));
+    parallel_for("Kokkos::Sort::CopyPermute", RangePolicy<MI355X>(exec, 0, len),
+                 Impl::PermuteRowsF<Tmp, ValuesViewType>{tmp, values, order, values_range_begin - range_begin});
+    parallel_for("Kokkos::Sort::Copy", RangePolicy<MI355X>(exec, 0, len),
+                 Impl::CopyRowsF<ValuesViewType, Tmp>{values, tmp, range_begin});
+  }
+  template <class ValuesViewType>
+  void sort(const ValuesViewType& values, int values_range_begin, int values_range_end) const {
+    Kokkos::fence("Kokkos::BinSort::sort: before");
+    exec_space exec;
+    sort(exec, values, values_range_begin, values_range_end);
+    exec.fence("Kokkos::BinSort::sort: after");
+  }
+  template <class ExecutionSpace, class ValuesViewType, class = std::enable_if_t<is_execution_space<ExecutionSpace>::value>>
+  void sort(const ExecutionSpace& exec, const ValuesViewType& values) const {
+    sort(exec, values, 0, range_end - range_begin);
+  }
+  template <class ValuesViewType>
+  void sort(const ValuesViewType& values) const {
+    sort(values, 0, range_end - range_begin);
+  }
+
+  offset_type get_permute_vector() const { return sort_order; }
+  offset_type get_bin_offsets() const { return bin_offsets; }
+  bin_count_type get_bin_count() const { return bin_count; }
+
+ private:
+  const_key_view_type keys;
+  int range_begin = 0, range_end = 0;
+  BinSortOp bin_op;
+  bool sort_within_bins = false;
+  offset_type sort_order;
+  View<int*, MI355XSpace> bin_count, bin_fill;
+  offset_type bin_offsets;
+};
 
 // ------------------------------------------------------------------ sort
 template <class ExecutionSpace, class DataType, class... Properties,
