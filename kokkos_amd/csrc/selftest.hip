@@ -1777,6 +1777,183 @@ KKMX_TEST(scatter_view_duplicated_replicas) {
 }
 
 // =========================================================================
+// subview (core/unit_test/TestViewSubview.hpp: layouts of the results,
+// values read and written through them, deep_copy of non-contiguous
+// windows, the allocation kept alive by a subview).
+// =========================================================================
+template <class V2>
+struct ReadSub2 {
+  V2 s;
+  Kokkos::View<double*, Exec> out;
+  int n1;
+  KOKKOS_FUNCTION void operator()(const int i) const { out(i) = s(i / n1, i % n1); }
+};
+template <class V1>
+struct ReadSub1 {
+  V1 s;
+  Kokkos::View<double*, Exec> out;
+  KOKKOS_FUNCTION void operator()(const int i) const { out(i) = s(i); }
+};
+template <class V2>
+struct NegateSub2 {
+  V2 s;
+  int n1;
+  KOKKOS_FUNCTION void operator()(const int i) const { s(i / n1, i % n1) = -s(i / n1, i % n1); }
+};
+
+template <class V2>
+std::vector<double> read2(const V2& s) {
+  const int n0 = int(s.extent(0)), n1 = int(s.extent(1));
+  Kokkos::View<double*, Exec> out("out", size_t(n0 * n1));
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n0 * n1), ReadSub2<V2>{s, out, n1});
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+  return std::vector<double>(h.data(), h.data() + n0 * n1);
+}
+template <class V1>
+std::vector<double> read1(const V1& s) {
+  const int n = int(s.extent(0));
+  Kokkos::View<double*, Exec> out("out", size_t(n));
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), ReadSub1<V1>{s, out});
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+  return std::vector<double>(h.data(), h.data() + n);
+}
+
+KKMX_TEST(subview_layouts_values_and_copies) {
+  using R3 = Kokkos::View<double***, Kokkos::LayoutRight, Exec>;
+  using L3 = Kokkos::View<double***, Kokkos::LayoutLeft, Exec>;
+  const int N0 = 4, N1 = 5, N2 = 6;
+  auto f = [](int i, int j, int k) { return 100.0 * i + 10.0 * j + k; };
+  R3 v("v", N0, N1, N2);
+  L3 w("w", N0, N1, N2);
+  {
+    auto hv = Kokkos::create_mirror_view(v);
+    auto hw = Kokkos::create_mirror_view(w);
+    for (int i = 0; i < N0; ++i)
+      for (int j = 0; j < N1; ++j)
+        for (int k = 0; k < N2; ++k) hv(i, j, k) = hw(i, j, k) = f(i, j, k);
+    Kokkos::deep_copy(v, hv);
+    Kokkos::deep_copy(w, hw);
+  }
+  // LayoutRight source
+  auto s1 = Kokkos::subview(v, 2, Kokkos::make_pair(1, 4), Kokkos::ALL);
+  static_assert(std::is_same<decltype(s1)::array_layout, Kokkos::LayoutRight>::value, "");
+  CHECK_EQ(s1.extent(0), size_t(3));
+  CHECK_EQ(s1.extent(1), size_t(N2));
+  CHECK_EQ(s1.stride(0), size_t(N2));
+  auto r1 = read2(s1);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < N2; ++b) CHECK_EQ(r1[a * N2 + b], f(2, 1 + a, b));
+  // rank 2 ending at the last dimension: LayoutRight with a padded stride
+  auto s2 = Kokkos::subview(v, Kokkos::ALL, 3, std::make_pair(2, 5));
+  static_assert(std::is_same<decltype(s2)::array_layout, Kokkos::LayoutRight>::value, "");
+  CHECK_EQ(s2.stride(0), size_t(N1 * N2));
+  auto r2 = read2(s2);
+  for (int a = 0; a < N0; ++a)
+    for (int b = 0; b < 3; ++b) CHECK_EQ(r2[a * 3 + b], f(a, 3, 2 + b));
+  auto s2b = Kokkos::subview(v, Kokkos::ALL, Kokkos::make_pair(1, 3), 4);
+  static_assert(std::is_same<decltype(s2b)::array_layout, Kokkos::LayoutStride>::value, "");
+  auto r2b = read2(s2b);
+  for (int a = 0; a < N0; ++a)
+    for (int b = 0; b < 2; ++b) CHECK_EQ(r2b[a * 2 + b], f(a, 1 + b, 4));
+  Kokkos::View<double*, Kokkos::LayoutRight, Exec> row = Kokkos::subview(v, 1, 2, Kokkos::ALL);
+  CHECK(row.span_is_contiguous());
+  auto r3 = read1(row);
+  for (int k = 0; k < N2; ++k) CHECK_EQ(r3[k], f(1, 2, k));
+  auto s4 = Kokkos::subview(v, 1, 2, 3);
+  static_assert(decltype(s4)::rank == 0, "");
+  double x4 = 0;
+  Kokkos::deep_copy(x4, s4);
+  CHECK_EQ(x4, f(1, 2, 3));
+  auto s5 = Kokkos::subview(s1, 1, Kokkos::ALL);  // a subview of a subview
+  auto r5 = read1(s5);
+  for (int k = 0; k < N2; ++k) CHECK_EQ(r5[k], f(2, 2, k));
+  // LayoutLeft source: a padded rank-2 window keeps LayoutLeft
+  auto t1 = Kokkos::subview(w, Kokkos::make_pair(1, 3), Kokkos::make_pair(0, 2), 4);
+  static_assert(std::is_same<decltype(t1)::array_layout, Kokkos::LayoutLeft>::value, "");
+  CHECK(!t1.span_is_contiguous());
+  auto q1 = read2(t1);
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) CHECK_EQ(q1[a * 2 + b], f(1 + a, b, 4));
+  auto t2 = Kokkos::subview(w, Kokkos::ALL, 2, Kokkos::ALL);
+  static_assert(std::is_same<decltype(t2)::array_layout, Kokkos::LayoutLeft>::value, "");
+  auto q2 = read2(t2);
+  for (int a = 0; a < N0; ++a)
+    for (int b = 0; b < N2; ++b) CHECK_EQ(q2[a * N2 + b], f(a, 2, b));
+  // write through a strided window; nothing else changes
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, N0 * 3), NegateSub2<decltype(s2)>{s2, 3});
+  {
+    auto hv = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), v);
+    for (int i = 0; i < N0; ++i)
+      for (int j = 0; j < N1; ++j)
+        for (int k = 0; k < N2; ++k)
+          CHECK_EQ(hv(i, j, k), (j == 3 && k >= 2 && k < 5) ? -f(i, j, k) : f(i, j, k));
+  }
+  // deep_copy into and out of non-contiguous windows
+  Kokkos::View<double**, Kokkos::LayoutLeft, Exec> dense("dense", 2, 2);
+  Kokkos::deep_copy(dense, t1);
+  auto hd = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), dense);
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) CHECK_EQ(hd(a, b), f(1 + a, b, 4));
+  Kokkos::deep_copy(t1, 7.0);
+  auto hw = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), w);
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < N1; ++j)
+      for (int k = 0; k < N2; ++k)
+        CHECK_EQ(hw(i, j, k), (i >= 1 && i < 3 && j < 2 && k == 4) ? 7.0 : f(i, j, k));
+  // across spaces: a padded device window to a host mirror and back
+  auto ht1 = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), t1);
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) CHECK_EQ(ht1(a, b), 7.0);
+  auto hs2 = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), s2b);
+  for (int a = 0; a < N0; ++a)
+    for (int b = 0; b < 2; ++b) CHECK_EQ(hs2(a, b), f(a, 1 + b, 4));
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) ht1(a, b) = 1000.0 + 10 * a + b;
+  Kokkos::deep_copy(t1, ht1);
+  auto hw2 = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), w);
+  for (int i = 0; i < N0; ++i)
+    for (int j = 0; j < N1; ++j)
+      for (int k = 0; k < N2; ++k)
+        CHECK_EQ(hw2(i, j, k), (i >= 1 && i < 3 && j < 2 && k == 4) ? 1000.0 + 10 * (i - 1) + j : f(i, j, k));
+  // a subview keeps the allocation alive
+  Kokkos::View<double*, Kokkos::LayoutRight, Exec> keep;
+  {
+    Kokkos::View<double**, Kokkos::LayoutRight, Exec> tmp("tmp", 3, 8);
+    Kokkos::deep_copy(tmp, 2.5);
+    keep = Kokkos::subview(tmp, 2, Kokkos::ALL);
+    CHECK_EQ(tmp.use_count(), 2);
+  }
+  CHECK_EQ(keep.use_count(), 1);
+  auto rk = read1(keep);
+  for (int k = 0; k < 8; ++k) CHECK_EQ(rk[k], 2.5);
+  // LayoutRight -> LayoutStride assignment keeps the strides
+  Kokkos::View<double**, Kokkos::LayoutStride, Exec> st = s1;
+  CHECK_EQ(st.stride(0), size_t(N2));
+  CHECK_EQ(read2(st)[N2 + 1], f(2, 2, 1));
+}
+
+// resize keeps elements by index (core/unit_test/TestResize.hpp): rank 2 in
+// both layouts, one extent growing and the other shrinking.
+template <class L>
+void resize_case() {
+  Kokkos::View<int**, L, Exec> v("v", 3, 4);
+  auto h = Kokkos::create_mirror_view(v);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 4; ++j) h(i, j) = 10 * i + j;
+  Kokkos::deep_copy(v, h);
+  Kokkos::resize(v, 5, 3);
+  CHECK_EQ(v.extent(0), size_t(5));
+  CHECK_EQ(v.extent(1), size_t(3));
+  auto g = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), v);
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 3; ++j) CHECK_EQ(g(i, j), i < 3 ? 10 * i + j : 0);
+}
+KKMX_TEST(resize_keeps_elements_by_index) {
+  resize_case<Kokkos::LayoutLeft>();
+  resize_case<Kokkos::LayoutRight>();
+}
+
+// =========================================================================
 // BinSort over a sub-range of the keys (algorithms/unit_tests/
 // TestBinSortB.hpp:75-185): distinct unsigned keys binned one per bin,
 // values in a LayoutStride View; and double keys with many per bin sorted

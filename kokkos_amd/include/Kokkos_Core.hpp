@@ -15,6 +15,7 @@
 #include <kkmx/Spaces.hpp>
 #include <kkmx/Atomic.hpp>
 #include <kkmx/View.hpp>
+#include <kkmx/Subview.hpp>
 #include <kkmx/Reducers.hpp>
 #include <kkmx/Policy.hpp>
 #include <kkmx/Tools.hpp>

@@ -235,6 +235,19 @@ void check_shapes(const DV& dst, const SV& src) {
 }  // namespace Impl
 
 namespace Impl {
+// The same elements as v, as an unmanaged View in memory space Mem with v's
+// extents and strides (a subview's strides are not implied by its extents).
+template <class Mem, class V>
+KOKKOS_INLINE_FUNCTION auto rewrap_unmanaged(const V& v) {
+  using W = View<typename V::data_type, typename V::array_layout, Mem, MemoryTraits<Unmanaged>>;
+  size_t ext[8], str[8];
+  for (unsigned r = 0; r < 8; ++r) {
+    ext[r] = v.extent(r);
+    str[r] = r < V::rank ? v.stride(r) : 1;
+  }
+  return W(SubviewCtor{}, v.data(), ext, str, nullptr);
+}
+
 // ViewFill (reference core/src/Kokkos_CopyViews.hpp:58-260): one pass over
 // the span when it is the index space, else over the index space.
 template <class V>
@@ -243,8 +256,8 @@ void host_fill(const V& dst, const typename V::const_value_type& value) {
     auto* p = dst.data();
     for (size_t i = 0; i < dst.span(); ++i) p[i] = value;
   } else {
-    using HV = View<typename V::data_type, typename V::array_layout, HostSpace, MemoryTraits<Unmanaged>>;
-    HV h(dst.data(), dst.layout());
+    auto h   = rewrap_unmanaged<HostSpace>(dst);
+    using HV = decltype(h);
     IndexFill<HV> f{h, value};
     for (size_t i = 0; i < dst.size(); ++i) {
       size_t idx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -271,8 +284,8 @@ void device_fill(const MI355X& exec, const V& dst, const typename V::const_value
     parallel_for("Kokkos::ViewFill", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, int64_t(dst.span())), f);
     return;
   }
-  using UV = View<typename V::data_type, typename V::array_layout, MI355XSpace, MemoryTraits<Unmanaged>>;
-  UV u(dst.data(), dst.layout());
+  auto u   = rewrap_unmanaged<MI355XSpace>(dst);
+  using UV = decltype(u);
   IndexFill<UV> f{u, value};
   for_each_index(exec, "Kokkos::ViewFill", u, f, [=] KOKKOS_FUNCTION(const int64_t i) {
     size_t idx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -308,10 +321,10 @@ void device_copy(const MI355X& exec, const DV& dst, const SV& src) {
     return;
   }
   if (layout_transpose(exec, dst, src)) return;
-  using UD = View<typename DV::data_type, typename DV::array_layout, MI355XSpace, MemoryTraits<Unmanaged>>;
-  using US = View<typename SV::const_data_type, typename SV::array_layout, MI355XSpace, MemoryTraits<Unmanaged>>;
-  UD d(dst.data(), dst.layout());
-  US s(src.data(), src.layout());
+  auto d   = rewrap_unmanaged<MI355XSpace>(dst);
+  auto s   = rewrap_unmanaged<MI355XSpace>(typename SV::const_type(src));
+  using UD = decltype(d);
+  using US = decltype(s);
   IndexCopy<UD, US> f{d, s};
   ElementCopy<UD, US> flat{d, s};
   for_each_index(exec, "Kokkos::ViewCopy", d, f, flat);
@@ -400,18 +413,35 @@ inline void deep_copy(const View<D1, P1...>& dst, const View<D2, P2...>& src) {
     Impl::ElementCopy<DV, SV> f{dst, src};
     for (int64_t i = 0; i < int64_t(dst.size()); ++i) f(i);
   } else {
-    // Different layouts across spaces: stage through a host copy of src in
-    // its own layout, then copy element-wise on the host / device.
-    typename SV::HostMirror tmp(view_alloc(WithoutInitializing, std::string("deep_copy_tmp")), src.layout());
-    deep_copy(tmp, src);
+    // Across spaces with different layouts or a non-contiguous operand:
+    // rearrange on whichever side can index both Views, with one contiguous
+    // transfer of a same-layout staging buffer in between.
+    using T = typename DV::non_const_value_type;
     if constexpr (Impl::host_accessible<typename DV::memory_space>()) {
-      Impl::ElementCopy<DV, typename SV::HostMirror> f{dst, tmp};
-      for (int64_t i = 0; i < int64_t(dst.size()); ++i) f(i);
+      // device -> host: reorder on the device into dst's layout, then copy
+      View<typename DV::traits::non_const_data_type, typename DV::array_layout, MI355XSpace> stage(
+          view_alloc(WithoutInitializing, std::string("deep_copy_stage")), dst.layout());
+      Impl::device_copy(MI355X(), stage, src);
+      MI355X().fence("Kokkos::deep_copy: staged device copy");
+      if (dst.span_is_contiguous()) {
+        KKMX_SAFE_CALL(mx_memcpy(dst.data(), stage.data(), dst.size() * sizeof(T)));
+      } else {
+        auto hs = create_mirror(stage);
+        KKMX_SAFE_CALL(mx_memcpy(hs.data(), stage.data(), dst.size() * sizeof(T)));
+        Impl::ElementCopy<DV, decltype(hs)> f{dst, hs};
+        for (int64_t i = 0; i < int64_t(dst.size()); ++i) f(i);
+      }
     } else {
-      typename DV::HostMirror tmp2(view_alloc(WithoutInitializing, std::string("deep_copy_tmp2")), dst.layout());
-      Impl::ElementCopy<typename DV::HostMirror, typename SV::HostMirror> f{tmp2, tmp};
-      for (int64_t i = 0; i < int64_t(dst.size()); ++i) f(i);
-      deep_copy(dst, tmp2);
+      // host -> device: a contiguous host copy in src's layout, one transfer,
+      // then reorder on the device
+      typename SV::HostMirror hs(view_alloc(WithoutInitializing, std::string("deep_copy_stage")), src.layout());
+      Impl::ElementCopy<typename SV::HostMirror, SV> f{hs, src};
+      for (int64_t i = 0; i < int64_t(src.size()); ++i) f(i);
+      View<typename SV::traits::non_const_data_type, typename SV::array_layout, MI355XSpace> stage(
+          view_alloc(WithoutInitializing, std::string("deep_copy_stage")), src.layout());
+      KKMX_SAFE_CALL(mx_memcpy(stage.data(), hs.data(), src.size() * sizeof(T)));
+      Impl::device_copy(MI355X(), dst, stage);
+      MI355X().fence("Kokkos::deep_copy: staged device copy");
     }
   }
   Kokkos::fence("Kokkos::deep_copy: copy between views, post deep copy fence");
@@ -520,15 +550,24 @@ inline void realloc(View<D, P...>& v, size_t n0 = 0, size_t n1 = 0, size_t n2 = 
   v           = V();
   v           = V(l, n0, n1, n2, n3);
 }
+// Keeps every element whose indices fit both the old and the new extents,
+// element by element (core/src/Kokkos_CopyViews.hpp:690-800, ViewRemap, and
+// :2512-2650, impl_resize): the common box of the two Views, seen through
+// strided windows, is deep-copied.
 template <class D, class... P>
 inline void resize(View<D, P...>& v, size_t n0 = 0, size_t n1 = 0, size_t n2 = 0, size_t n3 = 0) {
   using V = View<D, P...>;
   V nv(v.label(), n0, n1, n2, n3);
-  if (v.data()) {
-    auto hv  = create_mirror_view_and_copy(HostSpace(), v);
-    auto hnv = create_mirror_view(nv);
-    for (size_t i = 0; i < hnv.span() && i < hv.span(); ++i) hnv.data()[i] = hv.data()[i];
-    deep_copy(nv, hnv);
+  if (v.data() && nv.data()) {
+    LayoutStride from, to;
+    for (unsigned r = 0; r < V::rank; ++r) {
+      const size_t common = v.extent(r) < nv.extent(r) ? v.extent(r) : nv.extent(r);
+      from.dimension[r] = to.dimension[r] = common;
+      from.stride[r]    = v.stride(r);
+      to.stride[r]      = nv.stride(r);
+    }
+    using W = View<typename V::data_type, LayoutStride, typename V::device_type, MemoryTraits<Unmanaged>>;
+    deep_copy(W(nv.data(), to), W(v.data(), from));
   }
   v = nv;
 }

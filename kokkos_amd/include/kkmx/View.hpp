@@ -131,6 +131,8 @@ Impl::ViewCtorProp view_wrap(const Args&... args) {
 
 // ------------------------------------------------------------------ allocation records
 namespace Impl {
+struct SubviewCtor {};
+
 struct SharedAllocRecord {
   std::atomic<int> count{1};
   std::string label;
@@ -398,9 +400,20 @@ class View : public ViewTraits<DataType, Properties...> {
                 std::is_convertible<typename View<D2, P2...>::value_type*, value_type*>::value &&
                 View<D2, P2...>::rank == rank>>
   KOKKOS_INLINE_FUNCTION View(const View<D2, P2...>& o) : m_ptr(o.data()), m_rec(nullptr) {
-    static_assert(std::is_same<typename View<D2, P2...>::array_layout, array_layout>::value || rank <= 1,
+    // Any layout goes to LayoutStride; LayoutStride goes to LayoutLeft or
+    // LayoutRight when its unit-stride dimension really has stride 1
+    // (core/src/View/Kokkos_ViewMapping.hpp, ViewMapping assignment rules).
+    using L2 = typename View<D2, P2...>::array_layout;
+    static_assert(std::is_same<L2, array_layout>::value || std::is_same<array_layout, LayoutStride>::value ||
+                      std::is_same<L2, LayoutStride>::value || rank <= 1,
                   "View assignment requires matching layouts");
     for (unsigned r = 0; r < kR; ++r) { m_ext[r] = o.impl_ext(r); m_str[r] = o.impl_str(r); }
+    if constexpr (std::is_same<L2, LayoutStride>::value && !std::is_same<array_layout, LayoutStride>::value &&
+                  rank >= 1) {
+      const unsigned unit = std::is_same<array_layout, LayoutLeft>::value ? 0 : rank - 1;
+      if (m_ext[unit] > 1 && m_str[unit] != 1) Kokkos::abort("View assignment: the source is not unit-stride where the layout needs it");
+      m_str[unit] = 1;
+    }
 #if !defined(__HIP_DEVICE_COMPILE__)
     if (Impl::g_tracking_enabled && !memory_traits::is_unmanaged) {
       m_rec = o.impl_record();
@@ -461,6 +474,25 @@ class View : public ViewTraits<DataType, Properties...> {
     }
   }
 
+  // A window of another View's allocation (Kokkos::subview): pointer,
+  // extents and strides given; shares and counts the allocation record.
+  KOKKOS_INLINE_FUNCTION View(Impl::SubviewCtor, pointer_type p, const size_t* ext, const size_t* str,
+                              Impl::SharedAllocRecord* rec)
+      : m_ptr(p), m_rec(nullptr) {
+    for (unsigned r = 0; r < kR; ++r) {
+      m_ext[r] = r < rank ? ext[r] : 0;
+      m_str[r] = r < rank ? str[r] : 1;
+    }
+#if !defined(__HIP_DEVICE_COMPILE__)
+    if (rec && Impl::g_tracking_enabled && !memory_traits::is_unmanaged) {
+      m_rec = rec;
+      Impl::record_inc(m_rec);
+    }
+#else
+    (void)rec;
+#endif
+  }
+
   // Scratch-space constructor: carve from a team scratch pad.
   template <class ScratchSpace,
             class = std::enable_if_t<std::is_same<typename ScratchSpace::memory_space,
@@ -508,18 +540,18 @@ class View : public ViewTraits<DataType, Properties...> {
     for (unsigned r = 0; r < rank; ++r) s *= m_ext[r];
     return s;
   }
+  // Strides may exceed the extents below them in every layout (a subview
+  // keeps its parent's strides), so the span comes from the strides.
   KOKKOS_INLINE_FUNCTION constexpr size_t span() const {
-    if constexpr (std::is_same<array_layout, LayoutStride>::value) {
+    if constexpr (rank == 0) {
+      return 1;
+    } else {
       size_t s = 0;
       for (unsigned r = 0; r < rank; ++r) if (m_ext[r]) s += (m_ext[r] - 1) * m_str[r];
       return size() ? s + 1 : 0;
-    } else {
-      return size();
     }
   }
-  KOKKOS_INLINE_FUNCTION constexpr bool span_is_contiguous() const {
-    return !std::is_same<array_layout, LayoutStride>::value || span() == size();
-  }
+  KOKKOS_INLINE_FUNCTION constexpr bool span_is_contiguous() const { return span() == size(); }
   KOKKOS_INLINE_FUNCTION constexpr pointer_type data() const { return m_ptr; }
   KOKKOS_INLINE_FUNCTION constexpr bool is_allocated() const { return m_ptr != nullptr; }
   std::string label() const { return m_rec ? m_rec->label : std::string(); }
@@ -609,8 +641,9 @@ class View : public ViewTraits<DataType, Properties...> {
     }
     if constexpr (rank == 0) m_str[0] = 1;
   }
-  // Offsets: rank 1 is the bare index (contiguous for both layouts); higher
-  // ranks use Horner form so only extents, not strides, are read.
+  // Offsets: rank 1 is the bare index for LayoutLeft/LayoutRight (their
+  // unit-stride dimension); higher ranks multiply the other indices by their
+  // strides, which a subview may have inherited from a larger parent.
   KOKKOS_FORCEINLINE_FUNCTION size_t offset() const { return 0; }
   KOKKOS_FORCEINLINE_FUNCTION size_t offset(size_t i0) const {
     if constexpr (std::is_same<array_layout, LayoutStride>::value) return i0 * m_str[0];
@@ -621,12 +654,12 @@ class View : public ViewTraits<DataType, Properties...> {
     const size_t idx[] = {i0, i1, static_cast<size_t>(rest)...};
     constexpr unsigned n = 2 + sizeof...(Is);
     if constexpr (std::is_same<array_layout, LayoutRight>::value) {
-      size_t o = idx[0];
-      for (unsigned r = 1; r < n; ++r) o = o * m_ext[r] + idx[r];
+      size_t o = idx[n - 1];
+      for (unsigned r = 0; r + 1 < n; ++r) o += idx[r] * m_str[r];
       return o;
     } else if constexpr (std::is_same<array_layout, LayoutLeft>::value) {
-      size_t o = idx[n - 1];
-      for (int r = int(n) - 2; r >= 0; --r) o = o * m_ext[r] + idx[r];
+      size_t o = idx[0];
+      for (unsigned r = 1; r < n; ++r) o += idx[r] * m_str[r];
       return o;
     } else {
       size_t o = 0;
