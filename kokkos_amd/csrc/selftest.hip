@@ -1954,6 +1954,114 @@ KKMX_TEST(resize_keeps_elements_by_index) {
 }
 
 // =========================================================================
+// Team- and thread-level sorts (algorithms/unit_tests/TestNestedSort.hpp):
+// every row sorted by its team (TeamVectorRange) or by one thread's vector
+// lanes (ThreadVectorRange), keys with ties and a non-power-of-two length;
+// keys and values checked against a host run of the same network, so the
+// order of equal keys is checked too.
+// =========================================================================
+struct TeamSortRows {
+  Kokkos::View<int**, Kokkos::LayoutRight, Exec> keys;
+  Kokkos::View<int**, Kokkos::LayoutRight, Exec> vals;
+  KOKKOS_FUNCTION void operator()(const Kokkos::TeamPolicy<Exec>::member_type& m) const {
+    const int r = m.league_rank();
+    Kokkos::Experimental::sort_by_key_team(m, Kokkos::subview(keys, r, Kokkos::ALL),
+                                           Kokkos::subview(vals, r, Kokkos::ALL));
+  }
+};
+struct ThreadSortRows {
+  Kokkos::View<int**, Kokkos::LayoutRight, Exec> keys;
+  Kokkos::View<int**, Kokkos::LayoutRight, Exec> vals;
+  KOKKOS_FUNCTION void operator()(const Kokkos::TeamPolicy<Exec>::member_type& m) const {
+    const int r = m.league_rank() * m.team_size() + m.team_rank();
+    if (r < int(keys.extent(0)))
+      Kokkos::Experimental::sort_by_key_thread(m, Kokkos::subview(keys, r, Kokkos::ALL),
+                                               Kokkos::subview(vals, r, Kokkos::ALL));
+  }
+};
+struct TeamSortDescending {
+  Kokkos::View<double**, Kokkos::LayoutRight, Exec> keys;
+  KOKKOS_FUNCTION void operator()(const Kokkos::TeamPolicy<Exec>::member_type& m) const {
+    Kokkos::Experimental::sort_team(m, Kokkos::subview(keys, m.league_rank(), Kokkos::ALL),
+                                    [](const double a, const double b) { return a > b; });
+  }
+};
+
+// The network on the host, step by step (the device spreads each step's
+// pairs over lanes; the pairs of a step touch disjoint elements).
+void host_bitonic(std::vector<int>& k, std::vector<int>& v) {
+  const size_t n  = k.size();
+  unsigned levels = 0;
+  while ((size_t(1) << levels) < n) ++levels;
+  for (unsigned i = 0; i < levels; ++i)
+    for (unsigned j = 0; j <= i; ++j) {
+      const unsigned sh = i - j;
+      const size_t half = size_t(1) << sh;
+      for (size_t p = 0; p < (size_t(1) << levels) / 2; ++p) {
+        const size_t lo = ((p >> sh) << (sh + 1)) | (p & (half - 1));
+        const size_t hi = j == 0 ? (lo ^ (2 * half - 1)) : (lo | half);
+        if (hi < n && k[hi] < k[lo]) {
+          std::swap(k[lo], k[hi]);
+          std::swap(v[lo], v[hi]);
+        }
+      }
+    }
+}
+
+KKMX_TEST(nested_sort_team_and_thread) {
+  const int rows = 37, n = 100;
+  Kokkos::View<int**, Kokkos::LayoutRight, Exec> keys("keys", rows, n), vals("vals", rows, n);
+  auto hk = Kokkos::create_mirror_view(keys);
+  auto hv = Kokkos::create_mirror_view(vals);
+  auto fill = [&]() {
+    for (int r = 0; r < rows; ++r)
+      for (int i = 0; i < n; ++i) {
+        hk(r, i) = int((int64_t(r + 3) * 7919 * (i + 1)) % 23);  // many ties
+        hv(r, i) = i;
+      }
+    Kokkos::deep_copy(keys, hk);
+    Kokkos::deep_copy(vals, hv);
+  };
+  auto check = [&]() {
+    auto gk = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), keys);
+    auto gv = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), vals);
+    for (int r = 0; r < rows; ++r) {
+      std::vector<int> k(n), v(n);
+      for (int i = 0; i < n; ++i) {
+        k[i] = hk(r, i);
+        v[i] = hv(r, i);
+      }
+      host_bitonic(k, v);
+      for (int i = 0; i < n; ++i) {
+        CHECK_EQ(gk(r, i), k[i]);
+        CHECK_EQ(gv(r, i), v[i]);
+        if (i) CHECK(gk(r, i - 1) <= gk(r, i));
+      }
+    }
+  };
+  fill();
+  Kokkos::parallel_for(Kokkos::TeamPolicy<Exec>(rows, 64), TeamSortRows{keys, vals});
+  check();
+  fill();
+  Kokkos::parallel_for(Kokkos::TeamPolicy<Exec>((rows + 3) / 4, 4, 16), ThreadSortRows{keys, vals});
+  check();
+  // a user comparator: descending doubles
+  Kokkos::View<double**, Kokkos::LayoutRight, Exec> d("d", rows, n);
+  auto hd = Kokkos::create_mirror_view(d);
+  for (int r = 0; r < rows; ++r)
+    for (int i = 0; i < n; ++i) hd(r, i) = double((r * 31 + i * 17) % 97) / 8.0;
+  Kokkos::deep_copy(d, hd);
+  Kokkos::parallel_for(Kokkos::TeamPolicy<Exec>(rows, 32), TeamSortDescending{d});
+  auto gd = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), d);
+  for (int r = 0; r < rows; ++r) {
+    std::vector<double> want(n);
+    for (int i = 0; i < n; ++i) want[i] = hd(r, i);
+    std::sort(want.begin(), want.end(), [](double a, double b) { return a > b; });
+    for (int i = 0; i < n; ++i) CHECK_EQ(gd(r, i), want[i]);
+  }
+}
+
+// =========================================================================
 // BinSort over a sub-range of the keys (algorithms/unit_tests/
 // TestBinSortB.hpp:75-185): distinct unsigned keys binned one per bin,
 // values in a LayoutStride View; and double keys with many per bin sorted

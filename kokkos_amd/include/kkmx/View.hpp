@@ -411,7 +411,10 @@ class View : public ViewTraits<DataType, Properties...> {
     if constexpr (std::is_same<L2, LayoutStride>::value && !std::is_same<array_layout, LayoutStride>::value &&
                   rank >= 1) {
       const unsigned unit = std::is_same<array_layout, LayoutLeft>::value ? 0 : rank - 1;
-      if (m_ext[unit] > 1 && m_str[unit] != 1) Kokkos::abort("View assignment: the source is not unit-stride where the layout needs it");
+#if !defined(__HIP_DEVICE_COMPILE__)
+      if (m_ext[unit] > 1 && m_str[unit] != 1)
+        Kokkos::abort("View assignment: the source is not unit-stride where the layout needs it");
+#endif
       m_str[unit] = 1;
     }
 #if !defined(__HIP_DEVICE_COMPILE__)
@@ -565,7 +568,7 @@ class View : public ViewTraits<DataType, Properties...> {
   }
   KOKKOS_INLINE_FUNCTION size_t impl_ext(unsigned r) const { return m_ext[r < kR ? r : 0]; }
   KOKKOS_INLINE_FUNCTION size_t impl_str(unsigned r) const { return m_str[r < kR ? r : 0]; }
-  Impl::SharedAllocRecord* impl_record() const { return m_rec; }
+  KOKKOS_INLINE_FUNCTION Impl::SharedAllocRecord* impl_record() const { return m_rec; }
   static constexpr size_t static_ext(unsigned r) {
     constexpr size_t se[9] = {Impl::ViewDataAnalysis<DataType>::template static_extent<0>(),
                               Impl::ViewDataAnalysis<DataType>::template static_extent<1>(),
