@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <limits>
+#include <cmath>
 #include <sstream>
 #include <string>
 #include <thread>
@@ -1951,6 +1953,99 @@ void resize_case() {
 KKMX_TEST(resize_keeps_elements_by_index) {
   resize_case<Kokkos::LayoutLeft>();
   resize_case<Kokkos::LayoutRight>();
+}
+
+// =========================================================================
+// Math functions in kernels (core/unit_test/TestMathematicalFunctions.hpp):
+// the correctly rounded ones (sqrt, fma, rounding, fabs, fmin/fmax,
+// copysign, integer abs, min/max/clamp) equal the host's bit for bit; the
+// others agree with the host C library within 4 ulp.  And partition_space
+// (core/unit_test/TestExecSpacePartitioning.hpp): independent instances.
+// =========================================================================
+constexpr int kMathCols = 24;
+struct MathProbe {
+  Kokkos::View<const double*, Exec> x;
+  Kokkos::View<double**, Kokkos::LayoutRight, Exec> out;
+  KOKKOS_FUNCTION void operator()(const int i) const {
+    const double v = x(i), w = x((i + 7) % x.extent_int(0));
+    int c = 0;
+    out(i, c++) = Kokkos::sqrt(Kokkos::fabs(v));
+    out(i, c++) = Kokkos::fma(v, w, 0.25);
+    out(i, c++) = Kokkos::floor(v);
+    out(i, c++) = Kokkos::ceil(v);
+    out(i, c++) = Kokkos::trunc(v);
+    out(i, c++) = Kokkos::round(v);
+    out(i, c++) = Kokkos::fmin(v, w);
+    out(i, c++) = Kokkos::fmax(v, w);
+    out(i, c++) = Kokkos::copysign(v, w);
+    out(i, c++) = double(Kokkos::abs(int(v * 1000)));
+    out(i, c++) = Kokkos::clamp(v, -1.0, 1.5) + Kokkos::min(v, w) - Kokkos::max(v, w);
+    out(i, c++) = double(Kokkos::sqrt(float(Kokkos::abs(v))));  // float overload
+    // within a few ulp of the host
+    out(i, c++) = Kokkos::exp(v);
+    out(i, c++) = Kokkos::log(Kokkos::fabs(v) + 1e-3);
+    out(i, c++) = Kokkos::sin(v);
+    out(i, c++) = Kokkos::cos(v);
+    out(i, c++) = Kokkos::tanh(v);
+    out(i, c++) = Kokkos::pow(Kokkos::fabs(v) + 0.5, w);
+    out(i, c++) = Kokkos::atan2(v, w);
+    out(i, c++) = Kokkos::cbrt(v);
+    out(i, c++) = Kokkos::hypot(v, w);
+    out(i, c++) = Kokkos::erf(v);
+    out(i, c++) = Kokkos::expm1(v);
+    out(i, c++) = Kokkos::log1p(Kokkos::fabs(v));
+  }
+};
+
+struct PartSum {
+  Kokkos::View<int*, Exec> a;
+  KOKKOS_FUNCTION void operator()(const int i, int64_t& s) const { s += a(i); }
+};
+
+KKMX_TEST(math_functions_and_partition_space) {
+  const int n = 4096;
+  Kokkos::View<double*, Exec> x("x", n);
+  auto hx = Kokkos::create_mirror_view(x);
+  for (int i = 0; i < n; ++i) hx(i) = std::ldexp(double((int64_t(i) * 2654435761u) % 100003) - 50001.0, -13);
+  Kokkos::deep_copy(x, hx);
+  Kokkos::View<double**, Kokkos::LayoutRight, Exec> out("out", n, kMathCols);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), MathProbe{x, out});
+  auto g = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+  for (int i = 0; i < n; ++i) {
+    const double v = hx(i), w = hx((i + 7) % n);
+    const double exact[] = {std::sqrt(std::fabs(v)), std::fma(v, w, 0.25), std::floor(v), std::ceil(v),
+                            std::trunc(v), std::round(v), std::fmin(v, w), std::fmax(v, w), std::copysign(v, w),
+                            double(std::abs(int(v * 1000))),
+                            std::min(std::max(v, -1.0), 1.5) + std::min(v, w) - std::max(v, w),
+                            double(std::sqrt(float(std::fabs(v))))};
+    for (int c = 0; c < 12; ++c) CHECK_EQ(g(i, c), exact[c]);
+    const double near[] = {std::exp(v), std::log(std::fabs(v) + 1e-3), std::sin(v), std::cos(v), std::tanh(v),
+                           std::pow(std::fabs(v) + 0.5, w), std::atan2(v, w), std::cbrt(v), std::hypot(v, w),
+                           std::erf(v), std::expm1(v), std::log1p(std::fabs(v))};
+    for (int c = 0; c < 12; ++c) {
+      const double a = g(i, 12 + c), b = near[c];
+      CHECK(std::fabs(a - b) <= 4 * std::numeric_limits<double>::epsilon() * std::fabs(b) + 1e-300);
+    }
+  }
+  CHECK_EQ(Kokkos::numbers::pi, M_PI);
+  CHECK_EQ(Kokkos::numbers::e, M_E);
+
+  // two partitions: own streams, independent work, each fenced on its own
+  auto parts = Kokkos::Experimental::partition_space(Exec(), 1, 1);
+  CHECK(parts[0].stream() != parts[1].stream());
+  CHECK(parts[0].stream() != Exec().stream());
+  Kokkos::View<int*, Exec> a("a", 1 << 20), b("b", 1 << 20);
+  Kokkos::deep_copy(parts[0], a, 3);
+  Kokkos::deep_copy(parts[1], b, 5);
+  parts[0].fence();
+  parts[1].fence();
+  int64_t sa = 0, sb = 0;
+  Kokkos::parallel_reduce(Kokkos::RangePolicy<Exec>(parts[0], 0, 1 << 20), PartSum{a}, sa);
+  Kokkos::parallel_reduce(Kokkos::RangePolicy<Exec>(parts[1], 0, 1 << 20), PartSum{b}, sb);
+  CHECK_EQ(sa, int64_t(3) << 20);
+  CHECK_EQ(sb, int64_t(5) << 20);
+  auto many = Kokkos::Experimental::partition_space(Exec(), std::vector<int>{1, 2, 3});
+  CHECK_EQ(int(many.size()), 3);
 }
 
 // =========================================================================

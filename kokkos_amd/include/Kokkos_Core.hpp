@@ -12,7 +12,9 @@
 
 #include <Kokkos_Macros.hpp>
 #include <kkmx/Core_fwd.hpp>
+#include <kkmx/Math.hpp>
 #include <kkmx/Spaces.hpp>
+#include <kkmx/PartitionSpace.hpp>
 #include <kkmx/Atomic.hpp>
 #include <kkmx/View.hpp>
 #include <kkmx/Subview.hpp>
