@@ -1956,6 +1956,98 @@ KKMX_TEST(resize_keeps_elements_by_index) {
 }
 
 // =========================================================================
+// A typical application written with lambdas against the default spaces:
+// Range/MDRange/Team dispatch, reducers, a scan with total, team scratch,
+// atomics, subviews, mirrors, Kokkos::Array, sort, ScatterView, a std
+// algorithm and resize; every result is checked against its closed form.
+// =========================================================================
+KKMX_TEST(typical_application_with_lambdas) {
+  using ExecSpace = Kokkos::DefaultExecutionSpace;
+  const int N = 1000;
+  Kokkos::View<double*> a("a", N), b("b", N);
+  Kokkos::View<double* [3]> pos("pos", N);
+  Kokkos::View<double**, Kokkos::LayoutRight> m("m", N, 8);
+  Kokkos::parallel_for("init", N, KOKKOS_LAMBDA(const int i) {
+    a(i)      = i;
+    b(i)      = 2.0 * i;
+    pos(i, 0) = i;
+    pos(i, 2) = -i;
+  });
+  Kokkos::parallel_for(Kokkos::RangePolicy<ExecSpace, Kokkos::Schedule<Kokkos::Dynamic>>(0, N),
+                       KOKKOS_LAMBDA(const int i) { Kokkos::atomic_add(&a(i % 10), 1.0); });
+  double sum = 0;
+  Kokkos::parallel_reduce("dot", N, KOKKOS_LAMBDA(const int i, double& s) { s += a(i) * b(i); }, sum);
+  CHECK_EQ(sum, 665676000.0);  // 2 * sum i^2 + 200 * sum_{i<10} i
+  double mx = 0;
+  Kokkos::parallel_reduce(
+      N, KOKKOS_LAMBDA(const int i, double& v) { if (b(i) > v) v = b(i); }, Kokkos::Max<double>(mx));
+  CHECK_EQ(mx, 1998.0);
+  Kokkos::MinLoc<double, int>::value_type ml;
+  Kokkos::parallel_reduce(
+      N,
+      KOKKOS_LAMBDA(const int i, Kokkos::MinLoc<double, int>::value_type& v) {
+        if (b(i) < v.val) {
+          v.val = b(i);
+          v.loc = i;
+        }
+      },
+      Kokkos::MinLoc<double, int>(ml));
+  CHECK_EQ(ml.loc, 0);
+  Kokkos::View<int*> cnt("cnt", N), off("off", N);
+  Kokkos::deep_copy(cnt, 1);
+  int total = 0;
+  Kokkos::parallel_scan(
+      "offsets", N,
+      KOKKOS_LAMBDA(const int i, int& u, const bool final) {
+        if (final) off(i) = u;
+        u += cnt(i);
+      },
+      total);
+  CHECK_EQ(total, N);
+  Kokkos::parallel_for(Kokkos::MDRangePolicy<Kokkos::Rank<2>>({0, 0}, {N, 8}, {64, 4}),
+                       KOKKOS_LAMBDA(const int i, const int j) { m(i, j) = i + j; });
+  using team_t      = Kokkos::TeamPolicy<>::member_type;
+  using scratch_t   = Kokkos::View<double*, ExecSpace::scratch_memory_space>;
+  const int scratch = int(scratch_t::shmem_size(64));
+  Kokkos::parallel_for(Kokkos::TeamPolicy<>(N / 10, Kokkos::AUTO).set_scratch_size(0, Kokkos::PerTeam(scratch)),
+                       KOKKOS_LAMBDA(const team_t& t) {
+                         scratch_t s(t.team_scratch(0), 64);
+                         Kokkos::parallel_for(Kokkos::TeamThreadRange(t, 10), [&](const int k) { s(k) = k; });
+                         t.team_barrier();
+                         double r = 0;
+                         Kokkos::parallel_reduce(
+                             Kokkos::TeamThreadRange(t, 10), [&](const int k, double& u) { u += s(k); }, r);
+                         Kokkos::single(Kokkos::PerTeam(t), [&]() { a(t.league_rank()) += r; });
+                       });
+  auto row = Kokkos::subview(m, 3, Kokkos::ALL);
+  auto hr  = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), row);
+  for (int j = 0; j < 8; ++j) CHECK_EQ(hr(j), 3.0 + j);
+  auto hoff = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), off);
+  CHECK_EQ(hoff(N - 1), N - 1);
+  Kokkos::Array<double, 3> arr = {1, 2, 3};
+  CHECK_EQ(arr[1], 2.0);
+  Kokkos::sort(a);
+  auto ha = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), a);
+  for (int i = 1; i < N; ++i) CHECK(ha(i - 1) <= ha(i));
+  Kokkos::Experimental::ScatterView<double*> sv(b);
+  Kokkos::parallel_for(N, KOKKOS_LAMBDA(const int i) {
+    auto acc = sv.access();
+    acc(i % 7) += 1.0;
+  });
+  Kokkos::Experimental::contribute(b, sv);
+  auto hb = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), b);
+  CHECK_EQ(hb(0), 0.0 + 143);  // 1000 = 7 * 142 + 6: indices 0..5 get 143 hits
+  CHECK_EQ(hb(6), 12.0 + 142);
+  namespace KE = Kokkos::Experimental;
+  const double r2 = KE::reduce(ExecSpace(), a);
+  CHECK_EQ(r2, 505000.0);  // 499500 + 10 * 100 atomics + 100 teams * 45
+  Kokkos::resize(a, 2 * N);
+  auto hz = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), a);
+  CHECK_EQ(hz(N - 1), ha(N - 1));
+  CHECK_EQ(hz(2 * N - 1), 0.0);
+}
+
+// =========================================================================
 // Math functions in kernels (core/unit_test/TestMathematicalFunctions.hpp):
 // the correctly rounded ones (sqrt, fma, rounding, fabs, fmin/fmax,
 // copysign, integer abs, min/max/clamp) equal the host's bit for bit; the

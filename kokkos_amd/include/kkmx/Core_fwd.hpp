@@ -23,6 +23,19 @@
 
 namespace Kokkos {
 
+// ---------------------------------------------------------------- printf
+// Kokkos::printf (reference core/src/Kokkos_Printf.hpp): C printf on the
+// host and from kernels, returning nothing; a lone format string is printed
+// through "%s" so that a stray '%' in it is harmless.
+template <class... Args>
+KOKKOS_FORCEINLINE_FUNCTION void printf(const char* format, Args... args) {
+  if constexpr (sizeof...(Args) == 0) {
+    ::printf("%s", format);
+  } else {
+    ::printf(format, args...);
+  }
+}
+
 // ---------------------------------------------------------------- abort
 // Host: print and abort the process (reference core/src/impl/Kokkos_Error.cpp
 // host_abort).  Device: trap the wavefront; the fault surfaces at the next
