@@ -1956,6 +1956,56 @@ KKMX_TEST(resize_keeps_elements_by_index) {
 }
 
 // =========================================================================
+// kokkos_malloc / kokkos_realloc / kokkos_free (core/unit_test/
+// TestMalloc.hpp) and a runtime-length reduction into a C array
+// (TestReduce.hpp:123-140, the array result decays to a pointer).
+// =========================================================================
+struct ArraySums {
+  using value_type = double[];
+  const unsigned value_count = 3;
+  Kokkos::View<double*, Exec> x;
+  KOKKOS_FUNCTION void operator()(const int i, double* s) const {
+    s[0] += x(i);
+    s[1] += 1;
+    s[2] += x(i) * x(i);
+  }
+  KOKKOS_FUNCTION void init(double* s) const { s[0] = s[1] = s[2] = 0; }
+  KOKKOS_FUNCTION void join(double* a, const double* b) const {
+    for (int k = 0; k < 3; ++k) a[k] += b[k];
+  }
+};
+struct FillRaw {
+  int* p;
+  KOKKOS_FUNCTION void operator()(const int i) const { p[i] = 3 * i; }
+};
+
+KKMX_TEST(raw_allocations_and_array_results) {
+  int* p = static_cast<int*>(Kokkos::kokkos_malloc<>("raw", 1000 * sizeof(int)));
+  CHECK(p != nullptr);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, 1000), FillRaw{p});
+  Kokkos::fence();
+  p = static_cast<int*>(Kokkos::kokkos_realloc<>(p, 2000 * sizeof(int)));  // keeps the first 1000
+  std::vector<int> h(1000);
+  KKMX_SAFE_CALL(mx_memcpy(h.data(), p, 1000 * sizeof(int)));
+  for (int i = 0; i < 1000; ++i) CHECK_EQ(h[i], 3 * i);
+  Kokkos::kokkos_free<>(p);
+  void* hp = Kokkos::kokkos_malloc<Kokkos::HostSpace>(64);
+  hp       = Kokkos::kokkos_realloc<Kokkos::HostSpace>(hp, 128);
+  Kokkos::kokkos_free<Kokkos::HostSpace>(hp);
+
+  const int n = 4096;
+  Kokkos::View<double*, Exec> x("x", n);
+  auto hx = Kokkos::create_mirror_view(x);
+  for (int i = 0; i < n; ++i) hx(i) = i % 100;
+  Kokkos::deep_copy(x, hx);
+  double sums[3];
+  Kokkos::parallel_reduce(Kokkos::RangePolicy<Exec>(0, n), ArraySums{3, x}, sums);
+  CHECK_EQ(sums[0], 202560.0);    // 40 * 4950 + sum 0..95
+  CHECK_EQ(sums[1], double(n));
+  CHECK_EQ(sums[2], 13424320.0);  // 40 * 328350 + sum of squares 0..95
+}
+
+// =========================================================================
 // A typical application written with lambdas against the default spaces:
 // Range/MDRange/Team dispatch, reducers, a scan with total, team scratch,
 // atomics, subviews, mirrors, Kokkos::Array, sort, ScatterView, a std

@@ -29,6 +29,7 @@
 #include <kkmx/KernelTeam.hpp>
 #include <kkmx/Parallel.hpp>
 #include <kkmx/CopyViews.hpp>
+#include <kkmx/KokkosMalloc.hpp>
 #include <kkmx/Graph.hpp>
 #include <kkmx/SpacesImpl.hpp>
 

@@ -1,0 +1,8 @@
+// Kokkos_ScratchSpace.hpp under the reference's name (core/src/Kokkos_ScratchSpace.hpp): its contents are
+// part of this execution space's Kokkos_Core.hpp, which this header includes.
+#ifndef KKMX_FWD_KOKKOS_SCRATCHSPACE_HPP
+#define KKMX_FWD_KOKKOS_SCRATCHSPACE_HPP
+
+#include <Kokkos_Core.hpp>
+
+#endif

@@ -108,11 +108,23 @@ struct ReduceReturn<R, std::enable_if_t<std::is_pointer<remove_cvref_t<R>>::valu
   static bool fences(value_type*) { return true; }
   static unsigned count(value_type*) { return 0; }
 };
+// Host array (runtime-length values written into a C array, which the
+// reference receives as a decayed pointer).
+template <class R>
+struct ReduceReturn<R, std::enable_if_t<std::is_array<remove_cvref_t<R>>::value>> {
+  using reducer    = void;
+  using value_type = std::remove_extent_t<remove_cvref_t<R>>;
+  using mem        = HostSpace;
+  static value_type* ptr(value_type* p) { return p; }
+  static bool fences(value_type*) { return true; }
+  static unsigned count(value_type*) { return unsigned(std::extent<remove_cvref_t<R>>::value); }
+};
 // Host scalar / struct.
 template <class R>
 struct ReduceReturn<R, std::enable_if_t<!is_reducer<remove_cvref_t<R>>::value &&
                                         !is_view<remove_cvref_t<R>>::value &&
-                                        !std::is_pointer<remove_cvref_t<R>>::value>> {
+                                        !std::is_pointer<remove_cvref_t<R>>::value &&
+                                        !std::is_array<remove_cvref_t<R>>::value>> {
   using reducer    = void;
   using value_type = remove_cvref_t<R>;
   using mem        = HostSpace;
