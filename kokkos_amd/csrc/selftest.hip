@@ -1956,6 +1956,87 @@ KKMX_TEST(resize_keeps_elements_by_index) {
 }
 
 // =========================================================================
+// Kokkos::complex (core/unit_test/TestComplex.hpp): arithmetic and the
+// functions in a kernel against std::complex on the host (within a few ulp
+// of the larger part), a Sum reduction of complex values and complex
+// atomic_add (integer-valued parts, so the sums are exact).
+// =========================================================================
+using cplx = Kokkos::complex<double>;
+struct ComplexProbe {
+  Kokkos::View<const cplx*, Exec> z;
+  Kokkos::View<cplx**, Kokkos::LayoutRight, Exec> out;
+  KOKKOS_FUNCTION void operator()(const int i) const {
+    const cplx a = z(i), b = z((i + 3) % z.extent_int(0));
+    int c         = 0;
+    out(i, c++)   = a + b;
+    out(i, c++)   = a - 2.0 * b;
+    out(i, c++)   = a * b;
+    out(i, c++)   = a / b;
+    out(i, c++)   = Kokkos::conj(a) + cplx(Kokkos::abs(a), Kokkos::arg(a));
+    out(i, c++)   = Kokkos::exp(a);
+    out(i, c++)   = Kokkos::log(a);
+    out(i, c++)   = Kokkos::sqrt(a);
+    out(i, c++)   = Kokkos::pow(a, b);
+    out(i, c++)   = Kokkos::sin(a) + Kokkos::cosh(b);
+  }
+};
+struct ComplexSum {
+  Kokkos::View<const cplx*, Exec> z;
+  KOKKOS_FUNCTION void operator()(const int i, cplx& s) const { s += z(i); }
+};
+struct ComplexAtomic {
+  Kokkos::View<cplx*, Exec> bins;
+  KOKKOS_FUNCTION void operator()(const int i) const { Kokkos::atomic_add(&bins(i % 5), cplx(1.0, double(i % 3))); }
+};
+
+KKMX_TEST(complex_arithmetic_reduce_atomic) {
+  const int n = 512;
+  Kokkos::View<cplx*, Exec> z("z", n);
+  auto hz = Kokkos::create_mirror_view(z);
+  for (int i = 0; i < n; ++i) hz(i) = cplx(0.25 + (i % 37) / 8.0, ((i * 7) % 29) / 9.0 - 1.5);
+  Kokkos::deep_copy(z, hz);
+  Kokkos::View<cplx**, Kokkos::LayoutRight, Exec> out("out", n, 10);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), ComplexProbe{z, out});
+  auto g = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+  for (int i = 0; i < n; ++i) {
+    const std::complex<double> a(hz(i).real(), hz(i).imag());
+    const std::complex<double> b(hz((i + 3) % n).real(), hz((i + 3) % n).imag());
+    const std::complex<double> want[] = {a + b, a - 2.0 * b, a * b, a / b,
+                                         std::conj(a) + std::complex<double>(std::abs(a), std::arg(a)),
+                                         std::exp(a), std::log(a), std::sqrt(a), std::pow(a, b),
+                                         std::sin(a) + std::cosh(b)};
+    for (int c = 0; c < 10; ++c) {
+      const double scale = std::max(std::abs(want[c].real()), std::abs(want[c].imag())) + 1e-300;
+      CHECK(std::abs(g(i, c).real() - want[c].real()) <= 1e-14 * scale);
+      CHECK(std::abs(g(i, c).imag() - want[c].imag()) <= 1e-14 * scale);
+    }
+  }
+  // exact sums on integer-valued parts
+  Kokkos::View<cplx*, Exec> w("w", n);
+  auto hw = Kokkos::create_mirror_view(w);
+  cplx want(0.0, 0.0);
+  for (int i = 0; i < n; ++i) {
+    hw(i) = cplx(double(i), double(-2 * i));
+    want += hw(i);
+  }
+  Kokkos::deep_copy(w, hw);
+  cplx s;
+  Kokkos::parallel_reduce(Kokkos::RangePolicy<Exec>(0, n), ComplexSum{w}, s);
+  CHECK(s == want);
+  Kokkos::View<cplx*, Exec> bins("bins", 5);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, 3000), ComplexAtomic{bins});
+  auto hb = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), bins);
+  for (int k = 0; k < 5; ++k) {
+    double re = 0, im = 0;
+    for (int i = k; i < 3000; i += 5) {
+      re += 1.0;
+      im += double(i % 3);
+    }
+    CHECK(hb(k) == cplx(re, im));
+  }
+}
+
+// =========================================================================
 // kokkos_malloc / kokkos_realloc / kokkos_free (core/unit_test/
 // TestMalloc.hpp) and a runtime-length reduction into a C array
 // (TestReduce.hpp:123-140, the array result decays to a pointer).

@@ -13,6 +13,7 @@
 #include <Kokkos_Macros.hpp>
 #include <kkmx/Core_fwd.hpp>
 #include <kkmx/Math.hpp>
+#include <kkmx/Complex.hpp>
 #include <kkmx/Spaces.hpp>
 #include <kkmx/PartitionSpace.hpp>
 #include <kkmx/Atomic.hpp>

@@ -14,8 +14,8 @@
 //  * every other operator and every other 4/8-byte type: a compare-and-swap
 //    loop on the bit pattern.
 //  * 1- and 2-byte types: CAS on the aligned 4-byte word that contains them.
-//  * larger types: a hashed lock array in device memory, taken and released
-//    inside one loop iteration so lanes of one wavefront cannot deadlock
+//  * larger types: a hashed lock array in device memory; the lanes of a
+//    wavefront take turns so no two of them ever spin for a lock together
 //    (the reference uses a 2^17-entry lock hash, desul Lock_Array_HIP.hpp:76-116).
 // Host: GCC/Clang __atomic builtins with the same rules.
 #ifndef KKMX_ATOMIC_HPP
@@ -138,17 +138,24 @@ KOKKOS_INLINE_FUNCTION T fetch_oper(T* p, const T& val, Op op) {
     }
   } else {
 #if defined(__HIP_DEVICE_COMPILE__)
-    int* lock = &g_atomic_locks[lock_index(p)];
+    // The lanes of a wavefront take turns, lowest lane first: only the lane
+    // whose turn it is spins for the lock, updates and releases it.  A spin
+    // shared by several lanes would let the compiler move the update and
+    // the release after the spin loop, where a lane holding the lock cannot
+    // release it while another lane of its wavefront still spins for it.
+    int* lock           = &g_atomic_locks[lock_index(p)];
     T result{};
-    bool done = false;
-    while (!done) {
-      if (__hip_atomic_exchange(lock, 1, __ATOMIC_ACQUIRE,
-                                __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    uint64_t pending    = __builtin_amdgcn_ballot_w64(true);
+    while (pending) {
+      if (lane == unsigned(__builtin_ctzll(pending))) {
+        while (__hip_atomic_exchange(lock, 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        }
         result = *p;
         *p     = op(result, val);
         __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        done = true;
       }
+      pending &= pending - 1;
     }
     return result;
 #else
