@@ -1956,6 +1956,78 @@ KKMX_TEST(resize_keeps_elements_by_index) {
 }
 
 // =========================================================================
+// Nested MD ranges (core/unit_test/TestTeamMDRange.hpp): every point of a
+// per-team box visited exactly once at each level (threads, threads x
+// vector, one thread's vector lanes), both iteration directions, and the
+// sum reduction received by every lane.
+// =========================================================================
+struct TeamMDProbe {
+  Kokkos::View<int****, Kokkos::LayoutRight, Exec> hits;  // (team, i, j, k)
+  Kokkos::View<int**, Kokkos::LayoutRight, Exec> sums;     // (team, level)
+  KOKKOS_FUNCTION void operator()(const Kokkos::TeamPolicy<Exec>::member_type& t) const {
+    using M = Kokkos::TeamPolicy<Exec>::member_type;
+    const int r = t.league_rank();
+    const int a = hits.extent_int(1), b = hits.extent_int(2), c = hits.extent_int(3);
+    // a thread-level body runs on each of the thread's vector lanes: count once
+    Kokkos::parallel_for(Kokkos::TeamThreadMDRange<Kokkos::Rank<3>, M>(t, a, b, c),
+                         [&](const int i, const int j, const int k) {
+                           Kokkos::single(Kokkos::PerThread(t), [&]() { Kokkos::atomic_add(&hits(r, i, j, k), 1); });
+                         });
+    t.team_barrier();
+    Kokkos::parallel_for(
+        Kokkos::TeamVectorMDRange<Kokkos::Rank<3, Kokkos::Iterate::Right>, M>(t, a, b, c),
+        [&](const int i, const int j, const int k) { Kokkos::atomic_add(&hits(r, i, j, k), 10); });
+    t.team_barrier();
+    if (t.team_rank() == 0)
+      Kokkos::parallel_for(Kokkos::ThreadVectorMDRange<Kokkos::Rank<2>, M>(t, a, b), [&](const int i, const int j) {
+        for (int k = 0; k < c; ++k) Kokkos::atomic_add(&hits(r, i, j, k), 100);
+      });
+    t.team_barrier();
+    int s0 = 0, s1 = 0, s2 = 0;
+    Kokkos::parallel_reduce(
+        Kokkos::TeamThreadMDRange<Kokkos::Rank<3>, M>(t, a, b, c),
+        [&](const int i, const int j, const int k, int& u) { u += i * 100 + j * 10 + k; }, s0);
+    Kokkos::parallel_reduce(
+        Kokkos::TeamVectorMDRange<Kokkos::Rank<2, Kokkos::Iterate::Right>, M>(t, a, b),
+        [&](const int i, const int j, int& u) { u += i * 10 + j; }, s1);
+    Kokkos::parallel_reduce(Kokkos::ThreadVectorMDRange<Kokkos::Rank<2>, M>(t, b, c),
+                            [&](const int j, const int k, int& u) { u += j + k; }, s2);
+    // every lane holds the sums; lane 0 of each thread records them
+    Kokkos::single(Kokkos::PerThread(t), [&]() {
+      Kokkos::atomic_max(&sums(r, 0), s0);
+      Kokkos::atomic_max(&sums(r, 1), s1);
+      Kokkos::atomic_max(&sums(r, 2), s2);
+    });
+  }
+};
+
+KKMX_TEST(team_md_ranges) {
+  const int L = 6, A = 5, B = 7, C = 3;
+  Kokkos::View<int****, Kokkos::LayoutRight, Exec> hits("hits", L, A, B, C);
+  Kokkos::View<int**, Kokkos::LayoutRight, Exec> sums("sums", L, 3);
+  Kokkos::deep_copy(sums, 0);
+  Kokkos::parallel_for(Kokkos::TeamPolicy<Exec>(L, 8, 8), TeamMDProbe{hits, sums});
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), hits);
+  auto s = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), sums);
+  int w0 = 0, w1 = 0, w2 = 0;
+  for (int i = 0; i < A; ++i)
+    for (int j = 0; j < B; ++j)
+      for (int k = 0; k < C; ++k) w0 += i * 100 + j * 10 + k;
+  for (int i = 0; i < A; ++i)
+    for (int j = 0; j < B; ++j) w1 += i * 10 + j;
+  for (int j = 0; j < B; ++j)
+    for (int k = 0; k < C; ++k) w2 += j + k;
+  for (int r = 0; r < L; ++r) {
+    for (int i = 0; i < A; ++i)
+      for (int j = 0; j < B; ++j)
+        for (int k = 0; k < C; ++k) CHECK_EQ(h(r, i, j, k), 111);
+    CHECK_EQ(s(r, 0), w0);
+    CHECK_EQ(s(r, 1), w1);
+    CHECK_EQ(s(r, 2), w2);  // each thread reduced the whole box on its own vector lanes
+  }
+}
+
+// =========================================================================
 // Kokkos::complex (core/unit_test/TestComplex.hpp): arithmetic and the
 // functions in a kernel against std::complex on the host (within a few ulp
 // of the larger part), a Sum reduction of complex values and complex

@@ -28,6 +28,7 @@
 #include <kkmx/HostExec.hpp>
 #include <kkmx/KernelMDRange.hpp>
 #include <kkmx/KernelTeam.hpp>
+#include <kkmx/TeamMDRange.hpp>
 #include <kkmx/Parallel.hpp>
 #include <kkmx/CopyViews.hpp>
 #include <kkmx/KokkosMalloc.hpp>
