@@ -19,9 +19,11 @@
 // into a digit-sorted copy of the tile in LDS, and writes that copy out so
 // each digit's keys leave as one contiguous run, at offsets from a
 // digit-major exclusive scan of the per-segment histograms: every pass
-// reads its keys twice and writes them once, stably.  Signed and
-// floating-point keys are mapped to unsigned order on the first pass and
-// back on the last.  As in the reference's HIP path (thrust::sort, a radix
+// reads its keys twice and writes them once, stably.  The first histogram
+// also folds the AND and OR of all keys, and passes whose digit is the same
+// in every key (the high digits of small-range keys) are skipped.  Signed
+// and floating-point keys are mapped to unsigned order on the first pass
+// that runs and back on the last.  As in the reference's HIP path (thrust::sort, a radix
 // sort for arithmetic keys: algorithms/src/sorting/impl/
 // Kokkos_SortImpl.hpp:222-240), -0.0 and +0.0 compare equal and keep their
 // input order, and NaNs go below -inf (sign bit set) or above +inf.  Other
@@ -225,10 +227,11 @@ struct RadixKey {
   // Digit of an encoded key.  -0.0 (encoded ~top) takes the digits of +0.0
   // (encoded top), so the two compare equal and keep their input order, as
   // under operator<; the key itself keeps its bits.
-  KOKKOS_FORCEINLINE_FUNCTION static unsigned digit(U u, int shift) {
-    if constexpr (std::is_floating_point<T>::value) u = u == U(~top) ? top : u;
-    return unsigned(u >> shift) & 255u;
+  KOKKOS_FORCEINLINE_FUNCTION static U canon(U u) {
+    if constexpr (std::is_floating_point<T>::value) return u == U(~top) ? top : u;
+    else return u;
   }
+  KOKKOS_FORCEINLINE_FUNCTION static unsigned digit(U u, int shift) { return unsigned(canon(u) >> shift) & 255u; }
   KOKKOS_FORCEINLINE_FUNCTION static T from(U u) {
     if constexpr (std::is_floating_point<T>::value) u = (u & top) ? (u ^ top) : ~u;
     else if constexpr (std::is_signed<T>::value) u ^= top;
@@ -246,16 +249,27 @@ constexpr int kRadixTile  = kRadixLanes * kRadixItems;
 // (the last segment may be short).  k_radix_hist counts digit
 // R::digit(key, shift) over its segment, in one LDS histogram per wavefront,
 // and stores the counts digit-major: hist[d * nblocks + block].
+//
+// The First variant also folds the bitwise AND and OR of every (canonical)
+// key into summary[0] and summary[1]: digits where the two agree are the
+// same in every key, and their passes move nothing.
 template <class T, bool First>
 __global__ __launch_bounds__(256) void k_radix_hist(const void* in, const int64_t n, const int shift, uint32_t* hist,
-                                                    const int64_t nblocks, const int64_t tpb) {
+                                                    const int64_t nblocks, const int64_t tpb,
+                                                    typename RadixKey<T>::U* summary) {
   using R = RadixKey<T>;
   using U = typename R::U;
   constexpr int NW = kRadixLanes / 64;
   __shared__ uint32_t h[NW][256];
+  __shared__ U s_and, s_or;
   const int t = threadIdx.x, w = t >> 6;
 #pragma unroll
   for (int k = 0; k < NW; ++k) h[k][t] = 0;
+  if (First && t == 0) {
+    s_and = ~U(0);
+    s_or  = U(0);
+  }
+  U all_and = ~U(0), any_or = U(0);
   __syncthreads();
   const int64_t first = int64_t(blockIdx.x) * tpb * kRadixTile;
   const int64_t last  = first + tpb * kRadixTile < n ? first + tpb * kRadixTile : n;
@@ -268,14 +282,29 @@ __global__ __launch_bounds__(256) void k_radix_hist(const void* in, const int64_
       if (i < last) u[r] = First ? R::to(static_cast<const T*>(in)[i]) : static_cast<const U*>(in)[i];
     }
 #pragma unroll
-    for (int r = 0; r < kRadixItems; ++r)
-      if (tb + int64_t(r) * kRadixLanes + t < last) atomicAdd(&h[w][R::digit(u[r], shift)], 1u);
+    for (int r = 0; r < kRadixItems; ++r) {
+      if (tb + int64_t(r) * kRadixLanes + t < last) {
+        atomicAdd(&h[w][R::digit(u[r], shift)], 1u);
+        if (First) {
+          all_and &= R::canon(u[r]);
+          any_or |= R::canon(u[r]);
+        }
+      }
+    }
+  }
+  if (First) {
+    atomicAnd(&s_and, all_and);
+    atomicOr(&s_or, any_or);
   }
   __syncthreads();
   uint32_t c = 0;
 #pragma unroll
   for (int k = 0; k < NW; ++k) c += h[k][t];
   hist[int64_t(t) * nblocks + blockIdx.x] = c;
+  if (First && t == 0) {
+    atomicAnd(&summary[0], s_and);
+    atomicOr(&summary[1], s_or);
+  }
 }
 
 // Stable placement of a segment's keys at their digit's offsets, one tile at
@@ -439,6 +468,11 @@ inline void radix_workspace_release() {
   w.hooked = false;
 }
 
+struct IotaU32 {
+  uint32_t* p;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i) const { p[i] = uint32_t(i); }
+};
+
 template <class T, class PT, bool Iota>
 const void* radix_scatter_kernel(bool first, bool last) {
   if (first) return last ? reinterpret_cast<const void*>(&k_radix_scatter<T, true, true, PT, Iota>)
@@ -480,7 +514,7 @@ void radix_sort(const MI355X& exec, T* keys, int64_t n, PT* payload = nullptr) {
   const size_t table_bytes = size_t(256 * nblk) * sizeof(uint32_t);
   constexpr bool pairs     = !std::is_same<PT, NoPayload>::value;
   const size_t idx_bytes   = pairs ? (size_t(n) * sizeof(PT) + 255) & ~size_t(255) : 0;
-  const size_t need        = key_bytes + 2 * table_bytes + idx_bytes;
+  const size_t need        = key_bytes + 2 * table_bytes + idx_bytes + 256;
   RadixWorkspace& ws       = radix_workspace();
   std::lock_guard<std::mutex> lock(ws.mtx);
   if (ws.bytes < need) {
@@ -499,26 +533,59 @@ void radix_sort(const MI355X& exec, T* keys, int64_t n, PT* payload = nullptr) {
   uint32_t* hp = reinterpret_cast<uint32_t*>(base + key_bytes);
   uint32_t* op = reinterpret_cast<uint32_t*>(base + key_bytes + table_bytes);
   PT* ibuf[2]   = {payload, pairs ? reinterpret_cast<PT*>(base + key_bytes + 2 * table_bytes) : nullptr};
-  for (int p = 0; p < passes; ++p) {
-    const void* in   = buf[p & 1];
-    void* out        = buf[(p + 1) & 1];
+  U* summary    = reinterpret_cast<U*>(base + key_bytes + 2 * table_bytes + idx_bytes);
+  int64_t nb = nblk, tp = tpb;
+
+  // Pass 0's histogram, with the AND / OR summary of all keys; passes whose
+  // digit is the same in every key are identity permutations and are skipped.
+  const U init[2] = {~U(0), U(0)};
+  KKMX_SAFE_CALL(mx_memcpy_async(summary, init, sizeof(init), exec.stream()));
+  const void* keys_in = keys;
+  int shift0          = 0;
+  mx_launch_kernel(reinterpret_cast<const void*>(&k_radix_hist<T, true>), unsigned(nblk), kRadixLanes, 0,
+                   exec.stream(), keys_in, n, shift0, hp, nb, tp, summary);
+  U got[2] = {0, 0};
+  KKMX_SAFE_CALL(mx_memcpy_async(got, summary, sizeof(got), exec.stream()));
+  exec.fence("Kokkos::sort: key summary");
+  const U varying = got[0] ^ got[1];
+  int run[8], nrun = 0;
+  for (int p = 0; p < passes; ++p)
+    if ((varying >> (8 * p)) & U(255)) run[nrun++] = p;
+
+  if (nrun == 0) {  // every key has the same digits: the keys are sorted
+    if constexpr (Iota)
+      parallel_for("Kokkos::sort::identity_permutation", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, n),
+                   IotaU32{payload});
+    exec.fence("Kokkos::sort: nothing to move");
+    return;
+  }
+  for (int k = 0; k < nrun; ++k) {
+    const int p      = run[k];
+    const void* in   = buf[k & 1];
+    void* out        = buf[(k + 1) & 1];
     int shift        = 8 * p;
-    const bool first = p == 0, last = p == passes - 1;
-    const void* kh   = first ? reinterpret_cast<const void*>(&k_radix_hist<T, true>)
+    const bool first = k == 0, last = k == nrun - 1;
+    if (!(first && p == 0)) {  // pass 0's histogram is already there
+      const void* kh = first ? reinterpret_cast<const void*>(&k_radix_hist<T, true>)
                              : reinterpret_cast<const void*>(&k_radix_hist<T, false>);
-    int64_t nb = nblk, tp = tpb;
-    mx_launch_kernel(kh, unsigned(nblk), kRadixLanes, 0, exec.stream(), in, n, shift, hp, nb, tp);
+      mx_launch_kernel(kh, unsigned(nblk), kRadixLanes, 0, exec.stream(), in, n, shift, hp, nb, tp, summary);
+    }
     parallel_scan("Kokkos::sort::radix_offsets", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, 256 * nblk),
                   ExclusiveScanU32{hp, op});
     const void* ks      = radix_scatter_kernel<T, PT, Iota>(first, last);
     const uint32_t* cop = op;
-    const PT* iin       = ibuf[p & 1];
-    PT* iout            = ibuf[(p + 1) & 1];
+    const PT* iin       = ibuf[k & 1];
+    PT* iout            = ibuf[(k + 1) & 1];
     mx_launch_kernel(ks, unsigned(nblk), kRadixLanes, 0, exec.stream(), in, out, iin, iout, n, shift, cop, nb, tp);
   }
-  // an even number of passes leaves the keys (and the permutation) in
-  // place; the workspace is free for the next call once the stream has run
-  // them
+  if (nrun & 1) {  // an odd number of passes left the result in the workspace
+    parallel_for("Kokkos::sort::copy_back", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, n),
+                 CopyElems<U>{static_cast<const U*>(buf[1]), reinterpret_cast<U*>(keys)});
+    if constexpr (pairs)
+      parallel_for("Kokkos::sort::copy_back_payload", RangePolicy<MI355X, IndexType<int64_t>>(exec, 0, n),
+                   CopyElems<PT>{ibuf[1], payload});
+  }
+  // the workspace is free for the next call once the stream has run these
   exec.fence("Kokkos::sort: radix passes");
 }
 

@@ -122,3 +122,45 @@ def test_sort_by_key_with_comparator(paths, oracle, code):
     order = sorted(range(n), key=lambda i: -float(x[i]))
     assert np.array_equal(Dk.download(TYPES[code], n), x[order])
     assert np.array_equal(Dv.download(np.uint64, n), vals[order])
+
+
+@pytest.mark.parametrize("code,lo,hi", [(0, 0, 1000), (0, -5, 5), (1, 0, 1 << 20), (4, 7, 8), (5, 0, 1 << 40),
+                                        (1, -(1 << 40), -(1 << 39))])
+@pytest.mark.parametrize("n", [4097, 1 << 20])
+def test_small_range_keys_skip_constant_digits(paths, oracle, code, lo, hi, n):
+    """Keys whose high (or all) digits are the same everywhere: those passes
+    are skipped, with an odd or even number of passes left (the odd case ends
+    with a copy back), or none at all; results equal numpy's stable sort and,
+    for sort_by_key, its stable argsort."""
+    P = paths
+    dt = TYPES[code]
+    x = oracle.fill_uniform_i64(n, 300 + code, lo, hi - 1).astype(dt)
+    D = K.DeviceBuffer.from_array(x)
+    run(P, P.kkmx_sort(D.ptr, n, code))
+    assert np.array_equal(D.download(dt, n), np.sort(x, kind="stable"))
+    vals = np.arange(n, dtype=np.uint32)
+    Dk, Dv = K.DeviceBuffer.from_array(x), K.DeviceBuffer.from_array(vals)
+    run(P, P.kkmx_sort_by_key(Dk.ptr, Dv.ptr, n, code, 4, 0))
+    order = np.argsort(x, kind="stable")
+    assert np.array_equal(Dk.download(dt, n), x[order])
+    assert np.array_equal(Dv.download(np.uint32, n), vals[order])
+
+
+def test_float_keys_with_constant_low_digits_and_signed_zeros(paths, oracle):
+    """Doubles that are small integers (their low mantissa bytes are zero in
+    every key) and a mix of -0.0 and +0.0 only (equal keys: nothing moves,
+    the input order stays)."""
+    P = paths
+    n = 100000
+    x = (oracle.fill_uniform_i64(n, 310, 0, 7).astype(np.float64) - 3.0)
+    x[::9] = -0.0
+    D = K.DeviceBuffer.from_array(x)
+    run(P, P.kkmx_sort(D.ptr, n, 3))
+    want = np.sort(x, kind="stable")
+    assert np.array_equal(D.download(np.float64, n).view(np.uint64), want.view(np.uint64))
+    z = np.where(np.arange(n) % 3 == 0, -0.0, 0.0)
+    vals = np.arange(n, dtype=np.uint64)
+    Dk, Dv = K.DeviceBuffer.from_array(z), K.DeviceBuffer.from_array(vals)
+    run(P, P.kkmx_sort_by_key(Dk.ptr, Dv.ptr, n, 3, 8, 0))
+    assert np.array_equal(Dk.download(np.float64, n).view(np.uint64), z.view(np.uint64))
+    assert np.array_equal(Dv.download(np.uint64, n), vals)
