@@ -2546,6 +2546,39 @@ KKMX_TEST(binsort_subrange_and_strided_values) {
 }
 
 // =========================================================================
+// Kokkos::sort of strided keys (TestSort.hpp sorts subviews): every third
+// element of a buffer, whole and over [begin, end), on the device; the
+// elements in between stay put.
+// =========================================================================
+KKMX_TEST(sort_strided_keys) {
+  const int n = 30001;
+  Kokkos::View<int*, Exec> store("store", 3 * n);
+  auto hs = Kokkos::create_mirror_view(store);
+  for (int i = 0; i < 3 * n; ++i) hs(i) = int((int64_t(i) * 7919) % 100003) - 50000;
+  Kokkos::deep_copy(store, hs);
+  Kokkos::View<int*, Kokkos::LayoutStride, Exec> keys(store.data(), Kokkos::LayoutStride(n, 3));
+  Kokkos::sort(Exec(), keys);
+  auto g = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), store);
+  std::vector<int> want;
+  for (int i = 0; i < n; ++i) want.push_back(hs(3 * i));
+  std::sort(want.begin(), want.end());
+  for (int i = 0; i < n; ++i) {
+    CHECK_EQ(g(3 * i), want[i]);
+    CHECK_EQ(g(3 * i + 1), hs(3 * i + 1));
+    CHECK_EQ(g(3 * i + 2), hs(3 * i + 2));
+  }
+  // a window [100, 20000) of the strided keys, after reversing them
+  for (int i = 0; i < n; ++i) hs(3 * i) = -g(3 * i);
+  Kokkos::deep_copy(store, hs);
+  Kokkos::sort(Exec(), keys, 100, 20000);
+  Kokkos::deep_copy(g, store);
+  std::vector<int> win;
+  for (int i = 100; i < 20000; ++i) win.push_back(hs(3 * i));
+  std::sort(win.begin(), win.end());
+  for (int i = 0; i < n; ++i) CHECK_EQ(g(3 * i), (i >= 100 && i < 20000) ? win[i - 100] : hs(3 * i));
+}
+
+// =========================================================================
 // sort_by_key through the permutation path (values that are not contiguous
 // 4- or 8-byte words): a LayoutStride column of a rank-2 View and 2-byte
 // values (algorithms/unit_tests/TestSortByKey.hpp checks keys and values

@@ -762,8 +762,16 @@ void sort(const ExecutionSpace& exec, const View<DataType, Properties...>& view)
   if constexpr (Impl::radix_sortable<V>() && std::is_same<ExecutionSpace, MI355X>::value) {
     if (view.span_is_contiguous()) {
       Impl::radix_sort<T>(exec, view.data(), int64_t(view.extent(0)));
-      return;
+    } else {
+      // strided keys: gather into a contiguous copy, sort it, scatter back
+      View<T*, MI355XSpace> packed(view_alloc(WithoutInitializing, std::string("Kokkos::sort::packed")),
+                                   view.extent(0));
+      deep_copy(exec, packed, view);
+      Impl::radix_sort<T>(exec, packed.data(), int64_t(packed.extent(0)));
+      deep_copy(exec, view, packed);
+      exec.fence("Kokkos::sort: strided keys written back");
     }
+    return;
   }
   exec.fence();
   Impl::host_sort(view, 0, view.extent(0), std::less<T>());
@@ -795,6 +803,14 @@ void sort(const ExecutionSpace& exec, ViewType view, size_t begin, size_t end) {
       Impl::radix_sort<T>(exec, view.data() + begin, int64_t(end - begin));
       return;
     }
+    // strided keys: the same gather / sort / scatter over [begin, end)
+    View<T*, MI355XSpace> packed(view_alloc(WithoutInitializing, std::string("Kokkos::sort::packed")), end - begin);
+    auto window = subview(view, std::make_pair(begin, end));
+    deep_copy(exec, packed, window);
+    Impl::radix_sort<T>(exec, packed.data(), int64_t(packed.extent(0)));
+    deep_copy(exec, window, packed);
+    exec.fence("Kokkos::sort: strided keys written back");
+    return;
   }
   exec.fence();
   Impl::host_sort(view, begin, end, std::less<T>());
