@@ -133,6 +133,14 @@ Impl::ViewCtorProp view_wrap(const Args&... args) {
 namespace Impl {
 struct SubviewCtor {};
 
+template <unsigned N>
+struct ViewRankConstant {
+  using value_type                  = size_t;
+  static constexpr size_t value     = N;
+  KOKKOS_INLINE_FUNCTION constexpr operator size_t() const noexcept { return N; }
+  KOKKOS_INLINE_FUNCTION constexpr size_t operator()() const noexcept { return N; }
+};
+
 struct SharedAllocRecord {
   std::atomic<int> count{1};
   std::string label;
@@ -343,8 +351,10 @@ class View : public ViewTraits<DataType, Properties...> {
   using HostMirror             = host_mirror_type;
   using uniform_type           = View<typename traits::data_type, array_layout, device_type, memory_traits>;
 
-  static constexpr unsigned rank         = traits::rank;
-  static constexpr unsigned rank_dynamic = traits::rank_dynamic;
+  // Integral constants: usable as values (V::rank) and as calls (V::rank()),
+  // as in the reference (core/src/View/Kokkos_ViewLegacy.hpp:315-319).
+  static constexpr Impl::ViewRankConstant<traits::rank> rank                 = {};
+  static constexpr Impl::ViewRankConstant<traits::rank_dynamic> rank_dynamic = {};
   static constexpr unsigned Rank         = rank;
   static constexpr bool reference_type_is_lvalue_reference = !memory_traits::is_atomic;
 

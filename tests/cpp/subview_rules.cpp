@@ -29,5 +29,8 @@ int main() {
   static_assert(same<Kokkos::Subview<L3, int, int, int>::array_layout, Kokkos::LayoutLeft>);
   static_assert(Kokkos::Subview<L3, int, int, int>::rank == 0);
   static_assert(Kokkos::Subview<R3, int, P, ALL>::rank == 2);
+  // rank and rank_dynamic are usable as values and as calls
+  static_assert(R3::rank() == 3 && R3::rank == 3 && R3::rank_dynamic() == 3);
+  static_assert(Kokkos::View<double* [4], E>::rank() == 2 && Kokkos::View<double* [4], E>::rank_dynamic == 1);
   return 0;
 }
