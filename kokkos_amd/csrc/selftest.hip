@@ -2798,6 +2798,184 @@ KKMX_TEST(thread_safety_team_reduce) {
       error);
 }
 
+// =========================================================================
+// View layout queries, LayoutStride allocations, rank-0 copies, local
+// (in-kernel) deep copies and a vector-lane scan with reducers
+// (core/unit_test/TestViewAPI.hpp:25-40 required_allocation_size,
+// TestTeam.hpp:1310-1322 order_dimensions, TestLocalDeepCopy.hpp,
+// TestTeamVector.hpp:880-960 ThreadVectorRange scans with reducers).
+// =========================================================================
+KKMX_TEST(view_layouts_and_local_deep_copy) {
+  using V1  = Kokkos::View<int*, Exec>;
+  using V1R = Kokkos::View<int*, Kokkos::LayoutRight, Exec>;
+  CHECK_EQ(V1::required_allocation_size(100), size_t(400));
+  CHECK_EQ(V1R::required_allocation_size(Kokkos::LayoutRight(100)), size_t(400));
+  const int order[3] = {2, 0, 1}, dims[3] = {100, 10, 3};
+  const Kokkos::LayoutStride ls = Kokkos::LayoutStride::order_dimensions(3, order, dims);
+  CHECK_EQ(ls.stride[2], size_t(1));
+  CHECK_EQ(ls.stride[0], size_t(3));
+  CHECK_EQ(ls.stride[1], size_t(300));
+  using SV3 = Kokkos::View<double***, Kokkos::LayoutStride, Exec>;
+  using RV3 = Kokkos::View<double***, Kokkos::LayoutRight, Exec>;
+  CHECK_EQ(SV3::shmem_size(ls), RV3::shmem_size(100, 10, 3));
+  const int bad[3] = {0, 0, 1};
+  CHECK_EQ(Kokkos::LayoutStride::order_dimensions(3, bad, dims).stride[0], size_t(0));
+
+  // a managed LayoutStride allocation keeps the requested strides
+  Kokkos::View<int***, Kokkos::LayoutStride, Exec> sv("sv", ls);
+  CHECK_EQ(sv.span(), size_t(3000));
+  CHECK_EQ(sv.stride(1), size_t(300));
+  Kokkos::parallel_for(
+      Kokkos::MDRangePolicy<Exec, Kokkos::Rank<3>>({0, 0, 0}, {100, 10, 3}),
+      KOKKOS_LAMBDA(int i, int j, int k) { sv(i, j, k) = i * 10000 + j * 10 + k; });
+  std::vector<int> raw(3000);
+  Kokkos::View<int*, Kokkos::HostSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>> hraw(raw.data(), 3000);
+  Kokkos::View<int*, Exec, Kokkos::MemoryTraits<Kokkos::Unmanaged>> draw(sv.data(), 3000);
+  Kokkos::deep_copy(hraw, draw);
+  for (int i = 0; i < 100; ++i)
+    for (int j = 0; j < 10; ++j)
+      for (int k = 0; k < 3; ++k) CHECK_EQ(raw[3 * i + 300 * j + k], i * 10000 + j * 10 + k);
+
+  // rank-0 copies between layouts and spaces
+  Kokkos::View<int, Kokkos::LayoutLeft, Exec> s0("s0");
+  Kokkos::View<int, Kokkos::LayoutRight, Exec> s1("s1");
+  Kokkos::deep_copy(s0, 41);
+  Kokkos::deep_copy(s1, s0);
+  auto h1 = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), s1);
+  CHECK_EQ(h1(), 41);
+
+  // team and sequential local_deep_copy of rank-2 slices: contiguous
+  // (LayoutRight) and strided (LayoutLeft) slices, copies and fills
+  const int L = 9, A = 17, B = 5;
+  Kokkos::View<double***, Kokkos::LayoutRight, Exec> srcR("srcR", L, A, B), dstR("dstR", L, A, B);
+  Kokkos::View<double***, Kokkos::LayoutLeft, Exec> srcL("srcL", L, A, B), dstL("dstL", L, A, B),
+      seqL("seqL", L, A, B), fillL("fillL", L, A, B);
+  Kokkos::View<double**, Kokkos::LayoutRight, Exec> seq1("seq1", L, A);
+  Kokkos::parallel_for(
+      Kokkos::MDRangePolicy<Exec, Kokkos::Rank<3>>({0, 0, 0}, {L, A, B}), KOKKOS_LAMBDA(int t, int a, int b) {
+        srcR(t, a, b) = t * 1000.0 + a * 10.0 + b;
+        srcL(t, a, b) = -(t * 1000.0 + a * 10.0 + b);
+      });
+  using TP = Kokkos::TeamPolicy<Exec>;
+  Kokkos::parallel_for(
+      TP(L, 4, 16), KOKKOS_LAMBDA(const TP::member_type& m) {
+        const int t = m.league_rank();
+        Kokkos::Experimental::local_deep_copy(m, Kokkos::subview(dstR, t, Kokkos::ALL, Kokkos::ALL),
+                                              Kokkos::subview(srcR, t, Kokkos::ALL, Kokkos::ALL));
+        Kokkos::Experimental::local_deep_copy(m, Kokkos::subview(dstL, t, Kokkos::ALL, Kokkos::ALL),
+                                              Kokkos::subview(srcL, t, Kokkos::ALL, Kokkos::ALL));
+        Kokkos::Experimental::local_deep_copy(m, Kokkos::subview(fillL, t, Kokkos::ALL, Kokkos::ALL), 0.5 * t);
+      });
+  Kokkos::parallel_for(
+      Kokkos::RangePolicy<Exec>(0, L), KOKKOS_LAMBDA(int t) {
+        Kokkos::Experimental::local_deep_copy(Kokkos::subview(seqL, t, Kokkos::ALL, Kokkos::ALL),
+                                              Kokkos::subview(srcL, t, Kokkos::ALL, Kokkos::ALL));
+        Kokkos::Experimental::local_deep_copy(Kokkos::subview(seq1, t, Kokkos::ALL), double(t) + 0.25);
+      });
+  auto hR = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), dstR);
+  auto hL = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), dstL);
+  auto hS = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), seqL);
+  auto hF = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), fillL);
+  auto h1d = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), seq1);
+  for (int t = 0; t < L; ++t)
+    for (int a = 0; a < A; ++a) {
+      CHECK_EQ(h1d(t, a), double(t) + 0.25);
+      for (int b = 0; b < B; ++b) {
+        const double v = t * 1000.0 + a * 10.0 + b;
+        CHECK_EQ(hR(t, a, b), v);
+        CHECK_EQ(hL(t, a, b), -v);
+        CHECK_EQ(hS(t, a, b), -v);
+        CHECK_EQ(hF(t, a, b), 0.5 * t);
+      }
+    }
+
+  // ThreadVectorRange scans with Max and Prod reducers (exact: integers and
+  // powers of two), inclusive outputs and the reducer's total per thread
+  const int teams = 6, ts = 4, n = 37;
+  Kokkos::View<int*, Exec> in_i("in_i", teams * ts * n), out_i("out_i", teams * ts * n), tot_i("tot_i", teams * ts);
+  Kokkos::View<double*, Exec> in_d("in_d", teams * ts * n), out_d("out_d", teams * ts * n),
+      tot_d("tot_d", teams * ts);
+  Kokkos::parallel_for(
+      Kokkos::RangePolicy<Exec>(0, teams * ts * n), KOKKOS_LAMBDA(int i) {
+        in_i(i) = int((i * 7919) % 1009) - 500;
+        in_d(i) = (i % 3 == 0) ? 2.0 : (i % 3 == 1 ? 0.5 : (i % 7 == 0 ? 2.0 : 1.0));
+      });
+  Kokkos::parallel_for(
+      TP(teams, ts, 16), KOKKOS_LAMBDA(const TP::member_type& m) {
+        Kokkos::parallel_for(Kokkos::TeamThreadRange(m, ts), [&](int r) {
+          const int base = (m.league_rank() * ts + r) * n;
+          int mx         = 0;
+          Kokkos::parallel_scan(
+              Kokkos::ThreadVectorRange(m, n),
+              [&](int j, int& upd, bool final) {
+                if (in_i(base + j) > upd) upd = in_i(base + j);
+                if (final) out_i(base + j) = upd;
+              },
+              Kokkos::Max<int, Exec>(mx));
+          double pr = 0;
+          Kokkos::parallel_scan(
+              Kokkos::ThreadVectorRange(m, n),
+              [&](int j, double& upd, bool final) {
+                upd *= in_d(base + j);
+                if (final) out_d(base + j) = upd;
+              },
+              Kokkos::Prod<double, Exec>(pr));
+          Kokkos::single(Kokkos::PerThread(m), [&]() {
+            tot_i(base / n) = mx;
+            tot_d(base / n) = pr;
+          });
+        });
+      });
+  auto hin_i = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), in_i);
+  auto hin_d = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), in_d);
+  auto hout_i = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out_i);
+  auto hout_d = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out_d);
+  auto htot_i = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), tot_i);
+  auto htot_d = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), tot_d);
+  for (int r = 0; r < teams * ts; ++r) {
+    int mx    = std::numeric_limits<int>::lowest();
+    double pr = 1.0;
+    for (int j = 0; j < n; ++j) {
+      mx = std::max(mx, hin_i(r * n + j));
+      pr *= hin_d(r * n + j);
+      CHECK_EQ(hout_i(r * n + j), mx);
+      CHECK_EQ(hout_d(r * n + j), pr);
+    }
+    CHECK_EQ(htot_i(r), mx);
+    CHECK_EQ(htot_d(r), pr);
+  }
+
+  // MinMaxFirstLastLoc: ties take the first minimum and the last maximum
+  const int nv = 100003;
+  Kokkos::View<int*, Exec> vals("vals", nv);
+  Kokkos::parallel_for(
+      Kokkos::RangePolicy<Exec>(0, nv), KOKKOS_LAMBDA(int i) { vals(i) = (i * 37) % 11; });
+  using MMFL = Kokkos::MinMaxFirstLastLoc<int, int, Kokkos::HostSpace>;
+  MMFL::value_type mm;
+  Kokkos::parallel_reduce(
+      Kokkos::RangePolicy<Exec>(0, nv),
+      KOKKOS_LAMBDA(int i, MMFL::value_type& u) {
+        if (vals(i) < u.min_val || (vals(i) == u.min_val && i < u.min_loc)) {
+          u.min_val = vals(i);
+          u.min_loc = i;
+        }
+        if (vals(i) > u.max_val || (vals(i) == u.max_val && i > u.max_loc)) {
+          u.max_val = vals(i);
+          u.max_loc = i;
+        }
+      },
+      MMFL(mm));
+  int first_min = -1, last_max = -1;
+  for (int i = 0; i < nv; ++i) {
+    if (first_min < 0 && (i * 37) % 11 == 0) first_min = i;
+    if ((i * 37) % 11 == 10) last_max = i;
+  }
+  CHECK_EQ(mm.min_val, 0);
+  CHECK_EQ(mm.max_val, 10);
+  CHECK_EQ(mm.min_loc, first_min);
+  CHECK_EQ(mm.max_loc, last_max);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- C-ABI

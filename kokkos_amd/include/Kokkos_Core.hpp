@@ -12,6 +12,7 @@
 
 #include <Kokkos_Macros.hpp>
 #include <kkmx/Core_fwd.hpp>
+#include <kkmx/TypeInfo.hpp>
 #include <kkmx/Math.hpp>
 #include <kkmx/Complex.hpp>
 #include <kkmx/Spaces.hpp>
@@ -31,6 +32,7 @@
 #include <kkmx/TeamMDRange.hpp>
 #include <kkmx/Parallel.hpp>
 #include <kkmx/CopyViews.hpp>
+#include <kkmx/LocalDeepCopy.hpp>
 #include <kkmx/KokkosMalloc.hpp>
 #include <kkmx/Graph.hpp>
 #include <kkmx/SpacesImpl.hpp>

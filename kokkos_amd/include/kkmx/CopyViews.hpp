@@ -52,6 +52,21 @@ struct IndexCopy {
   }
 };
 
+// A layout of v's extents with no padding: a LayoutStride gets compact
+// strides, last index fastest.  Staging buffers are allocated from it, so a
+// single transfer of size() elements moves them whole.
+template <class V>
+typename V::array_layout compact_layout(const V& v) {
+  typename V::array_layout l{};
+  size_t s = 1;
+  for (int r = int(V::rank) - 1; r >= 0; --r) {
+    l.dimension[r] = v.extent(r);
+    if constexpr (std::is_same<typename V::array_layout, LayoutStride>::value) l.stride[r] = s;
+    s *= v.extent(r);
+  }
+  return l;
+}
+
 template <class DV, class SV>
 struct ElementCopy {
   DV dst;
@@ -65,7 +80,8 @@ struct ElementCopy {
       idx[d]         = static_cast<size_t>(r % static_cast<int64_t>(e));
       r /= static_cast<int64_t>(e);
     }
-    if constexpr (DV::rank == 1) dst(idx[0]) = src(idx[0]);
+    if constexpr (DV::rank == 0) dst() = src();
+    else if constexpr (DV::rank == 1) dst(idx[0]) = src(idx[0]);
     else if constexpr (DV::rank == 2) dst(idx[0], idx[1]) = src(idx[0], idx[1]);
     else if constexpr (DV::rank == 3) dst(idx[0], idx[1], idx[2]) = src(idx[0], idx[1], idx[2]);
     else if constexpr (DV::rank == 4) dst(idx[0], idx[1], idx[2], idx[3]) = src(idx[0], idx[1], idx[2], idx[3]);
@@ -420,7 +436,7 @@ inline void deep_copy(const View<D1, P1...>& dst, const View<D2, P2...>& src) {
     if constexpr (Impl::host_accessible<typename DV::memory_space>()) {
       // device -> host: reorder on the device into dst's layout, then copy
       View<typename DV::traits::non_const_data_type, typename DV::array_layout, MI355XSpace> stage(
-          view_alloc(WithoutInitializing, std::string("deep_copy_stage")), dst.layout());
+          view_alloc(WithoutInitializing, std::string("deep_copy_stage")), Impl::compact_layout(dst));
       Impl::device_copy(MI355X(), stage, src);
       MI355X().fence("Kokkos::deep_copy: staged device copy");
       if (dst.span_is_contiguous()) {
@@ -434,11 +450,11 @@ inline void deep_copy(const View<D1, P1...>& dst, const View<D2, P2...>& src) {
     } else {
       // host -> device: a contiguous host copy in src's layout, one transfer,
       // then reorder on the device
-      typename SV::HostMirror hs(view_alloc(WithoutInitializing, std::string("deep_copy_stage")), src.layout());
+      typename SV::HostMirror hs(view_alloc(WithoutInitializing, std::string("deep_copy_stage")), Impl::compact_layout(src));
       Impl::ElementCopy<typename SV::HostMirror, SV> f{hs, src};
       for (int64_t i = 0; i < int64_t(src.size()); ++i) f(i);
       View<typename SV::traits::non_const_data_type, typename SV::array_layout, MI355XSpace> stage(
-          view_alloc(WithoutInitializing, std::string("deep_copy_stage")), src.layout());
+          view_alloc(WithoutInitializing, std::string("deep_copy_stage")), Impl::compact_layout(src));
       KKMX_SAFE_CALL(mx_memcpy(stage.data(), hs.data(), src.size() * sizeof(T)));
       Impl::device_copy(MI355X(), dst, stage);
       MI355X().fence("Kokkos::deep_copy: staged device copy");

@@ -603,7 +603,8 @@ KOKKOS_INLINE_FUNCTION void parallel_scan(const Impl::TeamThreadRangeBoundariesS
   T dummy;
   parallel_scan(r, f, dummy);
 }
-template <class iType, class Member, class F, class T>
+template <class iType, class Member, class F, class T,
+          class = std::enable_if_t<!is_reducer<std::remove_const_t<T>>::value>>
 KOKKOS_INLINE_FUNCTION void parallel_scan(const Impl::ThreadVectorRangeBoundariesStruct<iType, Member>& r,
                                           const F& f, T& return_value) {
   const int vl = r.member.impl_vector_length();
@@ -635,6 +636,38 @@ KOKKOS_INLINE_FUNCTION void parallel_scan(const Impl::ThreadVectorRangeBoundarie
   using T = typename Impl::arg_from_back<typename Impl::op_args<F>::type, 1>::type;
   T dummy;
   parallel_scan(r, f, dummy);
+}
+// Scan over a thread's vector lanes with a reducer's identity and join in
+// place of 0 and +; the total lands in the reducer's result
+// (core/src/HIP/Kokkos_HIP_Team.hpp, parallel_scan(ThreadVectorRange,
+// closure, reducer)).
+template <class iType, class Member, class F, class R, class = std::enable_if_t<is_reducer<R>::value>>
+KOKKOS_INLINE_FUNCTION void parallel_scan(const Impl::ThreadVectorRangeBoundariesStruct<iType, Member>& r,
+                                          const F& f, const R& reducer) {
+  using T = typename R::value_type;
+  T identity;
+  reducer.init(identity);
+  T running = identity;
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
+    const int vl = r.member.impl_vector_length();
+    auto join    = [&](T& a, const T& b) { reducer.join(a, b); };
+    for (iType base = r.start; base < r.end; base += vl) {
+      const iType i = base + r.member.impl_vector_lane();
+      T v           = identity;
+      if (i < r.end) f(i, v, false);
+      T total;
+      T upd = running;
+      reducer.join(upd, r.member.scan_vector(v, identity, join, &total));
+      if (i < r.end) f(i, upd, true);
+      reducer.join(running, total);
+    }
+  } else
+#endif
+  {
+    for (iType i = r.start; i < r.end; ++i) f(i, running, true);
+  }
+  reducer.reference() = running;
 }
 
 // ------------------------------------------------------------------ single

@@ -220,6 +220,28 @@ struct MinFirstLoc : Impl::ReducerStorage<ValLocScalar<std::remove_cv_t<Scalar>,
     v.loc = reduction_identity<index_type>::min();
   }
 };
+// Minimum with its first location and maximum with its last location
+// (core/src/Kokkos_Parallel_Reduce.hpp:946-1010).
+template <class Scalar, class Index, class Space = HostSpace>
+struct MinMaxFirstLastLoc
+    : Impl::ReducerStorage<MinMaxLocScalar<std::remove_cv_t<Scalar>, std::remove_cv_t<Index>>, Space> {
+  using scalar_type = std::remove_cv_t<Scalar>;
+  using index_type  = std::remove_cv_t<Index>;
+  using value_type  = MinMaxLocScalar<scalar_type, index_type>;
+  KKMX_LOC_REDUCER_HEAD(MinMaxFirstLastLoc)
+  KOKKOS_INLINE_FUNCTION void join(value_type& d, const value_type& s) const {
+    if (s.min_val < d.min_val) { d.min_val = s.min_val; d.min_loc = s.min_loc; }
+    else if (!(d.min_val < s.min_val)) d.min_loc = (s.min_loc < d.min_loc) ? s.min_loc : d.min_loc;
+    if (d.max_val < s.max_val) { d.max_val = s.max_val; d.max_loc = s.max_loc; }
+    else if (!(s.max_val < d.max_val)) d.max_loc = (s.max_loc > d.max_loc) ? s.max_loc : d.max_loc;
+  }
+  KOKKOS_INLINE_FUNCTION void init(value_type& v) const {
+    v.max_val = reduction_identity<scalar_type>::max();
+    v.min_val = reduction_identity<scalar_type>::min();
+    v.max_loc = reduction_identity<index_type>::max();
+    v.min_loc = reduction_identity<index_type>::min();
+  }
+};
 template <class Index, class Space = HostSpace>
 struct FirstLoc : Impl::ReducerStorage<FirstLocScalar<std::remove_cv_t<Index>>, Space> {
   using index_type = std::remove_cv_t<Index>;

@@ -50,8 +50,34 @@ struct LayoutStride {
   enum : bool { is_extent_constructible = false };
   KOKKOS_INLINE_FUNCTION explicit constexpr LayoutStride(size_t N0 = 0, size_t S0 = 0, size_t N1 = 0,
                                                          size_t S1 = 0, size_t N2 = 0, size_t S2 = 0,
-                                                         size_t N3 = 0, size_t S3 = 0)
-      : dimension{N0, N1, N2, N3, 0, 0, 0, 0}, stride{S0, S1, S2, S3, 0, 0, 0, 0} {}
+                                                         size_t N3 = 0, size_t S3 = 0, size_t N4 = 0,
+                                                         size_t S4 = 0, size_t N5 = 0, size_t S5 = 0,
+                                                         size_t N6 = 0, size_t S6 = 0, size_t N7 = 0,
+                                                         size_t S7 = 0)
+      : dimension{N0, N1, N2, N3, N4, N5, N6, N7}, stride{S0, S1, S2, S3, S4, S5, S6, S7} {}
+  // Compact strides for the dimension order order[0] (stride 1), order[1],
+  // ...; an order that is not a permutation of 0..rank-1 leaves every
+  // stride zero (core/src/Kokkos_Layout.hpp:168-192).
+  template <class IO, class ID>
+  KOKKOS_INLINE_FUNCTION static LayoutStride order_dimensions(int const rank, IO const* const order,
+                                                              ID const* const dimen) {
+    LayoutStride l;
+    if (rank < 0 || rank > 8) return l;
+    unsigned seen = 0;
+    for (int r = 0; r < rank; ++r) {
+      const int o = int(order[r]);
+      if (o < 0 || o >= rank) return l;
+      seen |= 1u << o;
+    }
+    if (seen != (1u << rank) - 1u) return l;
+    size_t n = 1;
+    for (int r = 0; r < rank; ++r) {
+      l.stride[order[r]] = n;
+      n *= size_t(dimen[order[r]]);
+      l.dimension[r] = size_t(dimen[r]);
+    }
+    return l;
+  }
 };
 
 template <class T>
@@ -71,7 +97,7 @@ enum MemoryTraitsFlags : unsigned {
   Restrict     = 0x08,
   Aligned      = 0x10
 };
-template <unsigned T>
+template <unsigned T = 0>
 struct MemoryTraits {
   using memory_traits = MemoryTraits<T>;
   static constexpr unsigned impl_value = T;
@@ -460,9 +486,7 @@ class View : public ViewTraits<DataType, Properties...> {
   View(const std::string& label, const L& layout) : View(view_alloc(label), layout) {}
   template <class L, class = std::enable_if_t<is_array_layout<L>::value>>
   View(const Impl::ViewCtorProp& prop, const L& layout) : m_ptr(nullptr), m_rec(nullptr) {
-    size_t dyn[8];
-    for (unsigned r = 0; r < 8; ++r) dyn[r] = layout.dimension[r];
-    set_extents(dyn);
+    set_layout(layout);
     allocate(prop);
   }
 
@@ -478,13 +502,7 @@ class View : public ViewTraits<DataType, Properties...> {
   }
   template <class L, class = std::enable_if_t<is_array_layout<L>::value>>
   KOKKOS_INLINE_FUNCTION View(pointer_type p, const L& layout) : m_ptr(p), m_rec(nullptr) {
-    if constexpr (std::is_same<L, LayoutStride>::value) {
-      for (unsigned r = 0; r < kR; ++r) { m_ext[r] = layout.dimension[r]; m_str[r] = layout.stride[r]; }
-    } else {
-      size_t dyn[8];
-      for (unsigned r = 0; r < 8; ++r) dyn[r] = layout.dimension[r];
-      set_extents(dyn);
-    }
+    set_layout(layout);
   }
 
   // A window of another View's allocation (Kokkos::subview): pointer,
@@ -520,6 +538,26 @@ class View : public ViewTraits<DataType, Properties...> {
     size_t dyn[8] = {n0, n1, n2, n3, n4, n5, n6, n7};
     set_extents(dyn);
     m_ptr = static_cast<pointer_type>(space.get_shmem(span() * sizeof(value_type)));
+  }
+
+  // Bytes an allocation of these extents (or this layout) takes
+  // (core/src/View/Kokkos_ViewLegacy.hpp required_allocation_size).
+  static size_t required_allocation_size(size_t n0 = KOKKOS_IMPL_NOARG, size_t n1 = KOKKOS_IMPL_NOARG,
+                                         size_t n2 = KOKKOS_IMPL_NOARG, size_t n3 = KOKKOS_IMPL_NOARG,
+                                         size_t n4 = KOKKOS_IMPL_NOARG, size_t n5 = KOKKOS_IMPL_NOARG,
+                                         size_t n6 = KOKKOS_IMPL_NOARG, size_t n7 = KOKKOS_IMPL_NOARG) {
+    View v(pointer_type(nullptr), n0, n1, n2, n3, n4, n5, n6, n7);
+    return v.span() * sizeof(value_type);
+  }
+  template <class L, class = std::enable_if_t<is_array_layout<L>::value>>
+  static size_t required_allocation_size(const L& layout) {
+    View v(pointer_type(nullptr), layout);
+    return v.span() * sizeof(value_type);
+  }
+  template <class L, class = std::enable_if_t<is_array_layout<L>::value>>
+  static size_t shmem_size(const L& layout) {
+    View v(pointer_type(nullptr), layout);
+    return ((v.span() * sizeof(value_type) + 7) / 8) * 8 + 8;
   }
 
   static constexpr size_t shmem_size(size_t n0 = KOKKOS_IMPL_NOARG, size_t n1 = KOKKOS_IMPL_NOARG,
@@ -626,6 +664,25 @@ class View : public ViewTraits<DataType, Properties...> {
   }
   KOKKOS_INLINE_FUNCTION void init_static() {
     for (unsigned r = 0; r < rank; ++r) if (static_ext(r)) m_ext[r] = static_ext(r);
+  }
+  // Extents from a layout object; a LayoutStride brings its strides too.
+  template <class L>
+  KOKKOS_INLINE_FUNCTION void set_layout(const L& layout) {
+    if constexpr (std::is_same<L, LayoutStride>::value) {
+      for (unsigned r = 0; r < kR; ++r) {
+        m_ext[r] = r < rank ? layout.dimension[r] : 0;
+        m_str[r] = r < rank ? layout.stride[r] : 1;
+      }
+      if constexpr (rank == 0) m_str[0] = 1;
+    } else {
+      size_t dyn[8];
+      unsigned d = 0;
+      // Layout objects list every extent, static ones included.
+      for (unsigned r = 0; r < rank; ++r)
+        if (!static_ext(r)) dyn[d++] = layout.dimension[r];
+      for (; d < 8; ++d) dyn[d] = KOKKOS_IMPL_NOARG;
+      set_extents(dyn);
+    }
   }
   KOKKOS_INLINE_FUNCTION void set_extents(const size_t* dyn) {
     unsigned d = 0;

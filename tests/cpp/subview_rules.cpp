@@ -32,5 +32,10 @@ int main() {
   // rank and rank_dynamic are usable as values and as calls
   static_assert(R3::rank() == 3 && R3::rank == 3 && R3::rank_dynamic() == 3);
   static_assert(Kokkos::View<double* [4], E>::rank() == 2 && Kokkos::View<double* [4], E>::rank_dynamic == 1);
+  // type names, and MemoryTraits<> as the default traits
+  static_assert(Kokkos::Impl::TypeInfo<int>::name() == "int");
+  static_assert(Kokkos::Impl::TypeInfo<double[3]>::name() == "double[3]");
+  static_assert(Kokkos::Impl::TypeInfo<Kokkos::LayoutLeft>::name() == "Kokkos::LayoutLeft");
+  static_assert(Kokkos::MemoryTraits<>::impl_value == 0 && Kokkos::MemoryTraits<>::is_unmanaged == false);
   return 0;
 }
