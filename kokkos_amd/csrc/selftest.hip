@@ -2976,6 +2976,64 @@ KKMX_TEST(view_layouts_and_local_deep_copy) {
   CHECK_EQ(mm.max_loc, last_max);
 }
 
+// =========================================================================
+// Combined reductions inside a team: four results (plain sums, Sum and Max
+// reducers) from one TeamThreadRange, ThreadVectorRange and TeamVectorRange
+// pass each (core/unit_test/TestTeamCombinedReducers.hpp:100-260, closed
+// forms n(n+1)/2, n and n*n).
+// =========================================================================
+KKMX_TEST(team_combined_reducers) {
+  const int league = 10, n = 1000;
+  using TP = Kokkos::TeamPolicy<Exec>;
+  Kokkos::View<int**, Kokkos::LayoutRight, Exec> out("out", league, 12);
+  Kokkos::parallel_for(
+      TP(league, 8, 8), KOKKOS_LAMBDA(const TP::member_type& team) {
+        int a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        Kokkos::parallel_reduce(
+            Kokkos::TeamThreadRange(team, n),
+            [=](int i, int& v0, int& v1, int& v2, int& v3) {
+              v0 += i + 1;
+              v1 += i + 1;
+              v2 = (v2 < i + 1) ? i + 1 : v2;
+              v3 += n;
+            },
+            a0, Kokkos::Sum<int>(a1), Kokkos::Max<int>(a2), a3);
+        int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+        Kokkos::parallel_reduce(
+            Kokkos::ThreadVectorRange(team, n),
+            [=](int i, int& v0, int& v1, int& v2, int& v3) {
+              v0 += i + 1;
+              v1 += i + 1;
+              v2 = (v2 < i + 1) ? i + 1 : v2;
+              v3 += n;
+            },
+            b0, Kokkos::Sum<int>(b1), Kokkos::Max<int>(b2), b3);
+        int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        Kokkos::parallel_reduce(
+            Kokkos::TeamVectorRange(team, n),
+            [=](int i, int& v0, int& v1, int& v2, int& v3) {
+              v0 += i + 1;
+              v1 += i + 1;
+              v2 = (v2 < i + 1) ? i + 1 : v2;
+              v3 += n;
+            },
+            c0, Kokkos::Sum<int>(c1), Kokkos::Max<int>(c2), c3);
+        Kokkos::single(Kokkos::PerTeam(team), [=]() {
+          const int r = team.league_rank();
+          const int v[12] = {a0, a1, a2, a3, b0, b1, b2, b3, c0, c1, c2, c3};
+          for (int k = 0; k < 12; ++k) out(r, k) = v[k];
+        });
+      });
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+  for (int r = 0; r < league; ++r)
+    for (int g = 0; g < 3; ++g) {
+      CHECK_EQ(h(r, 4 * g + 0), n * (n + 1) / 2);
+      CHECK_EQ(h(r, 4 * g + 1), n * (n + 1) / 2);
+      CHECK_EQ(h(r, 4 * g + 2), n);
+      CHECK_EQ(h(r, 4 * g + 3), n * n);
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- C-ABI

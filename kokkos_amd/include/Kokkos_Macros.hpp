@@ -50,6 +50,11 @@
   do {                                                                   \
     if (!(__VA_ARGS__)) ::Kokkos::abort("Kokkos contract violation: " #__VA_ARGS__); \
   } while (0)
+#define KOKKOS_ENSURES(...) KOKKOS_EXPECTS(__VA_ARGS__)
+#define KOKKOS_ASSERT(...)                                                        \
+  do {                                                                            \
+    if (!(__VA_ARGS__)) ::Kokkos::abort("Kokkos assertion failed: " #__VA_ARGS__); \
+  } while (0)
 
 // A wavefront is 64 lanes on CDNA.
 #define KOKKOS_IMPL_MI355X_WAVE 64

@@ -282,7 +282,8 @@ void host_fill(const V& dst, const typename V::const_value_type& value) {
         idx[d] = r % dst.extent(d);
         r /= dst.extent(d);
       }
-      if constexpr (V::rank == 1) f(idx[0]);
+      if constexpr (V::rank == 0) f();
+      else if constexpr (V::rank == 1) f(idx[0]);
       else if constexpr (V::rank == 2) f(idx[0], idx[1]);
       else if constexpr (V::rank == 3) f(idx[0], idx[1], idx[2]);
       else if constexpr (V::rank == 4) f(idx[0], idx[1], idx[2], idx[3]);

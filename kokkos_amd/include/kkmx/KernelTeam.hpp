@@ -562,6 +562,112 @@ KOKKOS_INLINE_FUNCTION void parallel_reduce(const Impl::TeamVectorRangeBoundarie
   else result = v;
 }
 
+// ------------------------------------------------------------------ nested combined reductions
+// parallel_reduce(range, f, result0, result1, ...): several results (sums
+// into scalars, or reducers) from one pass, the functor taking one update
+// per result (core/src/Kokkos_Parallel_Reduce.hpp combined reducers, used
+// at team level by core/unit_test/TestTeamCombinedReducers.hpp).  The
+// values travel as one struct through the single-result reduction above,
+// joined member by member with each result's own init and join.
+namespace Impl {
+
+template <class R>
+struct is_nested_range : std::false_type {};
+template <class I, class M>
+struct is_nested_range<TeamThreadRangeBoundariesStruct<I, M>> : std::true_type {};
+template <class I, class M>
+struct is_nested_range<ThreadVectorRangeBoundariesStruct<I, M>> : std::true_type {};
+template <class I, class M>
+struct is_nested_range<TeamVectorRangeBoundariesStruct<I, M>> : std::true_type {};
+
+template <class... Rs>
+struct NestedVals;
+template <class R0>
+struct NestedVals<R0> {
+  typename NestedOps<R0>::value_type head;
+};
+template <class R0, class R1, class... Rs>
+struct NestedVals<R0, R1, Rs...> {
+  typename NestedOps<R0>::value_type head;
+  NestedVals<R1, Rs...> tail;
+};
+
+template <size_t K, class V>
+KOKKOS_INLINE_FUNCTION auto& nested_get(V& v) {
+  if constexpr (K == 0) return v.head;
+  else return nested_get<K - 1>(v.tail);
+}
+template <size_t K, class V>
+KOKKOS_INLINE_FUNCTION const auto& nested_get(const V& v) {
+  if constexpr (K == 0) return v.head;
+  else return nested_get<K - 1>(v.tail);
+}
+
+template <class... Rs>
+struct NestedCombined {
+  using reducer    = NestedCombined;
+  using value_type = NestedVals<Rs...>;
+
+  template <size_t... K>
+  KOKKOS_INLINE_FUNCTION void init_k(value_type& v, std::index_sequence<K...>) const {
+    (NestedOps<Rs>::init(*static_cast<const Rs*>(res_at<K>()), nested_get<K>(v)), ...);
+  }
+  template <size_t... K>
+  KOKKOS_INLINE_FUNCTION void join_k(value_type& a, const value_type& b, std::index_sequence<K...>) const {
+    (NestedOps<Rs>::join(*static_cast<const Rs*>(res_at<K>()), nested_get<K>(a), nested_get<K>(b)), ...);
+  }
+  template <size_t... K>
+  KOKKOS_INLINE_FUNCTION void store_k(const value_type& v, std::index_sequence<K...>) const {
+    (store_one(*res_at<K>(), nested_get<K>(v)), ...);
+  }
+  template <class R, class T>
+  KOKKOS_INLINE_FUNCTION static void store_one(R& r, const T& v) {
+    if constexpr (is_reducer<R>::value) r.reference() = v;
+    else r = v;
+  }
+  template <size_t K>
+  KOKKOS_INLINE_FUNCTION auto* res_at() const {
+    using R = std::tuple_element_t<K, std::tuple<Rs...>>;
+    return static_cast<R*>(ptrs[K]);
+  }
+  void* ptrs[sizeof...(Rs)];
+
+  KOKKOS_INLINE_FUNCTION void init(value_type& v) const { init_k(v, std::index_sequence_for<Rs...>{}); }
+  KOKKOS_INLINE_FUNCTION void join(value_type& a, const value_type& b) const {
+    join_k(a, b, std::index_sequence_for<Rs...>{});
+  }
+  struct Store {
+    const NestedCombined* c;
+    KOKKOS_INLINE_FUNCTION void operator=(const value_type& v) const {
+      c->store_k(v, std::index_sequence_for<Rs...>{});
+    }
+  };
+  KOKKOS_INLINE_FUNCTION Store reference() const { return Store{this}; }
+};
+
+template <class F, class V, size_t... K, class I>
+KOKKOS_INLINE_FUNCTION void nested_call(const F& f, const I& i, V& v, std::index_sequence<K...>) {
+  f(i, nested_get<K>(v)...);
+}
+
+}  // namespace Impl
+
+template <class Range, class F, class R0, class R1, class... Rs>
+KOKKOS_INLINE_FUNCTION std::enable_if_t<Impl::is_nested_range<Range>::value> parallel_reduce(const Range& r,
+                                                                                           const F& f, R0&& r0,
+                                                                                           R1&& r1, Rs&&... rs) {
+  using C = Impl::NestedCombined<Impl::remove_cvref_t<R0>, Impl::remove_cvref_t<R1>, Impl::remove_cvref_t<Rs>...>;
+  C c;
+  void* p[] = {const_cast<void*>(static_cast<const void*>(&r0)), const_cast<void*>(static_cast<const void*>(&r1)),
+               const_cast<void*>(static_cast<const void*>(&rs))...};
+  for (size_t k = 0; k < 2 + sizeof...(Rs); ++k) c.ptrs[k] = p[k];
+  using V = typename C::value_type;
+  parallel_reduce(
+      r,
+      [&](const auto& i, V& v) { Impl::nested_call(f, i, v, std::make_index_sequence<2 + sizeof...(Rs)>{}); },
+      c);
+}
+
 // ------------------------------------------------------------------ nested parallel_scan
 namespace Impl {
 template <class F, class iType, class T>

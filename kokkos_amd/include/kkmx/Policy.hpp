@@ -60,7 +60,12 @@ struct WorkItemProperty {
   static constexpr ImplWorkItemProperty<4> HintRegular{};
   static constexpr ImplWorkItemProperty<8> HintIrregular{};
   static constexpr ImplWorkItemProperty<16> ImplForceGlobalLaunch{};
-  using None_t = ImplWorkItemProperty<0>;
+  using None_t                  = ImplWorkItemProperty<0>;
+  using HintLightWeight_t       = ImplWorkItemProperty<1>;
+  using HintHeavyWeight_t       = ImplWorkItemProperty<2>;
+  using HintRegular_t           = ImplWorkItemProperty<4>;
+  using HintIrregular_t         = ImplWorkItemProperty<8>;
+  using ImplForceGlobalLaunch_t = ImplWorkItemProperty<16>;
 };
 template <class Policy, class Prop>
 Policy require(const Policy& p, const Prop&) {
@@ -540,6 +545,9 @@ class TeamPolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
     m_team_scratch[level]   = t.value;
     m_thread_scratch[level] = th.value;
     return *this;
+  }
+  TeamPolicy& set_scratch_size(int level, const Impl::PerThreadValue& th, const Impl::PerTeamValue& t) {
+    return set_scratch_size(level, t, th);
   }
   size_t team_scratch_size(int level) const { return m_team_scratch[level]; }
   size_t thread_scratch_size(int level) const { return m_thread_scratch[level]; }

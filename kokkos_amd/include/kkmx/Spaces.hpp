@@ -147,6 +147,12 @@ class MI355XHostPinnedSpace {
 };
 
 // ------------------------------------------------------------------ Serial
+// Tag asking for an execution space instance of its own
+// (core/src/Serial/Kokkos_Serial.hpp NewInstance).
+struct NewInstance {
+  explicit NewInstance() = default;
+};
+
 class Serial {
  public:
   using execution_space      = Serial;
@@ -157,8 +163,13 @@ class Serial {
   using scratch_memory_space = ScratchMemorySpace<Serial>;
 
   Serial() = default;
+  // Serial work runs inline on the calling thread, so a new instance
+  // behaves as the default one.
+  explicit Serial(NewInstance) {}
   static constexpr const char* name() { return "Serial"; }
   static int concurrency() { return 1; }
+  KOKKOS_INLINE_FUNCTION static unsigned impl_hardware_thread_id() { return 0; }
+  KOKKOS_INLINE_FUNCTION static unsigned impl_max_hardware_threads() { return 1; }
   void fence(const std::string& = "") const {}
   static void impl_static_fence(const std::string& = "") {}
   void print_configuration(std::ostream& os, bool = false) const {
