@@ -335,19 +335,22 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
       : m_space(space), m_lower(lower), m_upper(upper), m_tile(tile) {
     init_helper();
   }
-  template <class AT, size_t N>
+  // Bounds as Kokkos::Array of rank entries; the tile may list fewer
+  // dimensions (the rest are chosen at launch) and use another index type.
+  template <class AT, size_t N, class TT = AT, size_t M = N>
   MDRangePolicy(const Kokkos::Array<AT, N>& lower, const Kokkos::Array<AT, N>& upper,
-                const Kokkos::Array<AT, N>& tile = Kokkos::Array<AT, N>{})
+                const Kokkos::Array<TT, M>& tile = Kokkos::Array<TT, M>{})
       : MDRangePolicy(execution_space(), lower, upper, tile) {}
-  template <class AT, size_t N>
+  template <class AT, size_t N, class TT = AT, size_t M = N>
   MDRangePolicy(const execution_space& space, const Kokkos::Array<AT, N>& lower,
-                const Kokkos::Array<AT, N>& upper, const Kokkos::Array<AT, N>& tile = Kokkos::Array<AT, N>{})
+                const Kokkos::Array<AT, N>& upper, const Kokkos::Array<TT, M>& tile = Kokkos::Array<TT, M>{})
       : m_space(space) {
     static_assert(N == rank, "Kokkos::MDRangePolicy: bounds must have rank entries");
+    static_assert(M <= rank, "Kokkos::MDRangePolicy: the tile has more entries than the rank");
     for (unsigned r = 0; r < rank; ++r) {
       m_lower[r] = lower[r];
       m_upper[r] = upper[r];
-      m_tile[r]  = tile[r];
+      m_tile[r]  = r < M ? tile[r] : 0;
     }
     init_helper();
   }
