@@ -66,9 +66,11 @@ def measured_copy_peak(K, a, b, n_bytes, timer):
     return best
 
 
-def cpu_baseline(seconds_budget=12.0):
+def cpu_baseline(n=N, steps=20, warmup=3):
     """The oracle (OpenMP restatement of the reference's OpenMP backend) on the
-    host cores, on 2^25-element arrays of the same step, GB/s."""
+    host cores: the same 8 dispatches on arrays of the same size (2^28), each
+    step timed on its own; `warmup` untimed steps first, then the median (and
+    minimum) of `steps` timed steps, GB/s of algorithmic bytes."""
     import numpy as np
 
     import oracle as O
@@ -76,10 +78,9 @@ def cpu_baseline(seconds_budget=12.0):
     L = O.lib()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or L.or_max_threads()
     threads = max(1, min(threads, L.or_max_threads()))
-    n = 1 << 25
     a = O.fill_uniform_f64(n, 2, -1.0, 1.0)
     b = O.fill_uniform_f64(n, 3, -1.0, 1.0)
-    c = np.empty(n)
+    c = O.fill_uniform_f64(n, 4, -1.0, 1.0)
     x = O.fill_uniform_i64(n, 3, -(1 << 40), 1 << 40)
     s = O.fill_uniform_i32(n, 4, 0, 7)
     so = np.empty(n, np.int32)
@@ -96,24 +97,90 @@ def cpu_baseline(seconds_budget=12.0):
         L.or_reduce_i64(p(x), n, 0, threads)
         L.or_scan_incl_i32(p(s), p(so), n, threads, p(tot))
 
-    step()
-    bytes_per_step = sum(bpe for _, bpe in OPS) * n
-    reps, t0 = 0, time.perf_counter()
-    while True:
+    for _ in range(warmup):
         step()
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > seconds_budget or reps >= 50:
-            break
-    gbps = bytes_per_step * reps / el / 1e9
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        step()
+        ts.append(time.perf_counter() - t0)
+    bytes_per_step = sum(bpe for _, bpe in OPS) * n
+    med, mn = float(np.median(ts)), float(min(ts))
     return {
-        "value": round(gbps, 2),
+        "value": round(bytes_per_step / med / 1e9, 2),
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{reps} steps of the same 8 dispatches on 2^25-element arrays "
-        f"(oracle/kkmx_oracle.c, OpenMP, {threads} threads), {el:.1f} s",
+        "host_cpus": os.cpu_count(),
+        "median_step_s": round(med, 4),
+        "min_step_s": round(mn, 4),
+        "best_gbps": round(bytes_per_step / mn / 1e9, 2),
+        "sample": f"the same 8 dispatches on 2^{n.bit_length() - 1}-element arrays (oracle/kkmx_oracle.c, OpenMP, "
+        f"{threads} threads): {warmup} untimed steps, then the median of {steps} timed steps "
+        f"({sum(ts):.1f} s)",
     }
+
+
+def triad_exact(b, c, s=3.0, chunk=1 << 24):
+    """Correctly rounded fma(s, c, b) in float64 for |s| with at most 11
+    significant bits (s = 3.0 here), from numpy alone: s*c is exact in
+    x87 long double (64-bit significand), TwoSum gives b + s*c = S + e exactly,
+    and S is rounded to double with the double-rounding case (S exactly half
+    way between two doubles, e != 0) corrected toward e."""
+    LD = np.longdouble
+    if np.finfo(LD).nmant < 63:
+        raise RuntimeError("long double is not the x87 extended format")
+    out = np.empty_like(b)
+    for i in range(0, b.size, chunk):
+        bl = b[i:i + chunk].astype(LD)
+        yl = c[i:i + chunk].astype(LD) * LD(s)
+        S = bl + yl
+        bp = S - bl
+        e = (bl - (S - bp)) + (yl - bp)
+        d = S.astype(np.float64)
+        r = S - d.astype(LD)
+        dn = np.nextafter(d, np.where(r > 0, np.inf, -np.inf))
+        mid = (r != 0) & (r * 2 == dn.astype(LD) - d.astype(LD))
+        fix = mid & (e != 0) & (np.sign(e) == np.sign(r))
+        d[fix] = dn[fix]
+        out[i:i + chunk] = d
+    return out
+
+
+def verify_step(K, P, calls, a, b, c, x, s, so, res):
+    """Checks one more step of the timed dispatches, bit for bit, with numpy
+    on the downloaded inputs: copy, scale and add exactly; the triad against
+    the correctly rounded fused multiply-add the device performs; the three
+    int64 reductions (sums wrap like the device's); the int32 inclusive scan
+    against numpy's int32 cumulative sum (the total, 7 * 2^28 at most, fits)."""
+    ok = {}
+    ha = a.download(np.float64, N)
+    K.check(calls[0](), P)
+    hc = c.download(np.float64, N)
+    ok["copy"] = bool(np.array_equal(hc, ha))
+    K.check(calls[1](), P)
+    hb = b.download(np.float64, N)
+    ok["scale"] = bool(np.array_equal(hb, 3.0 * hc))
+    K.check(calls[2](), P)
+    hc = c.download(np.float64, N)
+    ok["add"] = bool(np.array_equal(hc, ha + hb))
+    del ha
+    K.check(calls[3](), P)
+    ok["triad"] = bool(np.array_equal(a.download(np.float64, N), triad_exact(hb, hc)))
+    del hb, hc
+    for k in (4, 5, 6):
+        K.check(calls[k](), P)
+    K.check(calls[7](), P)
+    K.fence()
+    hx = x.download(np.int64, N)
+    got = res.download(np.int64, 3)
+    ok["reduce_max_i64"] = int(got[0]) == int(hx.max())
+    ok["reduce_min_i64"] = int(got[1]) == int(hx.min())
+    ok["reduce_sum_i64"] = int(got[2]) == int(hx.sum())
+    del hx
+    ok["scan_incl_i32"] = bool(np.array_equal(so.download(np.int32, N), np.cumsum(s.download(np.int32, N),
+                                                                                   dtype=np.int32)))
+    return ok
 
 
 def _timeit(K, timer, fn, reps=5, warm=2):
@@ -530,12 +597,9 @@ def main():
     avg_ms = per_op_ms.mean(axis=0)
     op_gbps = {name: round(bpe * N / (avg_ms[k] * 1e-3) / 1e9, 1) for k, (name, bpe) in enumerate(OPS)}
 
-    # correctness of the timed results (bit-exact integer reductions, checked
-    # with numpy on the downloaded input; int64 sums wrap like the device's)
-    hx = x.download(np.int64, N)
-    got = res.download(np.int64, 3)
-    ok = int(got[0]) == int(hx.max()) and int(got[1]) == int(hx.min()) and int(got[2]) == int(hx.sum())
-    del hx
+    # correctness of the timed dispatches: one more step, every output checked
+    checks = verify_step(K, P, calls, a, b, c, x, s, so, res)
+    ok = all(checks.values())
 
     peak = measured_copy_peak(K, a.ptr, b.ptr, N * 8, K.EventTimer())
     del a, b, c, x, s, so
@@ -593,6 +657,7 @@ def main():
         "measured_copy_peak_gbps": round(peak, 1),
         "pct_of_measured_peak": {k: round(100 * v / peak, 1) for k, v in op_gbps.items()},
         "results_bit_exact": bool(ok),
+        "results_checked": checks,
         "wall_s": round(wall, 3),
         "device": {"arch": props["arch"], "cus": props["cu_count"]},
         "roofline": {
@@ -603,6 +668,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
+            "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel, "
+            "scaled by the counters' readings for a plain copy of known size (not measured in this run)",
         },
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }

@@ -176,6 +176,28 @@ def scan_excl_f64_compensated(x):
     return out, float(tot[0])
 
 
+def stream_copy(a, nthreads=1):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.empty_like(a)
+    lib().or_stream_copy(_p(a), _p(b), a.size, nthreads)
+    return b
+
+
+def stream_scale(c, s, nthreads=1):
+    c = np.ascontiguousarray(c, np.float64)
+    b = np.empty_like(c)
+    lib().or_stream_scale(_p(b), _p(c), c.size, s, nthreads)
+    return b
+
+
+def stream_add(a, b, nthreads=1):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    c = np.empty_like(a)
+    lib().or_stream_add(_p(a), _p(b), _p(c), a.size, nthreads)
+    return c
+
+
 def stream_triad(b, c, s, fused=True, nthreads=1):
     a = np.empty_like(b)
     lib().or_stream_triad(_p(a), _p(b), _p(c), b.size, s, 1 if fused else 0, nthreads)

@@ -207,6 +207,32 @@ def test_full_size_config2(paths, oracle):
     assert oracle.checksum(O.download(np.int32, N)) == oracle.checksum(oracle.scan_incl_i32(s, 8))
 
 
+def test_full_size_config2_stream(paths, oracle):
+    """2^28 doubles per array (seeds 2, 3, 4): one STREAM step in the bench's
+    order (copy c=a, scale b=3c, add c=a+b, triad a=b+3c), each output
+    compared with the oracle's by a checksum of the whole 2 GiB array."""
+    P = paths
+    N = 1 << 28
+    A, B, C = K.DeviceBuffer(N * 8), K.DeviceBuffer(N * 8), K.DeviceBuffer(N * 8)
+    run(P, P.kkmx_fill_uniform_f64(A.ptr, N, 2, -1.0, 1.0))
+    run(P, P.kkmx_fill_uniform_f64(B.ptr, N, 3, -1.0, 1.0))
+    run(P, P.kkmx_fill_uniform_f64(C.ptr, N, 4, -1.0, 1.0))
+    a = oracle.fill_uniform_f64(N, 2, -1.0, 1.0)
+    assert oracle.checksum(A.download(np.float64, N)) == oracle.checksum(a)
+    run(P, P.kkmx_stream_copy(A.ptr, C.ptr, N))
+    c = oracle.stream_copy(a, 8)
+    assert oracle.checksum(C.download(np.float64, N)) == oracle.checksum(c), "copy"
+    run(P, P.kkmx_stream_scale(B.ptr, C.ptr, N, 3.0))
+    b = oracle.stream_scale(c, 3.0, 8)
+    assert oracle.checksum(B.download(np.float64, N)) == oracle.checksum(b), "scale"
+    run(P, P.kkmx_stream_add(A.ptr, B.ptr, C.ptr, N))
+    c = oracle.stream_add(a, b, 8)
+    assert oracle.checksum(C.download(np.float64, N)) == oracle.checksum(c), "add"
+    run(P, P.kkmx_stream_triad(A.ptr, B.ptr, C.ptr, N, 3.0))
+    a = oracle.stream_triad(b, c, 3.0, fused=True, nthreads=8)
+    assert oracle.checksum(A.download(np.float64, N)) == oracle.checksum(a), "triad"
+
+
 def test_full_size_config1(paths, oracle):
     """2^24: f64 sum (seed 0) within 1e-12 of the exact sum; int32 U[0,16)
     exclusive scan into int64 (seed 1) bit-exact."""
