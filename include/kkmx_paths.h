@@ -204,6 +204,12 @@ int kkmx_set_range_config(int pattern, int block, int items, int max_blocks);
 /* MDRange parallel_for coarsening: tiles per workgroup along the slowest tile
  * dimension (0 = heuristic). */
 int kkmx_set_mdrange_depth(int depth);
+/* parallel_scan kernel selection: mode 0 = heuristic, 1 = tile-claim kernel
+ * (one ordinary launch, tiles claimed in launch order), 2 = static-order
+ * kernel (cooperative launch, workgroup b takes tiles b, b+G, ...); block > 0
+ * also sets the static kernel's lanes per workgroup, values per lane (8, 16
+ * or 32) and resident workgroups per CU (0 = as many as fit). */
+int kkmx_set_scan_mode(int mode, int block, int items, int blocks_per_cu);
 
 #ifdef __cplusplus
 }

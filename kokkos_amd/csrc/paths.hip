@@ -928,6 +928,15 @@ int kkmx_set_range_config(int pattern, int block, int items, int max_blocks) {
   return 0;
 }
 
+int kkmx_set_scan_mode(int mode, int block, int items, int blocks_per_cu) {
+  if (mode < 0 || mode > 2) return reject("kkmx_set_scan_mode: mode must be 0, 1 or 2");
+  auto& o   = Kokkos::Impl::scan_mode_override();
+  o.mode    = mode;
+  o.cfg_set = block > 0;
+  if (block > 0) o.cfg = Kokkos::Impl::ScanStaticConfig{block, items, blocks_per_cu < 0 ? 0 : blocks_per_cu};
+  return 0;
+}
+
 // ---------------------------------------------------------------- batched dispatch
 struct kkmx_batch {
   Kokkos::Experimental::BatchDispatch batch;

@@ -18,6 +18,7 @@
 #include <kkmx/Spaces.hpp>
 #include <kkmx/PartitionSpace.hpp>
 #include <kkmx/Atomic.hpp>
+#include <kkmx/Desul.hpp>
 #include <kkmx/View.hpp>
 #include <kkmx/Subview.hpp>
 #include <kkmx/Reducers.hpp>

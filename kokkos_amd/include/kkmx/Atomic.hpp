@@ -63,41 +63,57 @@ KOKKOS_FORCEINLINE_FUNCTION To bit_cast_(const From& f) {
   return t;
 }
 
-// Device lock array for types wider than 8 bytes.
+// Lock arrays for types wider than 8 bytes: one for the device, one for the
+// host, each shared by every operation and type, so any two atomic
+// operations on one address (a fetch_add and an exchange, say) take the same
+// lock.
 constexpr int kLockCount = 1 << 16;
 #if defined(__HIP__)
 __device__ inline int g_atomic_locks[kLockCount];
 #endif
+inline int g_host_atomic_locks[kLockCount];
 
 KOKKOS_INLINE_FUNCTION unsigned lock_index(const void* p) {
   auto a = reinterpret_cast<uintptr_t>(p);
   return static_cast<unsigned>((a >> 4) ^ (a >> 20)) & (kLockCount - 1);
 }
 
+// Memory scopes of the scoped operations below: the HIP scope constants,
+// plus kScopeCaller for operations that need no atomicity (desul's
+// MemoryScopeCaller).  Kokkos:: atomics use kScopeAgent (the whole GPU).
+constexpr int kScopeCaller    = -1;
+constexpr int kScopeWorkgroup = __HIP_MEMORY_SCOPE_WORKGROUP;
+constexpr int kScopeAgent     = __HIP_MEMORY_SCOPE_AGENT;
+constexpr int kScopeSystem    = __HIP_MEMORY_SCOPE_SYSTEM;
+
 // ---------------------------------------------------------- raw CAS (returns old)
-template <class T>
+template <int S, class T>
 KOKKOS_FORCEINLINE_FUNCTION T cas_word(T* p, T expected, T desired) {
   static_assert(sizeof(T) == 4 || sizeof(T) == 8, "word CAS");
   using U = typename uint_of_size<sizeof(T)>::type;
   U e     = bit_cast_<U>(expected);
   U d     = bit_cast_<U>(desired);
+  if constexpr (S == kScopeCaller) {
+    U* q = reinterpret_cast<U*>(p);
+    if (*q == e) *q = d;
+    else e = *q;
+    return bit_cast_<T>(e);
+  }
 #if defined(__HIP_DEVICE_COMPILE__)
-  __hip_atomic_compare_exchange_strong(reinterpret_cast<U*>(p), &e, d,
-                                       __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_compare_exchange_strong(reinterpret_cast<U*>(p), &e, d, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                       S == kScopeCaller ? kScopeAgent : S);
 #else
-  __atomic_compare_exchange_n(reinterpret_cast<U*>(p), &e, d, false,
-                              __ATOMIC_RELAXED, __ATOMIC_RELAXED);
+  __atomic_compare_exchange_n(reinterpret_cast<U*>(p), &e, d, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED);
 #endif
   return bit_cast_<T>(e);
 }
 
-template <class T>
+template <int S, class T>
 KOKKOS_FORCEINLINE_FUNCTION T load_word(T* p) {
   using U = typename uint_of_size<sizeof(T)>::type;
+  if constexpr (S == kScopeCaller) return *p;
 #if defined(__HIP_DEVICE_COMPILE__)
-  U v = __hip_atomic_load(reinterpret_cast<U*>(p), __ATOMIC_RELAXED,
-                          __HIP_MEMORY_SCOPE_AGENT);
+  U v = __hip_atomic_load(reinterpret_cast<U*>(p), __ATOMIC_RELAXED, S == kScopeCaller ? kScopeAgent : S);
 #else
   U v = __atomic_load_n(reinterpret_cast<U*>(p), __ATOMIC_RELAXED);
 #endif
@@ -105,13 +121,17 @@ KOKKOS_FORCEINLINE_FUNCTION T load_word(T* p) {
 }
 
 // Generic fetch-op: returns the old value; new value = op(old, val).
-template <class T, class Op>
+template <int S, class T, class Op>
 KOKKOS_INLINE_FUNCTION T fetch_oper(T* p, const T& val, Op op) {
-  if constexpr (sizeof(T) == 4 || sizeof(T) == 8) {
-    T old = load_word(p);
+  if constexpr (S == kScopeCaller) {
+    T old = *p;
+    *p    = op(old, val);
+    return old;
+  } else if constexpr (sizeof(T) == 4 || sizeof(T) == 8) {
+    T old = load_word<S>(p);
     while (true) {
       T desired = op(old, val);
-      T seen    = cas_word(p, old, desired);
+      T seen    = cas_word<S>(p, old, desired);
       if (bit_cast_<typename uint_of_size<sizeof(T)>::type>(seen) ==
           bit_cast_<typename uint_of_size<sizeof(T)>::type>(old))
         return old;
@@ -123,7 +143,7 @@ KOKKOS_INLINE_FUNCTION T fetch_oper(T* p, const T& val, Op op) {
     uint32_t* w     = reinterpret_cast<uint32_t*>(a & ~uintptr_t(3));
     unsigned shift  = static_cast<unsigned>(a & 3) * 8;
     uint32_t mask   = (sizeof(T) == 1 ? 0xffu : 0xffffu) << shift;
-    uint32_t oldw   = load_word(w);
+    uint32_t oldw   = load_word<S>(w);
     while (true) {
       T oldv;
       uint32_t bits = (oldw & mask) >> shift;
@@ -132,7 +152,7 @@ KOKKOS_INLINE_FUNCTION T fetch_oper(T* p, const T& val, Op op) {
       uint32_t nb  = 0;
       __builtin_memcpy(&nb, &newv, sizeof(T));
       uint32_t neww = (oldw & ~mask) | ((nb << shift) & mask);
-      uint32_t seen = cas_word(w, oldw, neww);
+      uint32_t seen = cas_word<S>(w, oldw, neww);
       if (seen == oldw) return oldv;
       oldw = seen;
     }
@@ -149,18 +169,17 @@ KOKKOS_INLINE_FUNCTION T fetch_oper(T* p, const T& val, Op op) {
     uint64_t pending    = __builtin_amdgcn_ballot_w64(true);
     while (pending) {
       if (lane == unsigned(__builtin_ctzll(pending))) {
-        while (__hip_atomic_exchange(lock, 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        while (__hip_atomic_exchange(lock, 1, __ATOMIC_ACQUIRE, S) != 0) {
         }
         result = *p;
         *p     = op(result, val);
-        __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, S);
       }
       pending &= pending - 1;
     }
     return result;
 #else
-    static int host_locks[kLockCount];
-    int* lock = &host_locks[lock_index(p)];
+    int* lock = &g_host_atomic_locks[lock_index(p)];
     while (__atomic_exchange_n(lock, 1, __ATOMIC_ACQUIRE) != 0) {
     }
     T result = *p;
@@ -182,113 +201,136 @@ struct OpAnd { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const
 struct OpOr  { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const T& b) const { return a | b; } };
 struct OpXor { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const T& b) const { return a ^ b; } };
 struct OpNand { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const T& b) const { return ~(a & b); } };
-struct OpLshift { template <class T, class S> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const S& b) const { return a << b; } };
-struct OpRshift { template <class T, class S> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const S& b) const { return a >> b; } };
+struct OpLshift { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const T& b) const { return T(a << b); } };
+struct OpRshift { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const T& b) const { return T(a >> b); } };
 struct OpIncMod { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const T& b) const { return a >= b ? T(0) : T(a + 1); } };
 struct OpDecMod { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const T& b) const { return (a == 0 || a > b) ? b : T(a - 1); } };
 struct OpSecond { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T&, const T& b) const { return b; } };
+struct OpFirst { template <class T> KOKKOS_FORCEINLINE_FUNCTION T operator()(const T& a, const T&) const { return a; } };
 
-}  // namespace Impl
-
-// ------------------------------------------------------------------ fetch_op
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_add(T* const p, Impl::nodeduce_t<const T> v) {
-  if constexpr (Impl::atomic_native_int<T>() || Impl::atomic_native_fp<T>()) {
+// ---------------------------------------------------------- scoped fetch_op
+// Native read-modify-writes where gfx950 has them (4/8-byte integers: add,
+// sub, min, max, and, or, xor; float/double add), CAS or locks otherwise.
+template <int S, class T>
+KOKKOS_INLINE_FUNCTION T s_fetch_add(T* p, const T v) {
+  if constexpr (S != kScopeCaller && (atomic_native_int<T>() || atomic_native_fp<T>())) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, S);
 #else
-    if constexpr (Impl::atomic_native_int<T>())
-      return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
-    else
-      return Impl::fetch_oper(p, v, Impl::OpAdd{});
+    if constexpr (atomic_native_int<T>()) return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+    else return fetch_oper<S>(p, v, OpAdd{});
 #endif
   } else {
-    return Impl::fetch_oper(p, v, Impl::OpAdd{});
+    return fetch_oper<S>(p, v, OpAdd{});
   }
 }
-
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_sub(T* const p, Impl::nodeduce_t<const T> v) {
-  if constexpr (Impl::atomic_native_int<T>()) {
+template <int S, class T>
+KOKKOS_INLINE_FUNCTION T s_fetch_sub(T* p, const T v) {
+  if constexpr (S != kScopeCaller && atomic_native_int<T>()) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __hip_atomic_fetch_add(p, T(0) - v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_add(p, T(0) - v, __ATOMIC_RELAXED, S);
 #else
     return __atomic_fetch_sub(p, v, __ATOMIC_RELAXED);
 #endif
-  } else if constexpr (Impl::atomic_native_fp<T>()) {
-    return atomic_fetch_add(p, -v);
+  } else if constexpr (S != kScopeCaller && atomic_native_fp<T>()) {
+    return s_fetch_add<S>(p, T(-v));
   } else {
-    return Impl::fetch_oper(p, v, Impl::OpSub{});
+    return fetch_oper<S>(p, v, OpSub{});
   }
 }
-
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_min(T* const p, Impl::nodeduce_t<const T> v) {
-  if constexpr (Impl::atomic_native_int<T>()) {
+template <int S, class T>
+KOKKOS_INLINE_FUNCTION T s_fetch_min(T* p, const T v) {
+  if constexpr (S != kScopeCaller && atomic_native_int<T>()) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, S);
 #else
-    return Impl::fetch_oper(p, v, Impl::OpMin{});
+    return fetch_oper<S>(p, v, OpMin{});
 #endif
   } else {
-    return Impl::fetch_oper(p, v, Impl::OpMin{});
+    return fetch_oper<S>(p, v, OpMin{});
   }
 }
-
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_max(T* const p, Impl::nodeduce_t<const T> v) {
-  if constexpr (Impl::atomic_native_int<T>()) {
+template <int S, class T>
+KOKKOS_INLINE_FUNCTION T s_fetch_max(T* p, const T v) {
+  if constexpr (S != kScopeCaller && atomic_native_int<T>()) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, S);
 #else
-    return Impl::fetch_oper(p, v, Impl::OpMax{});
+    return fetch_oper<S>(p, v, OpMax{});
 #endif
   } else {
-    return Impl::fetch_oper(p, v, Impl::OpMax{});
+    return fetch_oper<S>(p, v, OpMax{});
   }
 }
-
-#define KKMX_ATOMIC_BITOP(NAME, HIPB, GCCB, OPS)                                   \
-  template <class T>                                                               \
-  KOKKOS_INLINE_FUNCTION T atomic_fetch_##NAME(T* const p,                         \
-                                               Impl::nodeduce_t<const T> v) {      \
-    if constexpr (Impl::atomic_native_int<T>()) {                                  \
-      KOKKOS_IF_ON_DEVICE(                                                         \
-          (return HIPB(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);))        \
-      KOKKOS_IF_ON_HOST((return GCCB(p, v, __ATOMIC_RELAXED);))                    \
-    } else {                                                                       \
-      return Impl::fetch_oper(p, v, Impl::OPS{});                                  \
-    }                                                                              \
+#define KKMX_ATOMIC_BITOP(NAME, HIPB, GCCB, OPS)                     \
+  template <int S, class T>                                          \
+  KOKKOS_INLINE_FUNCTION T s_fetch_##NAME(T* p, const T v) {         \
+    if constexpr (S != kScopeCaller && atomic_native_int<T>()) {     \
+      KOKKOS_IF_ON_DEVICE((return HIPB(p, v, __ATOMIC_RELAXED, S);)) \
+      KOKKOS_IF_ON_HOST((return GCCB(p, v, __ATOMIC_RELAXED);))      \
+    } else {                                                         \
+      return fetch_oper<S>(p, v, OPS{});                             \
+    }                                                                \
   }
 KKMX_ATOMIC_BITOP(and, __hip_atomic_fetch_and, __atomic_fetch_and, OpAnd)
 KKMX_ATOMIC_BITOP(or, __hip_atomic_fetch_or, __atomic_fetch_or, OpOr)
 KKMX_ATOMIC_BITOP(xor, __hip_atomic_fetch_xor, __atomic_fetch_xor, OpXor)
 #undef KKMX_ATOMIC_BITOP
 
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_mul(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper(p, v, Impl::OpMul{}); }
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_div(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper(p, v, Impl::OpDiv{}); }
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_mod(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper(p, v, Impl::OpMod{}); }
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_nand(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper(p, v, Impl::OpNand{}); }
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_lshift(T* const p, const unsigned v) {
-  return Impl::fetch_oper(p, T(v), [] KOKKOS_FUNCTION(const T& a, const T& b) { return T(a << b); });
+template <int S, class T>
+KOKKOS_INLINE_FUNCTION T s_load(T* p) {
+  if constexpr (sizeof(T) == 4 || sizeof(T) == 8) return load_word<S>(p);
+  else return fetch_oper<S>(p, T{}, OpFirst{});
 }
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_rshift(T* const p, const unsigned v) {
-  return Impl::fetch_oper(p, T(v), [] KOKKOS_FUNCTION(const T& a, const T& b) { return T(a >> b); });
+template <int S, class T>
+KOKKOS_INLINE_FUNCTION void s_store(T* p, const T v) {
+  if constexpr (S == kScopeCaller) {
+    *p = v;
+  } else if constexpr (sizeof(T) == 4 || sizeof(T) == 8) {
+    using U = typename uint_of_size<sizeof(T)>::type;
+    KOKKOS_IF_ON_DEVICE((__hip_atomic_store(reinterpret_cast<U*>(p), bit_cast_<U>(v), __ATOMIC_RELAXED, S);))
+    KOKKOS_IF_ON_HOST((__atomic_store_n(reinterpret_cast<U*>(p), bit_cast_<U>(v), __ATOMIC_RELAXED);))
+  } else {
+    (void)fetch_oper<S>(p, v, OpSecond{});
+  }
 }
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_inc(T* const p) { return atomic_fetch_add(p, T(1)); }
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_dec(T* const p) { return atomic_fetch_sub(p, T(1)); }
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_inc_mod(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper(p, v, Impl::OpIncMod{}); }
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_fetch_dec_mod(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper(p, v, Impl::OpDecMod{}); }
+// Compare-exchange: returns the old value; stores `desired` when the old
+// value equals `expected` bit for bit.
+template <int S, class T>
+KOKKOS_INLINE_FUNCTION T s_compare_exchange(T* p, const T expected, const T desired) {
+  if constexpr (S != kScopeCaller && (sizeof(T) == 4 || sizeof(T) == 8)) {
+    return cas_word<S>(p, expected, desired);
+  } else {
+    T old{};
+    (void)fetch_oper<S>(p, desired, [&old, expected] KOKKOS_FUNCTION(const T& a, const T& d) {
+      old = a;
+      return (__builtin_memcmp(&a, &expected, sizeof(T)) == 0) ? d : a;
+    });
+    return old;
+  }
+}
+
+}  // namespace Impl
+
+// ------------------------------------------------------------------ fetch_op
+// Relaxed order, device (agent) scope, as core/src/Kokkos_Atomics_Desul_Wrapper.hpp:46-50.
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_add(T* const p, Impl::nodeduce_t<const T> v) { return Impl::s_fetch_add<Impl::kScopeAgent>(p, v); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_sub(T* const p, Impl::nodeduce_t<const T> v) { return Impl::s_fetch_sub<Impl::kScopeAgent>(p, v); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_min(T* const p, Impl::nodeduce_t<const T> v) { return Impl::s_fetch_min<Impl::kScopeAgent>(p, v); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_max(T* const p, Impl::nodeduce_t<const T> v) { return Impl::s_fetch_max<Impl::kScopeAgent>(p, v); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_and(T* const p, Impl::nodeduce_t<const T> v) { return Impl::s_fetch_and<Impl::kScopeAgent>(p, v); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_or(T* const p, Impl::nodeduce_t<const T> v) { return Impl::s_fetch_or<Impl::kScopeAgent>(p, v); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_xor(T* const p, Impl::nodeduce_t<const T> v) { return Impl::s_fetch_xor<Impl::kScopeAgent>(p, v); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_mul(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper<Impl::kScopeAgent>(p, v, Impl::OpMul{}); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_div(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper<Impl::kScopeAgent>(p, v, Impl::OpDiv{}); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_mod(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper<Impl::kScopeAgent>(p, v, Impl::OpMod{}); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_nand(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper<Impl::kScopeAgent>(p, v, Impl::OpNand{}); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_lshift(T* const p, const unsigned v) { return Impl::fetch_oper<Impl::kScopeAgent>(p, T(v), Impl::OpLshift{}); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_rshift(T* const p, const unsigned v) { return Impl::fetch_oper<Impl::kScopeAgent>(p, T(v), Impl::OpRshift{}); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_inc(T* const p) { return atomic_fetch_add(p, T(1)); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_dec(T* const p) { return atomic_fetch_sub(p, T(1)); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_inc_mod(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper<Impl::kScopeAgent>(p, v, Impl::OpIncMod{}); }
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_fetch_dec_mod(T* const p, Impl::nodeduce_t<const T> v) { return Impl::fetch_oper<Impl::kScopeAgent>(p, v, Impl::OpDecMod{}); }
 
 // ------------------------------------------------------------------ op_fetch
 template <class T> KOKKOS_INLINE_FUNCTION T atomic_add_fetch(T* const p, Impl::nodeduce_t<const T> v) { return atomic_fetch_add(p, v) + v; }
@@ -307,7 +349,7 @@ template <class T> KOKKOS_INLINE_FUNCTION T atomic_rshift_fetch(T* const p, cons
 template <class T> KOKKOS_INLINE_FUNCTION T atomic_inc_fetch(T* const p) { return atomic_fetch_inc(p) + T(1); }
 template <class T> KOKKOS_INLINE_FUNCTION T atomic_dec_fetch(T* const p) { return atomic_fetch_dec(p) - T(1); }
 
-// ------------------------------------------------------------------ op (no return)
+// ------------------------------------------------------------------ op (no result)
 template <class T> KOKKOS_INLINE_FUNCTION void atomic_add(T* const p, Impl::nodeduce_t<const T> v) { (void)atomic_fetch_add(p, v); }
 template <class T> KOKKOS_INLINE_FUNCTION void atomic_sub(T* const p, Impl::nodeduce_t<const T> v) { (void)atomic_fetch_sub(p, v); }
 template <class T> KOKKOS_INLINE_FUNCTION void atomic_min(T* const p, Impl::nodeduce_t<const T> v) { (void)atomic_fetch_min(p, v); }
@@ -327,29 +369,11 @@ template <class T> KOKKOS_INLINE_FUNCTION void atomic_increment(T* const p) { (v
 template <class T> KOKKOS_INLINE_FUNCTION void atomic_decrement(T* const p) { (void)atomic_fetch_dec(p); }
 
 // ------------------------------------------------------------------ load/store/exchange/CAS
-template <class T>
-KOKKOS_INLINE_FUNCTION T atomic_load(T* const p) {
-  if constexpr (sizeof(T) == 4 || sizeof(T) == 8) {
-    return Impl::load_word(p);
-  } else {
-    return Impl::fetch_oper(p, T{}, [] KOKKOS_FUNCTION(const T& a, const T&) { return a; });
-  }
-}
-template <class T>
-KOKKOS_INLINE_FUNCTION void atomic_store(T* const p, Impl::nodeduce_t<const T> v) {
-  if constexpr (sizeof(T) == 4 || sizeof(T) == 8) {
-    using U = typename Impl::uint_of_size<sizeof(T)>::type;
-    KOKKOS_IF_ON_DEVICE((__hip_atomic_store(reinterpret_cast<U*>(p), Impl::bit_cast_<U>(v),
-                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);))
-    KOKKOS_IF_ON_HOST((__atomic_store_n(reinterpret_cast<U*>(p), Impl::bit_cast_<U>(v),
-                                        __ATOMIC_RELAXED);))
-  } else {
-    (void)Impl::fetch_oper(p, v, Impl::OpSecond{});
-  }
-}
+template <class T> KOKKOS_INLINE_FUNCTION T atomic_load(T* const p) { return Impl::s_load<Impl::kScopeAgent>(p); }
+template <class T> KOKKOS_INLINE_FUNCTION void atomic_store(T* const p, Impl::nodeduce_t<const T> v) { Impl::s_store<Impl::kScopeAgent>(p, v); }
 template <class T>
 KOKKOS_INLINE_FUNCTION T atomic_exchange(T* const p, Impl::nodeduce_t<const T> v) {
-  return Impl::fetch_oper(p, v, Impl::OpSecond{});
+  return Impl::fetch_oper<Impl::kScopeAgent>(p, v, Impl::OpSecond{});
 }
 template <class T>
 KOKKOS_INLINE_FUNCTION void atomic_assign(T* const p, Impl::nodeduce_t<const T> v) {
@@ -358,16 +382,7 @@ KOKKOS_INLINE_FUNCTION void atomic_assign(T* const p, Impl::nodeduce_t<const T> 
 template <class T>
 KOKKOS_INLINE_FUNCTION T atomic_compare_exchange(T* const p, Impl::nodeduce_t<const T> expected,
                                                  Impl::nodeduce_t<const T> desired) {
-  if constexpr (sizeof(T) == 4 || sizeof(T) == 8) {
-    return Impl::cas_word(p, expected, desired);
-  } else {
-    T old{};
-    (void)Impl::fetch_oper(p, desired, [&old, expected] KOKKOS_FUNCTION(const T& a, const T& d) {
-      old = a;
-      return (__builtin_memcmp(&a, &expected, sizeof(T)) == 0) ? d : a;
-    });
-    return old;
-  }
+  return Impl::s_compare_exchange<Impl::kScopeAgent>(p, expected, desired);
 }
 template <class T>
 KOKKOS_INLINE_FUNCTION bool atomic_compare_exchange_strong(T* const p, Impl::nodeduce_t<const T> expected,

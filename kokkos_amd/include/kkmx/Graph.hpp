@@ -41,11 +41,17 @@ struct GraphNodeImpl {
 
 template <class Exec>
 class GraphImpl;
+template <class Exec>
+struct GraphNodeAggregateImpl;
+template <class Exec>
+struct GraphNodeRootImpl;
 
 template <>
 class GraphImpl<MI355X> {
  public:
-  using node_t = GraphNodeImpl<MI355X>;
+  using node_t           = GraphNodeImpl<MI355X>;
+  using aggregate_impl_t = GraphNodeAggregateImpl<MI355X>;
+  using root_node_impl_t = GraphNodeRootImpl<MI355X>;
   explicit GraphImpl(const MI355X& ex) : m_exec(ex) {}
   ~GraphImpl() {
     if (m_exec_graph) {
@@ -173,7 +179,9 @@ class GraphImpl<MI355X> {
 template <>
 class GraphImpl<Serial> {
  public:
-  using node_t = GraphNodeImpl<Serial>;
+  using node_t           = GraphNodeImpl<Serial>;
+  using aggregate_impl_t = GraphNodeAggregateImpl<Serial>;
+  using root_node_impl_t = GraphNodeRootImpl<Serial>;
   explicit GraphImpl(const Serial& ex) : m_exec(ex) {}
   node_t* create_node() {
     m_nodes.emplace_back(new node_t());
@@ -216,61 +224,188 @@ Policy rebind(const Policy& p, const Exec& e) {
   return q;
 }
 
+// Node kinds at the type level.  A GraphNodeRef's type names the kind of node
+// it refers to and the type of its predecessor's reference, as the
+// reference's node references do (core/src/Kokkos_GraphNode.hpp:41-60); the
+// nodes themselves are type-erased GraphNodeImpl records, so these types only
+// describe: a kernel recorded from parallel_for / reduce / scan, a single
+// call (`then`), a user capture, and the join of `when_all`.
+template <class Exec, class Policy, class Functor, class PatternTag, class... Args>
+struct GraphNodeKernelImpl {
+  using execution_space = Exec;
+  using policy_type     = Policy;
+  using functor_type    = Functor;
+  using pattern_tag     = PatternTag;
+};
+template <class Exec, class Functor>
+struct GraphNodeThenImpl {
+  using execution_space = Exec;
+  using functor_type    = Functor;
+};
+template <class Exec, class Functor>
+struct GraphNodeCaptureImpl {
+  using execution_space = Exec;
+  using functor_type    = Functor;
+};
+template <class Exec>
+struct GraphNodeAggregateImpl {
+  using execution_space = Exec;
+};
+template <class Exec>
+struct GraphNodeRootImpl {
+  using execution_space = Exec;
+};
+
+template <class T>
+struct is_graph_kernel : std::false_type {};
+template <class Exec, class Policy, class Functor, class PatternTag, class... Args>
+struct is_graph_kernel<GraphNodeKernelImpl<Exec, Policy, Functor, PatternTag, Args...>> : std::true_type {};
+template <class Exec, class Functor>
+struct is_graph_kernel<GraphNodeThenImpl<Exec, Functor>> : std::true_type {};
+template <class T>
+inline constexpr bool is_graph_kernel_v = is_graph_kernel<T>::value;
+
+template <class T>
+struct is_graph_capture : std::false_type {};
+template <class Exec, class Functor>
+struct is_graph_capture<GraphNodeCaptureImpl<Exec, Functor>> : std::true_type {};
+template <class T>
+inline constexpr bool is_graph_capture_v = is_graph_capture<T>::value;
+
+// The policy type of a kernel node: the user's policy with IsGraphKernelTag
+// appended to its traits.
+template <class Policy>
+struct graph_kernel_policy;
+template <template <class...> class P, class... Traits>
+struct graph_kernel_policy<P<Traits...>> {
+  using type = P<Traits..., IsGraphKernelTag>;
+};
+template <class Policy>
+using graph_kernel_policy_t = typename graph_kernel_policy<Policy>::type;
+
 }  // namespace Impl
 
 namespace Experimental {
 
-template <class Exec>
+// The kernel / predecessor parameter of a reference whose node kind is not
+// named in its type (core/src/Kokkos_Graph_fwd.hpp:29).
+struct TypeErasedTag {};
+
+template <class Exec = DefaultExecutionSpace>
 class Graph;
 
-template <class Exec>
+template <class Exec, class Kernel = TypeErasedTag, class Predecessor = TypeErasedTag>
 class GraphNodeRef {
- public:
-  using execution_space = Exec;
-  GraphNodeRef(std::shared_ptr<Impl::GraphImpl<Exec>> g, Impl::GraphNodeImpl<Exec>* n) : m_graph(std::move(g)), m_node(n) {}
+  template <class, class, class>
+  friend class GraphNodeRef;
 
-  template <class Policy, class F>
-  GraphNodeRef then_parallel_for(std::string label, const Policy& policy_in, const F& f) const {
-    auto policy = Impl::to_policy<F>(policy_in);
-    return add([label, policy, f](const Exec& e) { Kokkos::parallel_for(label, Impl::rebind(policy, e), f); });
+ public:
+  using execution_space   = Exec;
+  using graph_kernel      = Kernel;
+  using graph_predecessor = Predecessor;
+
+  GraphNodeRef() = default;
+  GraphNodeRef(std::shared_ptr<Impl::GraphImpl<Exec>> g, Impl::GraphNodeImpl<Exec>* n) : m_graph(std::move(g)), m_node(n) {}
+  // A reference converts to one that names less of its type (the kernel,
+  // the predecessor or both replaced by TypeErasedTag).
+  template <class K2, class P2,
+            class = std::enable_if_t<!std::is_same<GraphNodeRef, GraphNodeRef<Exec, K2, P2>>::value &&
+                                     (std::is_same<Kernel, TypeErasedTag>::value || std::is_same<Kernel, K2>::value) &&
+                                     (std::is_same<Predecessor, TypeErasedTag>::value ||
+                                      std::is_same<Predecessor, P2>::value)>>
+  GraphNodeRef(const GraphNodeRef<Exec, K2, P2>& other) : m_graph(other.m_graph), m_node(other.m_node) {}
+
+  template <class Policy, class F, class = std::enable_if_t<is_execution_policy<Impl::remove_cvref_t<Policy>>::value>>
+  auto then_parallel_for(std::string label, const Policy& policy_in, const F& f) const {
+    using kernel_t = Impl::GraphNodeKernelImpl<Exec, Impl::graph_kernel_policy_t<Impl::remove_cvref_t<Policy>>,
+                                               std::decay_t<F>, ParallelForTag>;
+    const auto policy = policy_in;
+    return add<kernel_t>([label, policy, f](const Exec& e) { Kokkos::parallel_for(label, Impl::rebind(policy, e), f); });
   }
-  template <class Policy, class F,
-            class = std::enable_if_t<is_execution_policy<Policy>::value || std::is_integral<Policy>::value>>
-  GraphNodeRef then_parallel_for(const Policy& policy, const F& f) const {
+  template <class Policy, class F, class = std::enable_if_t<is_execution_policy<Impl::remove_cvref_t<Policy>>::value>>
+  auto then_parallel_for(const Policy& policy, const F& f) const {
     return then_parallel_for(std::string(), policy, f);
   }
+  template <class F>
+  auto then_parallel_for(std::string label, std::size_t n, const F& f) const {
+    return then_parallel_for(std::move(label), RangePolicy<Exec>(0, n), f);
+  }
+  template <class F>
+  auto then_parallel_for(std::size_t n, const F& f) const {
+    return then_parallel_for(std::string(), RangePolicy<Exec>(0, n), f);
+  }
 
-  template <class Policy, class F, class Ret>
-  GraphNodeRef then_parallel_reduce(std::string label, const Policy& policy_in, const F& f, const Ret& ret) const {
+  template <class Policy, class F, class Ret,
+            class = std::enable_if_t<is_execution_policy<Impl::remove_cvref_t<Policy>>::value>>
+  auto then_parallel_reduce(std::string label, const Policy& policy_in, const F& f, const Ret& ret) const {
     check_result<Ret>();
-    auto policy = Impl::to_policy<F>(policy_in);
-    return add([label, policy, f, ret](const Exec& e) {
+    using kernel_t = Impl::GraphNodeKernelImpl<Exec, Impl::graph_kernel_policy_t<Impl::remove_cvref_t<Policy>>,
+                                               std::decay_t<F>, ParallelReduceTag, Ret>;
+    const auto policy = policy_in;
+    return add<kernel_t>([label, policy, f, ret](const Exec& e) {
       Ret r = ret;
       Kokkos::parallel_reduce(label, Impl::rebind(policy, e), f, r);
     });
   }
   template <class Policy, class F, class Ret,
-            class = std::enable_if_t<is_execution_policy<Policy>::value || std::is_integral<Policy>::value>>
-  GraphNodeRef then_parallel_reduce(const Policy& policy, const F& f, const Ret& ret) const {
+            class = std::enable_if_t<is_execution_policy<Impl::remove_cvref_t<Policy>>::value>>
+  auto then_parallel_reduce(const Policy& policy, const F& f, const Ret& ret) const {
     return then_parallel_reduce(std::string(), policy, f, ret);
   }
-
-  template <class Policy, class F>
-  GraphNodeRef then_parallel_scan(std::string label, const Policy& policy_in, const F& f) const {
-    auto policy = Impl::to_policy<F>(policy_in);
-    return add([label, policy, f](const Exec& e) { Kokkos::parallel_scan(label, Impl::rebind(policy, e), f); });
+  template <class F, class Ret>
+  auto then_parallel_reduce(std::string label, typename Exec::size_type n, const F& f, const Ret& ret) const {
+    return then_parallel_reduce(std::move(label), RangePolicy<Exec>(0, n), f, ret);
   }
-  template <class Policy, class F,
-            class = std::enable_if_t<is_execution_policy<Policy>::value || std::is_integral<Policy>::value>>
-  GraphNodeRef then_parallel_scan(const Policy& policy, const F& f) const {
+  template <class F, class Ret>
+  auto then_parallel_reduce(typename Exec::size_type n, const F& f, const Ret& ret) const {
+    return then_parallel_reduce(std::string(), RangePolicy<Exec>(0, n), f, ret);
+  }
+
+  template <class Policy, class F, class = std::enable_if_t<is_execution_policy<Impl::remove_cvref_t<Policy>>::value>>
+  auto then_parallel_scan(std::string label, const Policy& policy_in, const F& f) const {
+    using kernel_t = Impl::GraphNodeKernelImpl<Exec, Impl::graph_kernel_policy_t<Impl::remove_cvref_t<Policy>>,
+                                               std::decay_t<F>, ParallelScanTag>;
+    const auto policy = policy_in;
+    return add<kernel_t>([label, policy, f](const Exec& e) { Kokkos::parallel_scan(label, Impl::rebind(policy, e), f); });
+  }
+  template <class Policy, class F, class = std::enable_if_t<is_execution_policy<Impl::remove_cvref_t<Policy>>::value>>
+  auto then_parallel_scan(const Policy& policy, const F& f) const {
     return then_parallel_scan(std::string(), policy, f);
   }
-
-  // Run functor() once on the execution space.
   template <class F>
-  GraphNodeRef then(std::string label, const F& f) const {
-    return add([label, f](const Exec& e) {
-      Kokkos::parallel_for(label, RangePolicy<Exec, IndexType<int64_t>>(e, 0, 1), Impl::ThenWrapper<F>{f});
+  auto then_parallel_scan(std::string label, typename Exec::size_type n, const F& f) const {
+    return then_parallel_scan(std::move(label), RangePolicy<Exec>(0, n), f);
+  }
+  template <class F>
+  auto then_parallel_scan(typename Exec::size_type n, const F& f) const {
+    return then_parallel_scan(std::string(), RangePolicy<Exec>(0, n), f);
+  }
+
+  // Run functor() once.  The instance argument names the space the call is
+  // meant for; inside the graph it runs on the graph's node instance.
+  template <class Label, class F,
+            class = std::enable_if_t<std::is_invocable_r<void, const Impl::remove_cvref_t<F>>::value>>
+  auto then(Label&& label, const Exec&, const F& f) const {
+    using kernel_t = Impl::GraphNodeThenImpl<Exec, Impl::remove_cvref_t<F>>;
+    std::string name(std::forward<Label>(label));
+    return add<kernel_t>([name, f](const Exec& e) {
+      Kokkos::parallel_for(name, RangePolicy<Exec, IndexType<int64_t>>(e, 0, 1), Impl::ThenWrapper<F>{f});
+    });
+  }
+  template <class Label, class F,
+            class = std::enable_if_t<std::is_invocable_r<void, const Impl::remove_cvref_t<F>>::value>>
+  auto then(Label&& label, const F& f) const {
+    return then(std::forward<Label>(label), Exec{}, f);
+  }
+
+  // Work the caller launches itself on the given instance's stream (a
+  // library call, a hand-written kernel), recorded into the graph by stream
+  // capture like every other node.
+  template <class F, class = std::enable_if_t<std::is_invocable_r<void, const Impl::remove_cvref_t<F>, const Exec&>::value>>
+  auto mi355x_capture(const Exec&, const F& f) const {
+    using kernel_t = Impl::GraphNodeCaptureImpl<Exec, Impl::remove_cvref_t<F>>;
+    return add<kernel_t>([f](const Exec& e) {
+      if (!g_size_only_dispatch_for(e)) f(e);
     });
   }
 
@@ -279,7 +414,12 @@ class GraphNodeRef {
 
  private:
   std::shared_ptr<Impl::GraphImpl<Exec>> m_graph;
-  Impl::GraphNodeImpl<Exec>* m_node;
+  Impl::GraphNodeImpl<Exec>* m_node = nullptr;
+
+  static bool g_size_only_dispatch_for(const Exec&) {
+    if constexpr (std::is_same<Exec, MI355X>::value) return Impl::g_size_only_dispatch;
+    else return false;
+  }
 
   template <class Ret>
   static void check_result() {
@@ -294,13 +434,14 @@ class GraphNodeRef {
                     "Kokkos::Graph: reductions need a View or a reducer on a View as result");
     }
   }
-  GraphNodeRef add(std::function<void(const Exec&)> launch) const {
+  template <class KernelT>
+  GraphNodeRef<Exec, KernelT, GraphNodeRef> add(std::function<void(const Exec&)> launch) const {
     if (m_graph->instantiated())
       Kokkos::Impl::throw_runtime_exception("Kokkos::Graph: nodes cannot be added after instantiate()");
     auto* n   = m_graph->create_node();
     n->launch = std::move(launch);
     n->preds.push_back(m_node);
-    return GraphNodeRef(m_graph, n);
+    return GraphNodeRef<Exec, KernelT, GraphNodeRef>(m_graph, n);
   }
 };
 
@@ -308,9 +449,11 @@ template <class Exec>
 class Graph {
  public:
   using execution_space = Exec;
+  using graph           = Graph;
+  using root_t          = GraphNodeRef<Exec>;
   explicit Graph(Exec ex = Exec{}) : m_impl(std::make_shared<Impl::GraphImpl<Exec>>(ex)) { m_impl->root(); }
   const Exec& get_execution_space() const { return m_impl->exec(); }
-  GraphNodeRef<Exec> root_node() const { return GraphNodeRef<Exec>(m_impl, m_impl->root()); }
+  root_t root_node() const { return root_t(m_impl, m_impl->root()); }
   void instantiate() { m_impl->instantiate(); }
   void submit(const Exec& ex) const { m_impl->submit(ex); }
   void submit() const { m_impl->submit(m_impl->exec()); }
@@ -334,15 +477,16 @@ std::enable_if_t<!is_execution_space<Impl::remove_cvref_t<Closure>>::value, Grap
   return create_graph(Exec{}, static_cast<Closure&&>(closure));
 }
 
-template <class Exec, class... More>
-GraphNodeRef<Exec> when_all(const GraphNodeRef<Exec>& first, const More&... more) {
+template <class Exec, class K, class P, class... More>
+GraphNodeRef<Exec, Impl::GraphNodeAggregateImpl<Exec>, TypeErasedTag> when_all(const GraphNodeRef<Exec, K, P>& first,
+                                                                                const More&... more) {
   auto g = first.impl_graph();
   if (g->instantiated())
     Kokkos::Impl::throw_runtime_exception("Kokkos::Graph: nodes cannot be added after instantiate()");
   auto* n = g->create_node();
   n->preds.push_back(first.impl_node());
   (n->preds.push_back(more.impl_node()), ...);
-  return GraphNodeRef<Exec>(g, n);
+  return GraphNodeRef<Exec, Impl::GraphNodeAggregateImpl<Exec>, TypeErasedTag>(g, n);
 }
 
 }  // namespace Experimental

@@ -155,13 +155,23 @@ class ParallelReduce<CR, MDRangePolicy<Traits...>, Serial> {
     constexpr unsigned R = Policy::rank;
     int64_t lo[R], hi[R];
     for (unsigned r = 0; r < R; ++r) { lo[r] = m_p.m_lower[r]; hi[r] = m_p.m_upper[r]; }
-    V v;
-    m_cr.init(v);
-    md_host_loop<R, Policy::inner_direction == Iterate::Left>(lo, hi, [&](const int64_t* i) {
-      md_call<typename Policy::work_tag, typename CR::functor_type, R>(m_cr.functor(), i, v);
-    });
-    m_cr.final(v);
-    if (m_result) *m_result = v;
+    if constexpr (CR::is_array) {
+      std::vector<V> tmp(m_cr.value_count());
+      V* v = m_result ? m_result : tmp.data();
+      m_cr.init_array(v);
+      md_host_loop<R, Policy::inner_direction == Iterate::Left>(lo, hi, [&](const int64_t* i) {
+        md_call<typename Policy::work_tag, typename CR::functor_type, R>(m_cr.functor(), i, v);
+      });
+      m_cr.final_array(v);
+    } else {
+      V v;
+      m_cr.init(v);
+      md_host_loop<R, Policy::inner_direction == Iterate::Left>(lo, hi, [&](const int64_t* i) {
+        md_call<typename Policy::work_tag, typename CR::functor_type, R>(m_cr.functor(), i, v);
+      });
+      m_cr.final(v);
+      if (m_result) *m_result = v;
+    }
   }
 
  private:

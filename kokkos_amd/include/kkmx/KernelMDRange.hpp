@@ -220,6 +220,84 @@ __global__ __launch_bounds__(1024) void k_md_reduce(const CR cr, const MDLaunch 
   }
 }
 
+// Runtime-length results (value_type T[] with value_count entries): every
+// lane keeps its value_count accumulators in `acc` (dynamic LDS, or a global
+// slice per workgroup when they do not fit), the workgroup combines them in
+// an ordered pairwise tree, and the last workgroup joins the per-workgroup
+// partials in workgroup order.
+template <class CR, class Index, unsigned R, bool InnerLeft, bool OuterLeft>
+__global__ __launch_bounds__(1024) void k_md_reduce_array(const CR cr, const MDLaunch p,
+                                                          typename CR::value_type* gacc,
+                                                          typename CR::value_type* partials, unsigned* counter,
+                                                          typename CR::value_type* result) {
+  using V   = typename CR::value_type;
+  using Tag = typename CR::work_tag;
+  extern __shared__ __align__(16) unsigned char kkmx_dyn_lds[];
+  __shared__ int s_last;
+  const unsigned n = cr.value_count();
+  V* base          = gacc ? gacc + static_cast<size_t>(blockIdx.x) * blockDim.x * n : reinterpret_cast<V*>(kkmx_dyn_lds);
+  V* mine          = base + static_cast<size_t>(threadIdx.x) * n;
+  cr.init_array(mine);
+  if (static_cast<int>(threadIdx.x) < p.tile_points) {
+    constexpr int LD = OuterLeft ? int(R) - 1 : 0;
+    int64_t off[6];
+    md_lane_offsets<R, InnerLeft>(p, threadIdx.x, off);
+    int64_t ext[6];
+    for (unsigned d = 0; d < R; ++d) ext[d] = p.tile_end[d];
+    ext[LD]          = (p.tile_end[LD] + p.depth - 1) / p.depth;
+    const bool small = p.groups <= int64_t(0xffffffff);
+    for (int64_t b = blockIdx.x; b < p.groups; b += gridDim.x) {
+      int64_t tc[6], idx[6];
+      md_tile_coords<R, OuterLeft>(b, ext, small, tc);
+      bool in = true;
+      for (unsigned d = 0; d < R; ++d) {
+        if (int(d) == LD) continue;
+        idx[d] = p.lower[d] + tc[d] * p.tile[d] + off[d];
+        in &= idx[d] < p.upper[d];
+      }
+      if (!in) continue;
+      const int64_t t0   = tc[LD] * p.depth;
+      const int64_t left = p.tile_end[LD] - t0;
+      const int qn       = left < p.depth ? static_cast<int>(left) : p.depth;
+      int64_t i          = p.lower[LD] + t0 * p.tile[LD] + off[LD];
+      for (int q = 0; q < qn; ++q, i += p.tile[LD]) {
+        if (i < p.upper[LD]) {
+          idx[LD] = i;
+          md_invoke<Tag, Index, R>(cr.functor(), idx, mine);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (unsigned s = 1; s < blockDim.x; s <<= 1) {
+    if ((threadIdx.x % (2 * s)) == 0 && threadIdx.x + s < blockDim.x)
+      cr.join_array(mine, base + static_cast<size_t>(threadIdx.x + s) * n);
+    __syncthreads();
+  }
+  if (gridDim.x == 1) {
+    if (threadIdx.x == 0) {
+      cr.final_array(base);
+      for (unsigned k = 0; k < n; ++k) result[k] = base[k];
+    }
+    return;
+  }
+  if (threadIdx.x == 0) {
+    for (unsigned k = 0; k < n; ++k) partials[static_cast<size_t>(blockIdx.x) * n + k] = base[k];
+    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    s_last              = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (threadIdx.x == 0) {
+    cr.init_array(base);
+    for (unsigned b = 0; b < gridDim.x; ++b) cr.join_array(base, partials + static_cast<size_t>(b) * n);
+    cr.final_array(base);
+    for (unsigned k = 0; k < n; ++k) result[k] = base[k];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 template <class Policy>
 MDLaunch md_launch(const Policy& p) {
   MDLaunch l{};
@@ -275,42 +353,64 @@ inline void md_set_depth(MDLaunch& l, int block) {
 // the next 1 << (c / 4), the others 1 (reference MDRangeTuner,
 // core/src/impl/Kokkos_Tools_Generic.hpp:106-122,340-353).
 //
-// Without a tool, reductions over tiles the user left to the implementation
-// use 128 x 8 tiles when the extents allow: the MDRange sweeps measure them
-// 1.5-3x faster than the 64 x 4 default, which stays for parallel_for (within
-// a few percent of the best tile there).
+// Without a tool, such policies launch the shape the MDRange sweeps measured
+// best: parallel_for 64 x 4 (the fastest dimension shrunk to the next power
+// of two of a shorter extent), reductions 128 x 8 when the extents allow
+// (1.5-3x faster than 64 x 4 there); the others 1.
+template <class Policy>
+typename Policy::tile_type md_heuristic_tile(const Policy& p, int pattern) {
+  constexpr unsigned R = Policy::rank;
+  const bool left      = Policy::inner_direction == Iterate::Left;
+  const int fast = left ? 0 : int(R) - 1, next = left ? 1 : int(R) - 2;
+  const unsigned unset = p.impl_unset_tile_mask();
+  const int64_t ef = p.upper()[fast] - p.lower()[fast], en = p.upper()[next] - p.lower()[next];
+  int64_t want_fast = 64, want_next = 4;
+  if (pattern == 1 && ef >= 128 && en >= 8) {
+    want_fast = 128;
+    want_next = 8;
+  } else if (ef > 0 && ef < 64) {
+    want_fast = 1;
+    while (want_fast < ef) want_fast *= 2;
+  }
+  typename Policy::tile_type t = p.tile_size();
+  int64_t prod                 = 1;
+  for (unsigned r = 0; r < R; ++r)
+    if (!(unset & (1u << r))) prod *= t[r];
+  for (unsigned k = 0; k < R; ++k) {
+    const int r = left ? int(k) : int(R) - 1 - int(k);
+    if (!(unset & (1u << r))) continue;
+    int64_t v = r == fast ? want_fast : r == next ? want_next : 1;
+    while (v > 1 && prod * v > 1024) v /= 2;
+    t[r] = v;
+    prod *= v;
+  }
+  return t;
+}
 template <class Policy, class F>
 Policy md_tuned_policy(const Policy& p, int pattern, size_t value_bytes, bool floating) {
   if (!p.impl_tune_tile_size()) return p;
   constexpr unsigned R = Policy::rank;
   const bool left      = Policy::inner_direction == Iterate::Left;
   const int fast = left ? 0 : int(R) - 1, next = left ? 1 : int(R) - 2;
-  if (!Tools::Experimental::have_tuning_tool()) {
-    if (pattern != 1 || p.upper()[fast] - p.lower()[fast] < 128 || p.upper()[next] - p.lower()[next] < 8) return p;
-    typename Policy::tile_type t;
-    for (unsigned r = 0; r < R; ++r) t[r] = 1;
-    t[fast] = 128;
-    t[next] = 8;
-    Policy q = p;
-    q.impl_change_tile_size(t);
-    return q;
-  }
+  Policy q = p;
+  q.impl_change_tile_size(md_heuristic_tile(p, pattern));
+  // The tool chooses only whole shapes: tiles the caller fixed in part stay.
+  if (!Tools::Experimental::have_tuning_tool() || p.impl_unset_tile_mask() != (1u << R) - 1u) return q;
   int64_t points = 1;
   for (unsigned r = 0; r < R; ++r) points *= (p.upper()[r] - p.lower()[r]);
   int h = -1;
   for (int c = 0; c < 16; ++c) {
-    bool match = p.tile_size()[fast] == (16 << (c % 4)) && p.tile_size()[next] == (1 << (c / 4));
+    bool match = q.tile_size()[fast] == (16 << (c % 4)) && q.tile_size()[next] == (1 << (c / 4));
     for (int r = 0; r < int(R); ++r)
-      if (r != fast && r != next && p.tile_size()[r] != 1) match = false;
+      if (r != fast && r != next && q.tile_size()[r] != 1) match = false;
     if (match) h = c;
   }
   const int c = request_launch_candidate<F>(LaunchVar::MDRange, pattern, points, value_bytes, floating, h);
-  if (c < 0 || c == h) return p;
+  if (c < 0 || c == h) return q;
   typename Policy::tile_type t;
   for (unsigned r = 0; r < R; ++r) t[r] = 1;
   t[fast] = 16 << (c % 4);
   t[next] = 1 << (c / 4);
-  Policy q = p;
   q.impl_change_tile_size(t);
   return q;
 }
@@ -362,7 +462,6 @@ class ParallelReduce<CR, MDRangePolicy<Traits...>, MI355X> {
         m_result(result),
         m_on_device(exec_can_access<MI355X, Mem>::value) {}
   void execute() const {
-    static_assert(!CR::is_array, "MDRange reductions with runtime-length value types are not supported");
     MI355XInstance* inst = m_p.space().impl_internal_space_instance();
     std::lock_guard<std::mutex> lk(inst->mtx);
     MDLaunch l      = md_launch(m_p);
@@ -371,20 +470,37 @@ class ParallelReduce<CR, MDRangePolicy<Traits...>, MI355X> {
     if (l.ntiles > 0) md_set_depth(l, block);
     int64_t grid    = std::min<int64_t>(l.groups, static_cast<int64_t>(cus) * (2048 / block) * 2);
     if (grid < 1) grid = 1;
+    const unsigned cnt = m_cr.value_count();
+    const size_t bytes = sizeof(V) * cnt;
     V* dst         = m_result;
     bool copy_back = false;
     if (!m_on_device || !dst) {
-      dst       = static_cast<V*>(inst->pinned_space(sizeof(V)));
+      dst       = static_cast<V*>(inst->pinned_space(bytes));
       copy_back = (m_result != nullptr);
     }
-    V* partials       = static_cast<V*>(inst->partials_space(sizeof(V) * grid));
     CR cr             = m_cr;
     unsigned* counter = inst->control;
-    const void* k     = reinterpret_cast<const void*>(&k_md_reduce<CR, typename Policy::index_type, R, IL, OL>);
-    mx_launch_kernel(k, static_cast<unsigned>(grid), block, 0, inst->stream, cr, l, partials, counter, dst);
+    if constexpr (!CR::is_array) {
+      V* partials   = static_cast<V*>(inst->partials_space(sizeof(V) * grid));
+      const void* k = reinterpret_cast<const void*>(&k_md_reduce<CR, typename Policy::index_type, R, IL, OL>);
+      mx_launch_kernel(k, static_cast<unsigned>(grid), block, 0, inst->stream, cr, l, partials, counter, dst);
+    } else {
+      // Per-lane accumulators in LDS up to 64 KiB per workgroup, else in a
+      // global slice per workgroup placed after the partials.
+      const size_t acc_bytes = bytes * block;
+      const bool in_lds      = acc_bytes <= 64 * 1024;
+      const size_t part      = ((bytes * grid + 255) / 256) * 256;
+      char* scratch          = static_cast<char*>(inst->partials_space(part + (in_lds ? 0 : acc_bytes * grid)));
+      V* partials            = reinterpret_cast<V*>(scratch);
+      V* gacc                = in_lds ? nullptr : reinterpret_cast<V*>(scratch + part);
+      const void* k =
+          reinterpret_cast<const void*>(&k_md_reduce_array<CR, typename Policy::index_type, R, IL, OL>);
+      mx_launch_kernel(k, static_cast<unsigned>(grid), block, in_lds ? acc_bytes : 0, inst->stream, cr, l, gacc,
+                       partials, counter, dst);
+    }
     if (copy_back) {
       KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
-      std::memcpy(static_cast<void*>(m_result), dst, sizeof(V));
+      std::memcpy(static_cast<void*>(m_result), dst, bytes);
     }
   }
 

@@ -427,6 +427,34 @@ struct NestedOps<R, std::enable_if_t<is_reducer<R>::value>> {
   KOKKOS_INLINE_FUNCTION static void init(const R& r, value_type& v) { r.init(v); }
   KOKKOS_INLINE_FUNCTION static void join(const R& r, value_type& a, const value_type& b) { r.join(a, b); }
 };
+// Single-result nested reductions follow the reference's analysis of the
+// closure (core/src/HIP/Kokkos_HIP_Team.hpp:509-573): a reducer's init /
+// join / final; for a plain result, the closure's own init / join / final
+// when it defines join, else a sum.  `final` runs on the reduced value.
+template <class R, class F, class = void>
+struct NestedClosureOps {
+  using value_type = typename NestedOps<R>::value_type;
+  KOKKOS_INLINE_FUNCTION static void init(const R& r, const F&, value_type& v) { NestedOps<R>::init(r, v); }
+  KOKKOS_INLINE_FUNCTION static void join(const R& r, const F&, value_type& a, const value_type& b) {
+    NestedOps<R>::join(r, a, b);
+  }
+  KOKKOS_INLINE_FUNCTION static void final(const R& r, const F&, value_type& v) {
+    if constexpr (is_reducer<R>::value && has_final<R, void, value_type>::value) r.final(v);
+  }
+};
+template <class R, class F>
+struct NestedClosureOps<R, F, std::enable_if_t<!is_reducer<R>::value && has_join<F, void, R>::value>> {
+  using value_type = R;
+  KOKKOS_INLINE_FUNCTION static void init(const R&, const F& f, value_type& v) {
+    if constexpr (has_init<F, void, R>::value) f.init(v);
+    else v = value_type();
+  }
+  KOKKOS_INLINE_FUNCTION static void join(const R&, const F& f, value_type& a, const value_type& b) { f.join(a, b); }
+  KOKKOS_INLINE_FUNCTION static void final(const R&, const F& f, value_type& v) {
+    if constexpr (has_final<F, void, R>::value) f.final(v);
+  }
+};
+
 template <class R>
 KOKKOS_INLINE_FUNCTION void nested_store(const R& r, const typename NestedOps<R>::value_type& v) {
   if constexpr (is_reducer<R>::value) r.reference() = v;
@@ -503,20 +531,21 @@ template <class iType, class Member, class F, class R>
 KOKKOS_INLINE_FUNCTION void parallel_reduce(const Impl::TeamThreadRangeBoundariesStruct<iType, Member>& r,
                                             const F& f, R&& result) {
   using RR  = Impl::remove_cvref_t<R>;
-  using Ops = Impl::NestedOps<RR>;
+  using Ops = Impl::NestedClosureOps<RR, F>;
   using T   = typename Ops::value_type;
   const RR& red = result;
   T v;
-  Ops::init(red, v);
+  Ops::init(red, f, v);
   const int ts = r.member.team_size();
   for (iType i = r.start + r.member.team_rank(); i < r.end; i += ts) f(i, v);
 #if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
     T id;
-    Ops::init(red, id);
-    r.member.reduce_threads(v, id, [&red](T& a, const T& b) { Ops::join(red, a, b); });
+    Ops::init(red, f, id);
+    r.member.reduce_threads(v, id, [&red, &f](T& a, const T& b) { Ops::join(red, f, a, b); });
   }
 #endif
+  Ops::final(red, f, v);
   if constexpr (is_reducer<RR>::value) Impl::nested_store(red, v);
   else result = v;
 }
@@ -524,18 +553,19 @@ template <class iType, class Member, class F, class R>
 KOKKOS_INLINE_FUNCTION void parallel_reduce(const Impl::ThreadVectorRangeBoundariesStruct<iType, Member>& r,
                                             const F& f, R&& result) {
   using RR  = Impl::remove_cvref_t<R>;
-  using Ops = Impl::NestedOps<RR>;
+  using Ops = Impl::NestedClosureOps<RR, F>;
   using T   = typename Ops::value_type;
   const RR& red = result;
   T v;
-  Ops::init(red, v);
+  Ops::init(red, f, v);
   const int vl = r.member.impl_vector_length();
   for (iType i = r.start + r.member.impl_vector_lane(); i < r.end; i += vl) f(i, v);
 #if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
-    if (vl > 1) r.member.reduce_vector(v, [&red](T& a, const T& b) { Ops::join(red, a, b); });
+    if (vl > 1) r.member.reduce_vector(v, [&red, &f](T& a, const T& b) { Ops::join(red, f, a, b); });
   }
 #endif
+  Ops::final(red, f, v);
   if constexpr (is_reducer<RR>::value) Impl::nested_store(red, v);
   else result = v;
 }
@@ -543,21 +573,22 @@ template <class iType, class Member, class F, class R>
 KOKKOS_INLINE_FUNCTION void parallel_reduce(const Impl::TeamVectorRangeBoundariesStruct<iType, Member>& r,
                                             const F& f, R&& result) {
   using RR  = Impl::remove_cvref_t<R>;
-  using Ops = Impl::NestedOps<RR>;
+  using Ops = Impl::NestedClosureOps<RR, F>;
   using T   = typename Ops::value_type;
   const RR& red = result;
   T v;
-  Ops::init(red, v);
+  Ops::init(red, f, v);
   const int n  = r.member.team_size() * r.member.impl_vector_length();
   const int id = r.member.team_rank() * r.member.impl_vector_length() + r.member.impl_vector_lane();
   for (iType i = r.start + id; i < r.end; i += n) f(i, v);
 #if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (std::is_same<Member, Impl::TeamMember<MI355X>>::value) {
     T idv;
-    Ops::init(red, idv);
-    r.member.reduce_lanes(v, idv, [&red](T& a, const T& b) { Ops::join(red, a, b); });
+    Ops::init(red, f, idv);
+    r.member.reduce_lanes(v, idv, [&red, &f](T& a, const T& b) { Ops::join(red, f, a, b); });
   }
 #endif
+  Ops::final(red, f, v);
   if constexpr (is_reducer<RR>::value) Impl::nested_store(red, v);
   else result = v;
 }
@@ -895,6 +926,63 @@ __global__ __launch_bounds__(1024) void k_team_reduce(const CR cr, const TeamLau
   }
 }
 
+// Runtime-length results: every lane keeps value_count accumulators in
+// `acc` (after the team's LDS when they fit, else a global slice per
+// workgroup); lanes other than vector lane 0 are reset before the ordered
+// pairwise combine, since the vector lanes of a team thread share one value.
+template <class CR>
+__global__ __launch_bounds__(1024) void k_team_reduce_array(const CR cr, const TeamLaunch p, const int acc_offset,
+                                                            typename CR::value_type* gacc,
+                                                            typename CR::value_type* partials, unsigned* counter,
+                                                            typename CR::value_type* result) {
+  using V   = typename CR::value_type;
+  using Tag = typename CR::work_tag;
+  extern __shared__ __align__(16) char kkmx_team_lds[];
+  __shared__ int s_last;
+  const unsigned n  = cr.value_count();
+  const bool active = static_cast<int>(threadIdx.x) < p.team_size * p.vector_length;
+  char* l1          = p.l1_pool ? p.l1_pool + static_cast<size_t>(blockIdx.x) * p.l1_bytes : nullptr;
+  V* base = gacc ? gacc + static_cast<size_t>(blockIdx.x) * blockDim.x * n : reinterpret_cast<V*>(kkmx_team_lds + acc_offset);
+  V* mine = base + static_cast<size_t>(threadIdx.x) * n;
+  cr.init_array(mine);
+  for (int lr = blockIdx.x; lr < p.league_size; lr += gridDim.x) {
+    TeamMember<MI355X> m(kkmx_team_lds, kkmx_team_lds + p.l0_offset, p.l0_bytes, l1, p.l1_bytes, lr,
+                         p.league_size, p.team_size, p.vector_length);
+    if (active) call_functor<Tag>(cr.functor(), m, mine);
+    __syncthreads();
+  }
+  const bool counts = active && (static_cast<int>(threadIdx.x) % p.vector_length) == 0;
+  if (!counts) cr.init_array(mine);
+  __syncthreads();
+  for (unsigned s = 1; s < blockDim.x; s <<= 1) {
+    if ((threadIdx.x % (2 * s)) == 0 && threadIdx.x + s < blockDim.x)
+      cr.join_array(mine, base + static_cast<size_t>(threadIdx.x + s) * n);
+    __syncthreads();
+  }
+  if (gridDim.x == 1) {
+    if (threadIdx.x == 0) {
+      cr.final_array(base);
+      for (unsigned k = 0; k < n; ++k) result[k] = base[k];
+    }
+    return;
+  }
+  if (threadIdx.x == 0) {
+    for (unsigned k = 0; k < n; ++k) partials[static_cast<size_t>(blockIdx.x) * n + k] = base[k];
+    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    s_last              = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (threadIdx.x == 0) {
+    cr.init_array(base);
+    for (unsigned b = 0; b < gridDim.x; ++b) cr.join_array(base, partials + static_cast<size_t>(b) * n);
+    cr.final_array(base);
+    for (unsigned k = 0; k < n; ++k) result[k] = base[k];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // Team shapes a tuning tool chooses from (team_size, vector_length), all
 // within 1024 lanes.  The tool's answer only replaces the parts the policy
 // leaves to Kokkos::AUTO (reference TeamSizeTuner,
@@ -990,7 +1078,6 @@ class ParallelReduce<CR, TeamPolicy<Traits...>, MI355X> {
   ParallelReduce(const CR& cr, const Policy& p, V* result, Mem)
       : m_cr(cr), m_p(p), m_result(result), m_on_device(exec_can_access<MI355X, Mem>::value) {}
   void execute() const {
-    static_assert(!CR::is_array, "TeamPolicy reductions with runtime-length value types are not supported");
     MI355XInstance* inst = m_p.space().impl_internal_space_instance();
     std::lock_guard<std::mutex> lk(inst->mtx);
     int block = 0, grid = 0;
@@ -998,20 +1085,38 @@ class ParallelReduce<CR, TeamPolicy<Traits...>, MI355X> {
     TeamLaunch tl = team_launch_shape(m_p, m_cr.functor(), inst, block, grid, lds, 1, sizeof(V),
                                       std::is_floating_point<V>::value);
     if (grid == 0) grid = 1;
+    const unsigned cnt = m_cr.value_count();
+    const size_t bytes = sizeof(V) * cnt;
     V* dst         = m_result;
     bool copy_back = false;
     if (!m_on_device || !dst) {
-      dst       = static_cast<V*>(inst->pinned_space(sizeof(V)));
+      dst       = static_cast<V*>(inst->pinned_space(bytes));
       copy_back = (m_result != nullptr);
     }
-    V* partials = static_cast<V*>(inst->partials_space(sizeof(V) * grid));
-    CR cr       = m_cr;
+    CR cr             = m_cr;
     unsigned* counter = inst->control;
-    const void* k = reinterpret_cast<const void*>(&k_team_reduce<CR>);
-    mx_launch_kernel(k, static_cast<unsigned>(grid), block, lds, inst->stream, cr, tl, partials, counter, dst);
+    if constexpr (!CR::is_array) {
+      V* partials   = static_cast<V*>(inst->partials_space(sizeof(V) * grid));
+      const void* k = reinterpret_cast<const void*>(&k_team_reduce<CR>);
+      mx_launch_kernel(k, static_cast<unsigned>(grid), block, lds, inst->stream, cr, tl, partials, counter, dst);
+    } else {
+      // Per-lane accumulators after the team's LDS while the workgroup stays
+      // within 64 KiB, else in a global slice per workgroup.
+      const size_t acc_bytes = bytes * block;
+      int acc_offset         = static_cast<int>(((lds + 15) / 16) * 16);
+      const bool in_lds      = acc_offset + acc_bytes <= 64 * 1024;
+      const size_t part      = ((bytes * grid + 255) / 256) * 256;
+      char* scratch          = static_cast<char*>(inst->partials_space(part + (in_lds ? 0 : acc_bytes * grid)));
+      V* partials            = reinterpret_cast<V*>(scratch);
+      V* gacc                = in_lds ? nullptr : reinterpret_cast<V*>(scratch + part);
+      const size_t lds_total = in_lds ? acc_offset + acc_bytes : lds;
+      const void* k          = reinterpret_cast<const void*>(&k_team_reduce_array<CR>);
+      mx_launch_kernel(k, static_cast<unsigned>(grid), block, lds_total, inst->stream, cr, tl, acc_offset, gacc,
+                       partials, counter, dst);
+    }
     if (copy_back) {
       KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
-      std::memcpy(static_cast<void*>(m_result), dst, sizeof(V));
+      std::memcpy(static_cast<void*>(m_result), dst, bytes);
     }
   }
 
@@ -1054,14 +1159,25 @@ class ParallelReduce<CR, TeamPolicy<Traits...>, Serial> {
     const size_t l0 = m_p.team_scratch_size(0) + m_p.thread_scratch_size(0);
     const size_t l1 = m_p.team_scratch_size(1) + m_p.thread_scratch_size(1);
     std::vector<char> b0(l0 + 16), b1(l1 + 16);
-    V v;
-    m_cr.init(v);
-    for (int lr = 0; lr < m_p.league_size(); ++lr) {
-      TeamMember<Serial> m(b0.data(), l0, b1.data(), l1, lr, m_p.league_size());
-      call_functor<typename CR::work_tag>(m_cr.functor(), m, v);
+    if constexpr (CR::is_array) {
+      std::vector<V> tmp(m_cr.value_count());
+      V* v = m_result ? m_result : tmp.data();
+      m_cr.init_array(v);
+      for (int lr = 0; lr < m_p.league_size(); ++lr) {
+        TeamMember<Serial> m(b0.data(), l0, b1.data(), l1, lr, m_p.league_size());
+        call_functor<typename CR::work_tag>(m_cr.functor(), m, v);
+      }
+      m_cr.final_array(v);
+    } else {
+      V v;
+      m_cr.init(v);
+      for (int lr = 0; lr < m_p.league_size(); ++lr) {
+        TeamMember<Serial> m(b0.data(), l0, b1.data(), l1, lr, m_p.league_size());
+        call_functor<typename CR::work_tag>(m_cr.functor(), m, v);
+      }
+      m_cr.final(v);
+      if (m_result) *m_result = v;
     }
-    m_cr.final(v);
-    if (m_result) *m_result = v;
   }
 
  private:

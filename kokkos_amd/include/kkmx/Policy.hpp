@@ -18,6 +18,7 @@
 #include <kkmx/Spaces.hpp>
 
 #include <array>
+#include <tuple>
 #include <initializer_list>
 
 namespace Kokkos {
@@ -85,24 +86,113 @@ Policy prefer(const Policy& p, const MaximizeOccupancy&) { return p; }
 enum class Iterate { Default, Left, Right };
 template <unsigned N, Iterate OuterDir = Iterate::Default, Iterate InnerDir = Iterate::Default>
 struct Rank {
-  static_assert(N >= 2 && N <= 6, "Kokkos::Rank supports ranks 2 to 6");
+  static_assert(N >= 2 && N <= 8, "Kokkos::Rank supports ranks 2 to 8 (MDRangePolicy: 2 to 6)");
   static constexpr unsigned rank           = N;
   static constexpr Iterate outer_direction = OuterDir;
   static constexpr Iterate inner_direction = InnerDir;
   using iteration_pattern                  = Rank<N, OuterDir, InnerDir>;
 };
 
+// Fixed-size array usable in kernels (core/src/Kokkos_Array.hpp: aggregate
+// initialisation, size / empty / max_size / data, element-wise equality, free
+// begin / end / get and to_array).
 template <class T, size_t N>
 struct Array {
+  using value_type      = T;
+  using size_type       = size_t;
+  using difference_type = ptrdiff_t;
+  using reference       = T&;
+  using const_reference = const T&;
+  using pointer         = T*;
+  using const_pointer   = const T*;
   T m_internal_implementation_private_member_data[N ? N : 1];
   KOKKOS_INLINE_FUNCTION constexpr T& operator[](size_t i) { return m_internal_implementation_private_member_data[i]; }
   KOKKOS_INLINE_FUNCTION constexpr const T& operator[](size_t i) const { return m_internal_implementation_private_member_data[i]; }
   KOKKOS_INLINE_FUNCTION static constexpr size_t size() { return N; }
-  KOKKOS_INLINE_FUNCTION constexpr T* data() { return m_internal_implementation_private_member_data; }
-  KOKKOS_INLINE_FUNCTION constexpr const T* data() const { return m_internal_implementation_private_member_data; }
+  KOKKOS_INLINE_FUNCTION static constexpr bool empty() { return N == 0; }
+  KOKKOS_INLINE_FUNCTION constexpr size_t max_size() const { return N; }
+  KOKKOS_INLINE_FUNCTION constexpr T* data() { return N ? m_internal_implementation_private_member_data : nullptr; }
+  KOKKOS_INLINE_FUNCTION constexpr const T* data() const {
+    return N ? m_internal_implementation_private_member_data : nullptr;
+  }
+  friend KOKKOS_INLINE_FUNCTION constexpr bool operator==(const Array& a, const Array& b) {
+    for (size_t i = 0; i < N; ++i)
+      if (!(a[i] == b[i])) return false;
+    return true;
+  }
+  friend KOKKOS_INLINE_FUNCTION constexpr bool operator!=(const Array& a, const Array& b) { return !(a == b); }
 };
+// The empty array holds nothing, so a const one needs no initialiser.
+template <class T>
+struct Array<T, 0> {
+  using value_type      = T;
+  using size_type       = size_t;
+  using difference_type = ptrdiff_t;
+  using reference       = T&;
+  using const_reference = const T&;
+  using pointer         = T*;
+  using const_pointer   = const T*;
+  template <class I>
+  KOKKOS_INLINE_FUNCTION T& operator[](const I&) {
+    Kokkos::abort("Kokkos::Array<T, 0>: no elements");
+    return *reinterpret_cast<T*>(this);
+  }
+  template <class I>
+  KOKKOS_INLINE_FUNCTION const T& operator[](const I&) const {
+    Kokkos::abort("Kokkos::Array<T, 0>: no elements");
+    return *reinterpret_cast<const T*>(this);
+  }
+  KOKKOS_INLINE_FUNCTION static constexpr size_t size() { return 0; }
+  KOKKOS_INLINE_FUNCTION static constexpr bool empty() { return true; }
+  KOKKOS_INLINE_FUNCTION constexpr size_t max_size() const { return 0; }
+  KOKKOS_INLINE_FUNCTION constexpr T* data() { return nullptr; }
+  KOKKOS_INLINE_FUNCTION constexpr const T* data() const { return nullptr; }
+  friend KOKKOS_INLINE_FUNCTION constexpr bool operator==(const Array&, const Array&) { return true; }
+  friend KOKKOS_INLINE_FUNCTION constexpr bool operator!=(const Array&, const Array&) { return false; }
+};
+template <class T, size_t N>
+KOKKOS_INLINE_FUNCTION constexpr T* begin(Array<T, N>& a) noexcept { return a.data(); }
+template <class T, size_t N>
+KOKKOS_INLINE_FUNCTION constexpr const T* begin(const Array<T, N>& a) noexcept { return a.data(); }
+template <class T, size_t N>
+KOKKOS_INLINE_FUNCTION constexpr T* end(Array<T, N>& a) noexcept { return a.data() + N; }
+template <class T, size_t N>
+KOKKOS_INLINE_FUNCTION constexpr const T* end(const Array<T, N>& a) noexcept { return a.data() + N; }
+template <size_t I, class T, size_t N>
+KOKKOS_INLINE_FUNCTION constexpr T& get(Array<T, N>& a) noexcept {
+  static_assert(I < N, "Kokkos::get: index out of range");
+  return a[I];
+}
+template <size_t I, class T, size_t N>
+KOKKOS_INLINE_FUNCTION constexpr const T& get(const Array<T, N>& a) noexcept {
+  static_assert(I < N, "Kokkos::get: index out of range");
+  return a[I];
+}
+template <size_t I, class T, size_t N>
+KOKKOS_INLINE_FUNCTION constexpr T&& get(Array<T, N>&& a) noexcept {
+  static_assert(I < N, "Kokkos::get: index out of range");
+  return static_cast<T&&>(a[I]);
+}
+template <class T, size_t N>
+KOKKOS_INLINE_FUNCTION constexpr Array<std::remove_cv_t<T>, N> to_array(T (&a)[N]) {
+  Array<std::remove_cv_t<T>, N> r{};
+  for (size_t i = 0; i < N; ++i) r[i] = a[i];
+  return r;
+}
+template <class T, size_t N>
+KOKKOS_INLINE_FUNCTION constexpr Array<std::remove_cv_t<T>, N> to_array(T (&&a)[N]) {
+  Array<std::remove_cv_t<T>, N> r{};
+  for (size_t i = 0; i < N; ++i) r[i] = static_cast<T&&>(a[i]);
+  return r;
+}
 
 namespace Impl {
+// Trait a graph adds to the policy type of each kernel node it records
+// (core/src/impl/Kokkos_GraphImpl_fwd.hpp:65-67); it changes nothing in how
+// the kernel runs.
+struct IsGraphKernelTag {};
+struct KernelInGraphProperty {};
+
 template <class T>
 struct is_schedule : std::false_type {};
 template <class T>
@@ -145,7 +235,8 @@ struct PolicyTraits<H, T...> {
   static constexpr bool h_lb    = is_launch_bounds<H>::value;
   static constexpr bool h_rank  = is_rank<H>::value;
   static constexpr bool h_wip   = is_work_item_property<H>::value;
-  static constexpr bool h_tag   = !(h_exec || h_sched || h_index || h_lb || h_rank || h_wip);
+  static constexpr bool h_graph = std::is_same<H, IsGraphKernelTag>::value;
+  static constexpr bool h_tag   = !(h_exec || h_sched || h_index || h_lb || h_rank || h_wip || h_graph);
   template <class X, bool B = std::is_integral<X>::value>
   struct idx {
     using type = IndexType<X>;
@@ -265,22 +356,63 @@ class RangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
 };
 
 // ------------------------------------------------------------------ MDRangePolicy
-namespace Impl {
-// Tile defaults for an execution space (reference HIP:
-// core/src/HIP/Kokkos_HIP_MDRangePolicy.hpp:24-48).  On MI355X the fastest
-// dimension maps to the 64 lanes of a wavefront.
-template <class Space>
-struct MDRangeDefaults {
-  static constexpr Iterate outer = Iterate::Right;
-  static constexpr Iterate inner = Iterate::Right;
-  static constexpr int max_total_tile = 1 << 30;
+// Iteration directions an execution space uses when Rank<N> leaves them to it
+// (core/src/KokkosExp_MDRangePolicy.hpp:49-60).  On MI355X both are Left:
+// the first index is fastest, as in LayoutLeft Views, and it runs along the
+// 64 lanes of a wavefront.
+template <class ExecSpace>
+struct default_outer_direction {
+  using type                     = Iterate;
+  static constexpr Iterate value = Iterate::Right;
+};
+template <class ExecSpace>
+struct default_inner_direction {
+  using type                     = Iterate;
+  static constexpr Iterate value = Iterate::Right;
 };
 template <>
-struct MDRangeDefaults<MI355X> {
-  static constexpr Iterate outer = Iterate::Left;
-  static constexpr Iterate inner = Iterate::Left;
-  static constexpr int max_total_tile = 1024;
+struct default_outer_direction<MI355X> {
+  using type                     = Iterate;
+  static constexpr Iterate value = Iterate::Left;
 };
+template <>
+struct default_inner_direction<MI355X> {
+  using type                     = Iterate;
+  static constexpr Iterate value = Iterate::Left;
+};
+
+namespace Impl {
+// Tile limits and defaults of an execution space
+// (core/src/KokkosExp_MDRangePolicy.hpp:137-156): the largest tile product a
+// workgroup runs, the extent given to the fastest dimension and to each other
+// one when the caller leaves the tile to the implementation.  Host spaces
+// take the whole fastest extent.
+struct TileSizeProperties {
+  int max_threads;
+  int default_largest_tile_size;
+  int default_tile_size;
+  int max_total_tile_size;
+};
+template <class ExecutionSpace>
+TileSizeProperties get_tile_size_properties(const ExecutionSpace&) {
+  TileSizeProperties t;
+  t.max_threads               = std::numeric_limits<int>::max();
+  t.default_largest_tile_size = 0;
+  t.default_tile_size         = 2;
+  t.max_total_tile_size       = std::numeric_limits<int>::max();
+  return t;
+}
+// MI355X: one lane per tile point, at most 1024 lanes per workgroup; a full
+// wavefront along the fastest dimension.
+template <>
+inline TileSizeProperties get_tile_size_properties<MI355X>(const MI355X&) {
+  TileSizeProperties t;
+  t.max_threads               = 1024;
+  t.default_largest_tile_size = 64;
+  t.default_tile_size         = 2;
+  t.max_total_tile_size       = 1024;
+  return t;
+}
 }  // namespace Impl
 
 template <class... Properties>
@@ -293,18 +425,19 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
   using iteration_pattern = typename traits::iteration_pattern;
   static_assert(!std::is_void<iteration_pattern>::value, "MDRangePolicy requires a Kokkos::Rank<N>");
   static constexpr unsigned rank = iteration_pattern::rank;
+  static_assert(rank <= 6, "Kokkos::MDRangePolicy supports ranks 2 to 6");
   using index_type        = typename traits::index_type;
   using array_index_type  = std::int64_t;  // as the reference, whatever index_type is
   using point_type        = Kokkos::Array<array_index_type, rank>;
   using tile_type         = Kokkos::Array<array_index_type, rank>;
   using execution_policy  = MDRangePolicy<Properties...>;
   using impl_policy_kind  = MDRangePolicy<>;
-  static constexpr Iterate outer_direction =
-      iteration_pattern::outer_direction == Iterate::Default ? Impl::MDRangeDefaults<execution_space>::outer
-                                                             : iteration_pattern::outer_direction;
-  static constexpr Iterate inner_direction =
-      iteration_pattern::inner_direction == Iterate::Default ? Impl::MDRangeDefaults<execution_space>::inner
-                                                             : iteration_pattern::inner_direction;
+  static constexpr Iterate outer_direction = iteration_pattern::outer_direction == Iterate::Default
+                                                 ? default_outer_direction<execution_space>::value
+                                                 : iteration_pattern::outer_direction;
+  static constexpr Iterate inner_direction = iteration_pattern::inner_direction == Iterate::Default
+                                                 ? default_inner_direction<execution_space>::value
+                                                 : iteration_pattern::inner_direction;
 
   MDRangePolicy() = default;
 
@@ -364,6 +497,8 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
   KOKKOS_INLINE_FUNCTION int64_t m_num_tiles_total() const { return m_num_tiles; }
   int64_t num_tiles() const { return m_num_tiles; }
   bool impl_tune_tile_size() const { return m_tune; }
+  // Bit r set: tile[r] was left to the implementation.
+  unsigned impl_unset_tile_mask() const { return m_tile_unset; }
   // Replace the tile shape (used by the tuning hooks when tiles were left to
   // the implementation).
   void impl_change_tile_size(const tile_type& t) {
@@ -375,6 +510,28 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
     for (unsigned r = 0; r < rank; ++r) p *= m_tile[r];
     return p;
   }
+  // The tile shape the space recommends for these bounds, from the fastest
+  // dimension outwards: the fastest takes default_largest_tile_size (the
+  // whole extent when that is 0), every other one default_tile_size while the
+  // product stays below max_total_tile_size, else 1.
+  tile_type tile_size_recommended() const {
+    const Impl::TileSizeProperties pr = Impl::get_tile_size_properties(m_space);
+    tile_type t{};
+    const bool left = inner_direction == Iterate::Left;
+    int64_t prod    = 1;
+    for (unsigned k = 0; k < rank; ++k) {
+      const unsigned r = left ? k : rank - 1 - k;
+      if (k == 0) {
+        const int64_t ext = m_upper[r] - m_lower[r];
+        t[r] = pr.default_largest_tile_size == 0 ? (ext > 1 ? ext : 1) : pr.default_largest_tile_size;
+      } else {
+        t[r] = prod * pr.default_tile_size < pr.max_total_tile_size ? pr.default_tile_size : 1;
+      }
+      prod *= t[r];
+    }
+    return t;
+  }
+  int max_total_tile_size() const { return Impl::get_tile_size_properties(m_space).max_total_tile_size; }
 
   point_type m_lower{};
   point_type m_upper{};
@@ -383,6 +540,7 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
   int64_t m_num_tiles = 0;
   int64_t m_prod_tile_dims = 0;
   bool m_tune = false;
+  unsigned m_tile_unset = 0;  // bit r: the caller left tile[r] to the implementation
 
  private:
   execution_space m_space;
@@ -395,32 +553,19 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
                           std::to_string(m_upper[r]) + ") in dimension " + std::to_string(r) + ".\n";
         Kokkos::abort(msg.c_str());
       }
-    // Default tiles: when unspecified, the fastest dimension takes a full
-    // wavefront (or the extent if smaller), others 1..4 up to 256 points; the
-    // tuner may replace them (reference KokkosExp_MDRangePolicy.hpp:372-426).
-    bool any_unset = false;
-    for (unsigned r = 0; r < rank; ++r) any_unset |= (m_tile[r] <= 0);
-    if (any_unset) {
-      m_tune = true;
-      const int fast = (inner_direction == Iterate::Left) ? 0 : int(rank) - 1;
-      const int step = (inner_direction == Iterate::Left) ? 1 : -1;
-      int64_t budget = std::is_same<execution_space, MI355X>::value ? 256 : (1 << 20);
-      for (int k = 0, r = fast; k < int(rank); ++k, r += step) {
-        if (m_tile[r] > 0) { budget /= m_tile[r]; continue; }
-        int64_t ext = m_upper[r] - m_lower[r];
-        int64_t t;
-        if (!std::is_same<execution_space, MI355X>::value) t = ext > 0 ? ext : 1;
-        else if (k == 0) t = 64;
-        else t = (k == 1) ? 4 : 1;
-        if (ext > 0 && t > ext && k == 0) {
-          t = 1;
-          while (t < ext) t *= 2;
-        }
-        if (t > budget) t = budget > 0 ? budget : 1;
-        if (t < 1) t = 1;
-        m_tile[r] = static_cast<array_index_type>(t);
-        budget /= t;
+    // Tiles left to the implementation (any entry <= 0) take the
+    // recommended shape of the space's tile properties; the launch may
+    // still pick another shape for them (the tuning hooks, or the MI355X
+    // heuristic per pattern), which m_tune records.
+    for (unsigned r = 0; r < rank; ++r)
+      if (m_tile[r] <= 0) {
+        m_tune = true;
+        m_tile_unset |= 1u << r;
       }
+    if (m_tune) {
+      const tile_type rec = tile_size_recommended();
+      for (unsigned r = 0; r < rank; ++r)
+        if (m_tile[r] <= 0) m_tile[r] = rec[r];
     }
     recompute();
   }
@@ -433,12 +578,12 @@ class MDRangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
       m_num_tiles   *= m_tile_end[r];
       m_prod_tile_dims *= m_tile[r];
     }
-    if (m_prod_tile_dims > Impl::MDRangeDefaults<execution_space>::max_total_tile) {
+    const int max_total = Impl::get_tile_size_properties(m_space).max_total_tile_size;
+    if (m_prod_tile_dims > max_total) {
       Kokkos::Impl::throw_runtime_exception(
           std::string("Kokkos::MDRange Error: Exceeded rank bounds with ") + execution_space::name() +
           " execution space: tile dimensions product " + std::to_string(m_prod_tile_dims) +
-          " is larger than the maximum of " +
-          std::to_string(Impl::MDRangeDefaults<execution_space>::max_total_tile));
+          " is larger than the maximum of " + std::to_string(max_total));
     }
   }
 };
@@ -605,5 +750,14 @@ class TeamPolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
 };
 
 }  // namespace Kokkos
+
+// Structured bindings of Kokkos::Array.
+template <class T, size_t N>
+struct std::tuple_size<Kokkos::Array<T, N>> : std::integral_constant<size_t, N> {};
+template <size_t I, class T, size_t N>
+struct std::tuple_element<I, Kokkos::Array<T, N>> {
+  using type = T;
+};
+
 
 #endif

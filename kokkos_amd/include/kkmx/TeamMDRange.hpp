@@ -27,7 +27,7 @@ namespace Impl {
 
 template <class Space>
 constexpr Iterate team_md_direction(Iterate d) {
-  return d == Iterate::Default ? MDRangeDefaults<Space>::outer : d;
+  return d == Iterate::Default ? default_outer_direction<Space>::value : d;
 }
 
 template <class RankT, class TeamHandle, int Level>

@@ -67,12 +67,21 @@ using endDeepCopyFunction           = Kokkos_Profiling_endDeepCopyFunction;
 using beginFenceFunction            = Kokkos_Profiling_beginFenceFunction;
 using endFenceFunction              = Kokkos_Profiling_endFenceFunction;
 using declareMetadataFunction       = Kokkos_Profiling_declareMetadataFunction;
+using dualViewSyncFunction          = Kokkos_Profiling_dualViewSyncFunction;
+using dualViewModifyFunction        = Kokkos_Profiling_dualViewModifyFunction;
 using requestToolSettingsFunction   = Kokkos_Tools_requestToolSettingsFunction;
 using provideToolProgrammingInterfaceFunction = Kokkos_Tools_provideToolProgrammingInterfaceFunction;
 using ToolSettings             = Kokkos_Tools_ToolSettings;
 using ToolProgrammingInterface = Kokkos_Tools_ToolProgrammingInterface;
 
 namespace Experimental {
+// The tool-settings and programming-interface types live in
+// Tools::Experimental in the reference (impl/Kokkos_Profiling_Interface.hpp:132-154).
+using ToolSettings                            = Kokkos_Tools_ToolSettings;
+using ToolProgrammingInterface                = Kokkos_Tools_ToolProgrammingInterface;
+using toolInvokedFenceFunction                = Kokkos_Tools_toolInvokedFenceFunction;
+using provideToolProgrammingInterfaceFunction = Kokkos_Tools_provideToolProgrammingInterfaceFunction;
+using requestToolSettingsFunction             = Kokkos_Tools_requestToolSettingsFunction;
 using VariableInfo          = Kokkos_Tools_VariableInfo;
 using VariableValue         = Kokkos_Tools_VariableValue;
 using SetOrRange            = Kokkos_Tools_VariableInfo_SetOrRange;
@@ -117,6 +126,8 @@ struct EventSet {
   beginFenceFunction begin_fence                      = nullptr;
   endFenceFunction end_fence                          = nullptr;
   declareMetadataFunction declare_metadata            = nullptr;
+  dualViewSyncFunction sync_dual_view                 = nullptr;
+  dualViewModifyFunction modify_dual_view             = nullptr;
   provideToolProgrammingInterfaceFunction provide_tool_programming_interface = nullptr;
   requestToolSettingsFunction request_tool_settings   = nullptr;
   outputTypeDeclarationFunction declare_output_type   = nullptr;
@@ -356,6 +367,8 @@ KKMX_SETTER(set_end_deep_copy_callback, end_deep_copy, endDeepCopyFunction)
 KKMX_SETTER(set_begin_fence_callback, begin_fence, beginFenceFunction)
 KKMX_SETTER(set_end_fence_callback, end_fence, endFenceFunction)
 KKMX_SETTER(set_declare_metadata_callback, declare_metadata, declareMetadataFunction)
+KKMX_SETTER(set_dual_view_sync_callback, sync_dual_view, dualViewSyncFunction)
+KKMX_SETTER(set_dual_view_modify_callback, modify_dual_view, dualViewModifyFunction)
 KKMX_SETTER(set_request_tool_settings_callback, request_tool_settings, requestToolSettingsFunction)
 KKMX_SETTER(set_provide_tool_programming_interface_callback, provide_tool_programming_interface,
             provideToolProgrammingInterfaceFunction)
@@ -627,6 +640,9 @@ inline void impl_request_tool_settings_refresh() {
 }  // namespace Tools
 
 namespace Profiling {
+// impl/Kokkos_Profiling_DeviceInfo.hpp:24 and the Profiling spellings of the
+// Tools types.
+using KokkosPDeviceInfo = Kokkos_Profiling_KokkosPDeviceInfo;
 using Tools::beginParallelFor;
 using Tools::beginParallelReduce;
 using Tools::beginParallelScan;
