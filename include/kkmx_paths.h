@@ -142,7 +142,9 @@ int kkmx_spmv_csr(const int64_t* rowptr_dev, const int32_t* col_dev, const doubl
  *   atomic_add (core/src/Kokkos_Atomics_Desul_Wrapper.hpp:120).
  * kkmx_zipf_keys: Zipf(s) keys over [0, nbins) by inverse CDF.
  * kkmx_histogram: mode 0 = global atomics, 1 = bins [0, lds_bins) privatised
- *   in team scratch (LDS atomics), the rest global.
+ *   in team scratch (LDS atomics), the rest global; 2 = as 1 with 16-byte key
+ *   loads and the lowest 256 bins replicated 16 times in LDS (keys 16-byte
+ *   aligned; otherwise mode 1 runs).
  * kkmx_scatter_add_f64: bins[idx[i]] += vals[i] with FP64 atomic_add. */
 int kkmx_xorshift64_stream(uint64_t* out_dev, int64_t n, uint64_t seed);
 int kkmx_gups(int64_t* table_dev, int64_t table_size, const uint64_t* stream_dev, int64_t nupdates);
@@ -204,12 +206,6 @@ int kkmx_set_range_config(int pattern, int block, int items, int max_blocks);
 /* MDRange parallel_for coarsening: tiles per workgroup along the slowest tile
  * dimension (0 = heuristic). */
 int kkmx_set_mdrange_depth(int depth);
-/* parallel_scan kernel selection: mode 0 = heuristic, 1 = tile-claim kernel
- * (one ordinary launch, tiles claimed in launch order), 2 = static-order
- * kernel (cooperative launch, workgroup b takes tiles b, b+G, ...); block > 0
- * also sets the static kernel's lanes per workgroup, values per lane (8, 16
- * or 32) and resident workgroups per CU (0 = as many as fit). */
-int kkmx_set_scan_mode(int mode, int block, int items, int blocks_per_cu);
 
 #ifdef __cplusplus
 }

@@ -22,10 +22,6 @@
  *   mx_launch                    core/src/HIP/Kokkos_HIP_KernelLaunch.hpp:382-389,594-641 (hip_parallel_launch)
  *   mx_func_attributes           core/src/HIP/Kokkos_HIP_KernelLaunch.hpp:194-206 (cached hipFuncGetAttributes)
  *   mx_occupancy                 core/src/HIP/Kokkos_HIP_BlockSize_Deduction.hpp:107-150
- *   mx_launch_cooperative        no counterpart: the reference's HIP scan launches twice
- *                                (core/src/HIP/Kokkos_HIP_ParallelScan_Range.hpp:313-388); this
- *                                build's single-pass scan needs all workgroups resident at once
- *   mx_stream_capturing          core/src/HIP/Kokkos_HIP_Graph_Impl.hpp (graph construction)
  *   mx_graph_*                   core/src/HIP/Kokkos_HIP_Graph_Impl.hpp:82-89,118-221
  *   mx_error_string              core/src/HIP/Kokkos_HIP_Instance.cpp:456-465 (hip_internal_error_throw)
  */
@@ -113,13 +109,6 @@ int mx_func_attributes(const void* fn, int* max_threads_per_block,
 int mx_occupancy(const void* fn, int block_size, size_t dyn_lds_bytes,
                  int* blocks_per_cu);
 int mx_get_last_error(void);
-/* Launch whose gx workgroups of bx lanes are all resident at once
- * (hipLaunchCooperativeKernel); fails when the grid cannot be. */
-int mx_launch_cooperative(const void* fn, unsigned gx, unsigned bx, size_t lds_bytes,
-                          void** args, mx_stream_t stream);
-int mx_cooperative_supported(int* supported);
-/* *capturing = 1 while the stream records into a graph. */
-int mx_stream_capturing(mx_stream_t stream, int* capturing);
 
 /* Graphs: stream capture, instantiation and replay. */
 int mx_graph_capture_begin(mx_stream_t stream);

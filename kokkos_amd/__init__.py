@@ -111,7 +111,6 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_scan_excl_i64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_scan_excl_f64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_set_range_config", [C.c_int, C.c_int, C.c_int, C.c_int])
-        _sig(lib, "kkmx_set_scan_mode", [C.c_int, C.c_int, C.c_int, C.c_int])
         _sig(lib, "kkmx_stencil7", [vp, vp, i64, f64, f64, C.c_int, C.c_int, C.c_int])
         _sig(lib, "kkmx_stencil7_sweeps", [vp, vp, i64, C.c_int, f64, f64, C.c_int, C.c_int, C.c_int])
         _sig(lib, "kkmx_csr_generate", [vp, vp, vp, i64, i64, i64, i64, u64, C.POINTER(i64)])
@@ -132,10 +131,6 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_batch_run", [vp, C.c_int, C.c_int])
     elif name == "kkmx_peak":
         _sig(lib, "kkmx_peak_copy", [vp, vp, i64, C.c_int, C.c_int, vp])
-    elif name == "kkmx_selftest":
-        _sig(lib, "kkmx_selftest_count", [])
-        _sig(lib, "kkmx_selftest_name", [C.c_int], C.c_char_p)
-        _sig(lib, "kkmx_selftest_run", [C.c_int, C.c_char_p, C.c_int])
 
 
 def check(rc: int, lib: C.CDLL | None = None) -> None:
@@ -230,21 +225,6 @@ def init(device: int = 0) -> C.CDLL:
     p = load("kkmx_paths")
     check(p.kkmx_init(device), p)
     return p
-
-
-def selftests() -> list[str]:
-    """Names of the restated reference unit tests in libkkmx_selftest.so."""
-    L = load("kkmx_selftest")
-    return [L.kkmx_selftest_name(i).decode() for i in range(L.kkmx_selftest_count())]
-
-
-def run_selftest(name: str) -> tuple[int, str]:
-    """Run one restated unit test on the GPU: (status, message); 0 = pass."""
-    L = load("kkmx_selftest")
-    idx = selftests().index(name)
-    buf = C.create_string_buffer(4096)
-    rc = L.kkmx_selftest_run(idx, buf, len(buf))
-    return rc, buf.value.decode(errors="replace")
 
 
 def fence() -> None:

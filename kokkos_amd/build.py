@@ -1,9 +1,12 @@
 """Build the MI355X execution space's shared libraries in-tree with hipcc.
 
-Libraries (written next to their sources, under kokkos_amd/lib/):
+Libraries (under kokkos_amd/lib/):
   libkkmx_rt.so     C-ABI runtime (include/kkmx_runtime.h)
   libkkmx_paths.so  hot-path C-ABI entry points (include/kkmx_paths.h)
   further libraries listed in LIBS below.
+The restated reference unit tests are test code: their library is built from
+tests/cpp/selftest.hip into tests/cpp/lib/ (TEST_LIBS below), next to the
+tests that load it.
 
 Usage: python -m kokkos_amd.build [--jobs N] [names...]
 """
@@ -34,29 +37,41 @@ COMMON = [
     "-Wno-unused-command-line-argument",
 ]
 
-# name -> (sources, extra flags, link against runtime)
+# name -> (sources relative to the package, extra flags, link against runtime)
 LIBS = {
     "kkmx_rt": (["csrc/runtime.cpp"], [], False),
     "kkmx_paths": (["csrc/paths.hip"], [], True),
     "kkmx_peak": (["csrc/peak.hip"], [], True),
-    "kkmx_selftest": (["csrc/selftest.hip"], [], True),
     "kkmx_tuner": (["csrc/tuner.hip"], [], True),
     "kkmx_sweep": (["csrc/sweep.hip"], [], True),
+}
+# Test libraries: name -> (sources relative to the repository root, output dir).
+TEST_LIBDIR = ROOT / "tests" / "cpp" / "lib"
+TEST_LIBS = {
+    "kkmx_selftest": (["tests/cpp/selftest.hip"], TEST_LIBDIR),
 }
 
 
 def lib_path(name: str) -> Path:
+    if name in TEST_LIBS:
+        return TEST_LIBS[name][1] / f"lib{name}.so"
     return LIBDIR / f"lib{name}.so"
 
 
 def build_one(name: str, verbose: bool = False) -> Path:
-    srcs, extra, link_rt = LIBS[name]
-    LIBDIR.mkdir(exist_ok=True)
+    if name in TEST_LIBS:
+        rel, outdir = TEST_LIBS[name]
+        srcs, extra, link_rt = [str(ROOT / s) for s in rel], [], True
+        rpath = "-Wl,-rpath," + os.path.join("$ORIGIN", os.path.relpath(LIBDIR, outdir))
+    else:
+        rel, extra, link_rt = LIBS[name]
+        srcs, outdir, rpath = [str(PKG / s) for s in rel], LIBDIR, "-Wl,-rpath,$ORIGIN"
+    outdir.mkdir(parents=True, exist_ok=True)
     out = lib_path(name)
     tmp = out.with_suffix(".so.tmp")
-    cmd = [HIPCC, *COMMON, *extra, *[str(PKG / s) for s in srcs], "-o", str(tmp)]
+    cmd = [HIPCC, *COMMON, *extra, *srcs, "-o", str(tmp)]
     if link_rt:
-        cmd += ["-L", str(LIBDIR), "-lkkmx_rt", "-Wl,-rpath,$ORIGIN"]
+        cmd += ["-L", str(LIBDIR), "-lkkmx_rt", rpath]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -68,7 +83,7 @@ def build_one(name: str, verbose: bool = False) -> Path:
 
 
 def build(names=None, jobs: int = 4, verbose: bool = False) -> list[Path]:
-    names = list(names or LIBS.keys())
+    names = list(names or [*LIBS.keys(), *TEST_LIBS.keys()])
     outs = []
     # The runtime first: everything else links against it.
     if "kkmx_rt" in names:

@@ -166,9 +166,14 @@ struct InclusiveScan {
 // ---------------------------------------------------------------- stencil
 // 7-point Laplacian-type update over the interior of an (n+2)^3 LayoutLeft
 // grid (first index fastest), ping-ponged between two Views.
-using Grid3 = Kokkos::View<double***, Kokkos::LayoutLeft, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+// The input grid is read-only during a sweep, which the functor states with
+// a const RandomAccess View (its loads may then move across the stores to v).
+using Grid3   = Kokkos::View<double***, Kokkos::LayoutLeft, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+using Grid3In = Kokkos::View<const double***, Kokkos::LayoutLeft, Kokkos::MI355XSpace,
+                             Kokkos::MemoryTraits<Kokkos::Unmanaged | Kokkos::RandomAccess>>;
 struct Stencil7 {
-  Grid3 u, v;
+  Grid3In u;
+  Grid3 v;
   double c0, c1;
   KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, const int64_t j, const int64_t k) const {
     const double s = u(i - 1, j, k) + u(i + 1, j, k) + u(i, j - 1, k) + u(i, j + 1, k) + u(i, j, k - 1) + u(i, j, k + 1);
@@ -346,6 +351,73 @@ struct HistPrivatised {
     m.team_barrier();
     Kokkos::parallel_for(Kokkos::TeamVectorRange(m, lds_bins), [&](const int b) {
       const int32_t c = h(b);
+      if (c) Kokkos::atomic_add(&bins(b), c);
+    });
+  }
+};
+// Mode 2 of the histogram: the same team-private LDS bins, fed by 16-byte
+// loads (four keys per load, kWideLoads loads in flight per lane before the
+// first LDS atomic), with the lowest `hot` bins replicated kHotCopies times
+// so that lanes counting the same frequent key hit different LDS words (lane
+// l uses copy l % kHotCopies; copies are hot+1 words apart, which also spreads
+// one bin's copies over the banks).  The team's share is a contiguous run of
+// 4-key groups; team 0 also counts the n % 4 trailing keys.  The keys must be
+// 16-byte aligned (checked by the entry point).
+struct HistWide {
+  static constexpr int kHotCopies = 16;
+  static constexpr int kWideLoads = 4;
+  DevView<const int32_t> keys;
+  DevView<int32_t> bins;
+  int64_t n;
+  int32_t lds_bins;
+  int32_t hot;
+  using member_type = Kokkos::TeamPolicy<Kokkos::MI355X>::member_type;
+  using Lds = Kokkos::View<int32_t*, Kokkos::MI355X::scratch_memory_space, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+  static size_t scratch_bytes(int32_t lds_bins, int32_t hot) {
+    return (size_t(lds_bins) + size_t(kHotCopies) * (hot + 1)) * 4;
+  }
+  KOKKOS_INLINE_FUNCTION void count(const Lds& h, int32_t copy_base, int32_t k) const {
+    if (k < hot) Kokkos::atomic_add(&h(lds_bins + copy_base + k), 1);
+    else if (k < lds_bins) Kokkos::atomic_add(&h(k), 1);
+    else Kokkos::atomic_add(&bins(k), 1);
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(const member_type& m) const {
+    const int32_t stride_copies = hot + 1;
+    const int32_t words         = lds_bins + kHotCopies * stride_copies;
+    Lds h(m.team_scratch(0), words);
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, words), [&](const int b) { h(b) = 0; });
+    m.team_barrier();
+    const int lane          = m.team_rank();
+    const int T             = m.team_size();
+    const int32_t copy_base = (lane % kHotCopies) * stride_copies;
+    const int64_t groups    = n / 4;
+    const int64_t nteams    = m.league_size();
+    const int64_t chunk     = (groups + nteams - 1) / nteams;
+    const int64_t g0        = chunk * m.league_rank();
+    const int64_t g1        = g0 + chunk < groups ? g0 + chunk : groups;
+    const int4* kv          = reinterpret_cast<const int4*>(keys.data());
+    for (int64_t g = g0 + lane; g < g1; g += int64_t(kWideLoads) * T) {
+      int4 q[kWideLoads];
+#pragma unroll
+      for (int u = 0; u < kWideLoads; ++u) {
+        const int64_t gi = g + int64_t(u) * T;
+        q[u]             = gi < g1 ? kv[gi] : int4{0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int u = 0; u < kWideLoads; ++u) {
+        if (g + int64_t(u) * T >= g1) continue;  // past the team's share
+        count(h, copy_base, q[u].x);
+        count(h, copy_base, q[u].y);
+        count(h, copy_base, q[u].z);
+        count(h, copy_base, q[u].w);
+      }
+    }
+    if (m.league_rank() == 0 && lane < n - groups * 4) count(h, copy_base, keys(groups * 4 + lane));
+    m.team_barrier();
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, lds_bins), [&](const int b) {
+      int32_t c = h(b);
+      if (b < hot)
+        for (int r = 0; r < kHotCopies; ++r) c += h(lds_bins + r * stride_copies + b);
       if (c) Kokkos::atomic_add(&bins(b), c);
     });
   }
@@ -752,7 +824,8 @@ int kkmx_scan_excl_f64(const double* in, double* out, int64_t n, double* total) 
 int kkmx_stencil7(const double* u, double* v, int64_t n, double c0, double c1, int t0, int t1, int t2) {
   if (n < 1 || !u || !v) return reject("kkmx_stencil7: bad arguments");
   return guarded([&] {
-    Grid3 U(const_cast<double*>(u), n + 2, n + 2, n + 2), V(v, n + 2, n + 2, n + 2);
+    Grid3In U(u, n + 2, n + 2, n + 2);
+    Grid3 V(v, n + 2, n + 2, n + 2);
     using P = Kokkos::MDRangePolicy<Kokkos::Rank<3>, Kokkos::IndexType<int64_t>>;
     if (t0 > 0)
       Kokkos::parallel_for("kkmx::stencil7", P({1, 1, 1}, {n + 1, n + 1, n + 1}, {t0, t1, t2}), Stencil7{U, V, c0, c1});
@@ -864,7 +937,8 @@ int kkmx_zipf_keys(int32_t* keys, int64_t n, double s, int32_t nbins, uint64_t s
   });
 }
 int kkmx_histogram(const int32_t* keys, int64_t n, int32_t* bins, int32_t nbins, int mode, int lds_bins) {
-  if (n < 0 || nbins < 1 || !bins || (n && !keys)) return reject("kkmx_histogram: bad arguments");
+  if (n < 0 || nbins < 1 || !bins || (n && !keys) || mode < 0 || mode > 2)
+    return reject("kkmx_histogram: bad arguments");
   return guarded([&] {
     DevView<const int32_t> K(keys, n);
     DevView<int32_t> B(bins, nbins);
@@ -875,6 +949,13 @@ int kkmx_histogram(const int32_t* keys, int64_t n, int32_t* bins, int32_t nbins,
     int lb = lds_bins > 0 ? lds_bins : 32768;
     if (lb > nbins) lb = nbins;
     const int cus = Kokkos::MI355X::impl_cu_count();
+    if (mode == 2 && reinterpret_cast<uintptr_t>(keys) % 16 == 0) {
+      const int32_t hot = lb < 256 ? lb : 256;
+      auto pol = Kokkos::TeamPolicy<Kokkos::MI355X>(cus, 1024)
+                     .set_scratch_size(0, Kokkos::PerTeam(HistWide::scratch_bytes(lb, hot)));
+      Kokkos::parallel_for("kkmx::hist_lds_wide", pol, HistWide{K, B, n, lb, hot});
+      return;
+    }
     auto pol = Kokkos::TeamPolicy<Kokkos::MI355X>(cus, 1024).set_scratch_size(0, Kokkos::PerTeam(size_t(lb) * 4));
     Kokkos::parallel_for("kkmx::hist_lds", pol, HistPrivatised{K, B, n, lb});
   });
@@ -925,15 +1006,6 @@ int kkmx_set_range_config(int pattern, int block, int items, int max_blocks) {
   }
   o.cfg[pattern] = Kokkos::Impl::RangeConfig{block, items < 1 ? 1 : items, max_blocks < 0 ? 0 : max_blocks};
   o.set[pattern] = true;
-  return 0;
-}
-
-int kkmx_set_scan_mode(int mode, int block, int items, int blocks_per_cu) {
-  if (mode < 0 || mode > 2) return reject("kkmx_set_scan_mode: mode must be 0, 1 or 2");
-  auto& o   = Kokkos::Impl::scan_mode_override();
-  o.mode    = mode;
-  o.cfg_set = block > 0;
-  if (block > 0) o.cfg = Kokkos::Impl::ScanStaticConfig{block, items, blocks_per_cu < 0 ? 0 : blocks_per_cu};
   return 0;
 }
 

@@ -249,24 +249,6 @@ int mx_func_attributes(const void* fn, int* max_threads_per_block,
   return 0;
 }
 
-int mx_launch_cooperative(const void* fn, unsigned gx, unsigned bx, size_t lds_bytes, void** args,
-                          mx_stream_t stream) {
-  if (gx == 0) return 0;
-  return rc(hipLaunchCooperativeKernel(fn, dim3(gx), dim3(bx), args, static_cast<unsigned>(lds_bytes),
-                                       static_cast<hipStream_t>(stream)));
-}
-int mx_stream_capturing(mx_stream_t stream, int* capturing) {
-  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-  const hipError_t e = hipStreamIsCapturing(static_cast<hipStream_t>(stream), &st);
-  *capturing = (e == hipSuccess && st != hipStreamCaptureStatusNone) ? 1 : 0;
-  return rc(e);
-}
-int mx_cooperative_supported(int* supported) {
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return rc(e);
-  return rc(hipDeviceGetAttribute(supported, hipDeviceAttributeCooperativeLaunch, dev));
-}
 int mx_occupancy(const void* fn, int block_size, size_t dyn_lds_bytes,
                  int* blocks_per_cu) {
   return rc(hipOccupancyMaxActiveBlocksPerMultiprocessor(

@@ -569,189 +569,6 @@ __global__ __launch_bounds__(1024) void k_range_scan_rr(
   }
 }
 
-// ------------------------------------------------------------------ K3, static tile order
-// Look-back of the static-order scan.  Workgroup b of G owns tiles b, b+G,
-// b+2G, ...; its previous tile, tile-G, is already inclusive and its prefix
-// `own` is in registers, so the walk covers at most tiles tile-G+1 .. tile-1
-// (tile `lo`-1 acts as an inclusive tile holding `own`, or the identity
-// before tile 0).  Lane j of the workgroup reads the K consecutive status
-// words w0+jK .. w0+jK+K-1 of a window of K*blockDim tiles ending at `hi`,
-// so one round trip covers that many predecessors; the nearest inclusive
-// tile ends the walk, and the ordered block reduction keeps tile order.
-template <int K, class State, class V, class Join>
-__device__ V scan_lookback_static(const State& st, int64_t tile, int64_t lo, const V& own, uint32_t epoch,
-                                  const V& identity, const Join& join, V* lds_vals, int* lds_pos) {
-  const int T         = blockDim.x;
-  const int tid       = threadIdx.x;
-  const uint32_t want = (epoch & 0x3fffffffu);
-  V excl              = identity;
-  int64_t hi          = tile - 1;
-  while (true) {
-    const int64_t w0 = hi - int64_t(K) * T + 1 + int64_t(tid) * K;
-    V vals[K];
-    int last_incl = -1;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int64_t t = w0 + k;
-      vals[k]         = identity;
-      if (t >= lo) {
-        uint32_t f = st.read(t, vals[k]);
-        int spins  = 0;
-        while ((f >> 2) != want || (f & 3) == 0) {
-          if (++spins > 8) __builtin_amdgcn_s_sleep(1);
-          f = st.read(t, vals[k]);
-        }
-        if ((f & 3) == kScanIncl) last_incl = k;
-      } else if (t == lo - 1) {
-        vals[k]   = own;
-        last_incl = k;
-      } else if (t < lo - 1) {
-        last_incl = k;  // before the boundary: identity, never reached
-      }
-    }
-    if (tid == 0) *lds_pos = -1;
-    __syncthreads();
-    const uint64_t m = wave::ballot(last_incl >= 0);
-    if (m) {
-      const int hl = 63 - __builtin_clzll(m);
-      const int ki = __builtin_amdgcn_readlane(last_incl, hl);
-      if (wave::lane() == 0) atomicMax(lds_pos, ((tid & ~63) + hl) * K + ki);
-    }
-    __syncthreads();
-    const int kpos = *lds_pos;
-    V acc          = identity;
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (tid * K + k >= kpos) join(acc, vals[k]);
-    const V agg = block_reduce(acc, identity, join, lds_vals, T);
-    V r         = agg;
-    join(r, excl);
-    excl = r;
-    if (kpos >= 0) return excl;
-    hi -= int64_t(K) * T;
-    __syncthreads();  // lds_pos is reset at the top of the next round
-  }
-}
-
-// Persistent scan over a static tile order, for launches whose G workgroups
-// are all resident at once (cooperative launch): workgroup b takes tiles b,
-// b+G, b+2G, ... in order, with no tile counter to claim from.  Forward
-// progress: the lowest unfinished tile has every predecessor finished, and
-// its workgroup is resident and done with its own earlier tiles, so it is
-// working on that tile.  Tiles are small (blockDim * IPT values, 8-16 KiB of
-// input), so the final pass's re-reads of what the functor loads still hit
-// the XCD's L2.  Per tile: pass 1 with the values kept in registers (as
-// k_range_scan_rr), aggregate published, bounded look-back, inclusive
-// prefix published, final calls.  The epoch comes from the instance's scan
-// word; the last workgroup to finish advances it (every workgroup has read it
-// by then).
-template <class CR, class Index, int IPT, int K, bool WithTotal>
-__global__ __launch_bounds__(512) void k_range_scan_static(
-    const CR cr, const int64_t begin, const int64_t end, const int64_t ntiles,
-    ScanTileState<typename CR::value_type> st, unsigned* control,
-    typename CR::value_type* total_out) {
-  using V   = typename CR::value_type;
-  using Tag = typename CR::work_tag;
-  __shared__ LdsSlots16<V> s_wave;
-  __shared__ LdsSlots16<V> s_red;
-  __shared__ uint32_t s_epoch;
-  __shared__ int s_pos;
-  const auto join      = [&cr](V& a, const V& b) { cr.join(a, b); };
-  uint64_t* claim_word = reinterpret_cast<uint64_t*>(control + 2);
-  if (threadIdx.x == 0)
-    s_epoch = static_cast<uint32_t>(__hip_atomic_load(claim_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
-  __syncthreads();
-  const uint32_t epoch = s_epoch;
-  const int64_t G      = gridDim.x;
-  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6, l = wave::lane();
-  V identity;
-  cr.init(identity);
-  const int64_t tile_elems = static_cast<int64_t>(blockDim.x) * IPT;
-  V own                    = identity;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += G) {
-    const int64_t tbase = begin + tile * tile_elems;
-    const int64_t base  = tbase + static_cast<int64_t>(w) * (64 * IPT) + l;
-    const bool full     = tbase + tile_elems <= end;
-    V inc[IPT];
-    {
-      V v[IPT];
-      if (full) {
-#pragma unroll
-        for (int r = 0; r < IPT; ++r) {
-          v[r] = identity;
-          call_functor<Tag>(cr.functor(), static_cast<Index>(base + r * 64), v[r], false);
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < IPT; ++r) {
-          const int64_t i = base + r * 64;
-          v[r]            = identity;
-          if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), v[r], false);
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < IPT; ++r) inc[r] = wave::inclusive_scan(v[r], identity, join);
-    }
-    V wave_total = identity;
-#pragma unroll
-    for (int r = 0; r < IPT; ++r) join(wave_total, wave::readlane(inc[r], 63));
-    if (l == 0) s_wave.get()[w] = wave_total;
-    __syncthreads();
-    const V x    = (l < nw) ? s_wave.get()[l] : identity;
-    const V xs   = wave::inclusive_scan(x, identity, join);
-    const V tagg = wave::readlane(xs, nw - 1);
-    const V wpre = (w == 0) ? identity : wave::readlane(xs, w - 1);
-
-    V prefix = identity;
-    if (tile == 0) {
-      if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanIncl), tagg);
-    } else {
-      if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanAgg), tagg);
-      const int64_t lo = tile - G + 1 > 0 ? tile - G + 1 : 0;
-      prefix = scan_lookback_static<K>(st, tile, lo, own, epoch, identity, join, s_red.get(), &s_pos);
-      if (threadIdx.x == 0) {
-        V t = prefix;
-        join(t, tagg);
-        st.publish(tile, scan_flag(epoch, kScanIncl), t);
-      }
-    }
-    own = prefix;
-    join(own, tagg);
-    if constexpr (WithTotal) {
-      if (threadIdx.x == 0 && tile == ntiles - 1) *total_out = own;
-    }
-    V running = prefix;
-    join(running, wpre);
-    if (full) {
-#pragma unroll
-      for (int r = 0; r < IPT; ++r) {
-        V u = running;
-        join(u, wave::shift_up(inc[r], identity));
-        join(running, wave::readlane(inc[r], 63));
-        call_functor<Tag>(cr.functor(), static_cast<Index>(base + r * 64), u, true);
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < IPT; ++r) {
-        V u = running;
-        join(u, wave::shift_up(inc[r], identity));
-        join(running, wave::readlane(inc[r], 63));
-        const int64_t i = base + r * 64;
-        if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), u, true);
-      }
-    }
-    __syncthreads();  // s_wave is rewritten by the next tile
-  }
-  if (threadIdx.x == 0) {
-    const unsigned prev = __hip_atomic_fetch_add(control + 4, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev == static_cast<unsigned>(G) - 1u) {
-      __hip_atomic_store(control + 4, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(claim_word, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
-
 // ------------------------------------------------------------------ host helpers
 template <class V>
 struct ResultSink {
@@ -1145,69 +962,6 @@ const void* range_scan_kernel(int ipt) {
   }
 }
 
-// Launch shape of the static-order scan: workgroup lanes, values per lane
-// per tile, resident workgroups per CU (0 = as many as fit).
-struct ScanStaticConfig {
-  int block         = 256;
-  int items         = 16;
-  int blocks_per_cu = 0;
-};
-// Process-wide selection of the scan kernel, for sweeps: mode 0 = heuristic,
-// 1 = tile-claim kernel, 2 = static-order kernel (when its value type allows;
-// never inside a graph capture); cfg_set overrides the static launch shape.
-struct ScanModeOverride {
-  int mode     = 0;
-  bool cfg_set = false;
-  ScanStaticConfig cfg{};
-};
-inline ScanModeOverride& scan_mode_override() {
-  static ScanModeOverride o;
-  return o;
-}
-
-// The static-order kernel keeps IPT values per lane in registers; it takes
-// values of at most 8 bytes.  Below about a million indices the tile-claim
-// kernel's single ordinary launch is as fast.
-template <class V>
-constexpr bool scan_static_eligible() {
-  return sizeof(V) <= 8 && std::is_trivially_copyable<V>::value;
-}
-inline ScanStaticConfig default_scan_static_config(size_t value_bytes) {
-  return value_bytes <= 4 ? ScanStaticConfig{256, 16, 0} : ScanStaticConfig{256, 8, 0};
-}
-constexpr int64_t kScanStaticMinExtent = int64_t(1) << 20;
-
-template <class CR, class Index, bool WithTotal>
-const void* range_scan_static_kernel(int ipt) {
-  switch (ipt >= 32 ? 32 : ipt >= 16 ? 16 : 8) {
-    case 32: return reinterpret_cast<const void*>(&k_range_scan_static<CR, Index, 32, 4, WithTotal>);
-    case 16: return reinterpret_cast<const void*>(&k_range_scan_static<CR, Index, 16, 4, WithTotal>);
-    default: return reinterpret_cast<const void*>(&k_range_scan_static<CR, Index, 8, 4, WithTotal>);
-  }
-}
-
-// Resident workgroups per CU of a kernel at a block size (cached per kernel
-// and block size; the answer depends only on the code object).
-inline int scan_blocks_per_cu(const void* k, int block) {
-  static std::mutex mtx;
-  static std::vector<std::pair<std::pair<const void*, int>, int>> cache;
-  std::lock_guard<std::mutex> lk(mtx);
-  for (auto& e : cache)
-    if (e.first.first == k && e.first.second == block) return e.second;
-  int bpc = 0;
-  KKMX_SAFE_CALL(mx_occupancy(k, block, 0, &bpc));
-  cache.push_back({{k, block}, bpc});
-  return bpc;
-}
-
-inline bool cooperative_launch_supported() {
-  static const int s = [] {
-    int v = 0;
-    return mx_cooperative_supported(&v) == 0 ? v : 0;
-  }();
-  return s != 0;
-}
-
 // Shared body of ParallelScan and ParallelScanWithTotal.
 template <class CR, class Policy, bool WithTotal>
 void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::value_type* total,
@@ -1242,43 +996,12 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
   const int ipt = c.items >= 64 ? 64 : c.items >= 32 ? 32 : c.items >= 16 ? 16 : 8;
   const int64_t tile   = static_cast<int64_t>(c.block) * ipt;
   const int64_t ntiles = ceil_div(e - b, tile);
-
-  // Static-order kernel: eager launches of large extents (a graph capture
-  // records the tile-claim kernel, whose ordinary launch needs no
-  // co-residency), unless a launch shape was requested explicitly.
-  const ScanModeOverride& mo = scan_mode_override();
-  bool use_static            = false;
-  ScanStaticConfig sc{};
-  int64_t ntiles_static = 0, grid_static = 0;
-  const void* ks        = nullptr;
-  if constexpr (scan_static_eligible<V>()) {
-    const bool wanted = mo.mode == 2 || (mo.mode == 0 && cfg == nullptr && !range_overrides().set[2] &&
-                                         e - b >= kScanStaticMinExtent);
-    if (wanted && cooperative_launch_supported()) {
-      int capturing = 0;
-      if (!g_size_only_dispatch) KKMX_SAFE_CALL(mx_stream_capturing(inst->stream, &capturing));
-      sc              = mo.cfg_set ? mo.cfg : default_scan_static_config(sizeof(V));
-      sc.block        = normalise_block(sc.block) > 512 ? 512 : normalise_block(sc.block);
-      sc.items        = sc.items >= 32 ? 32 : sc.items >= 16 ? 16 : 8;
-      ks              = range_scan_static_kernel<CR, Index, WithTotal>(sc.items);
-      ntiles_static   = ceil_div(e - b, static_cast<int64_t>(sc.block) * sc.items);
-      int bpc         = scan_blocks_per_cu(ks, sc.block);
-      if (sc.blocks_per_cu > 0 && sc.blocks_per_cu < bpc) bpc = sc.blocks_per_cu;
-      const int cus   = MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256;
-      grid_static     = static_cast<int64_t>(bpc) * cus;
-      if (grid_static > ntiles_static) grid_static = ntiles_static;
-      use_static      = !capturing && !g_size_only_dispatch && grid_static > 0;
-    }
-  }
-  // Status words for whichever kernel runs; a size-only pass (graph
-  // construction) sizes for both.
-  const int64_t ntiles_state = ntiles_static > ntiles ? ntiles_static : ntiles;
   ScanTileState<V> st;
-  void* status = scan_state_kind<V>::value == 0 ? inst->scan_flags_space(ScanTileState<V>::bytes(ntiles_state))
-                                                : inst->scan_status_space(ScanTileState<V>::bytes(ntiles_state));
+  void* status = scan_state_kind<V>::value == 0 ? inst->scan_flags_space(ScanTileState<V>::bytes(ntiles))
+                                                : inst->scan_status_space(ScanTileState<V>::bytes(ntiles));
   void* vals   = nullptr;
   if constexpr (scan_state_kind<V>::value == 0)
-    vals = inst->scan_values_space(ScanTileState<V>::value_bytes(ntiles_state));
+    vals = inst->scan_values_space(ScanTileState<V>::value_bytes(ntiles));
   st.bind(status, vals);
   V* tot_dst     = total;
   bool copy_back = false;
@@ -1286,17 +1009,11 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
     tot_dst   = static_cast<V*>(inst->pinned_space(sizeof(V)));
     copy_back = (total != nullptr);
   }
+  int64_t bb = b, ee = e, nt = ntiles;
   unsigned* control = inst->control;
-  if (use_static) {
-    int64_t bb = b, ee = e, nt = ntiles_static;
-    void* argv[] = {&cr, &bb, &ee, &nt, &st, &control, &tot_dst, nullptr};
-    KKMX_SAFE_CALL(mx_launch_cooperative(ks, static_cast<unsigned>(grid_static), sc.block, 0, argv, inst->stream));
-  } else {
-    int64_t bb = b, ee = e, nt = ntiles;
-    const void* k = range_scan_kernel<CR, Index, WithTotal>(ipt);
-    mx_launch_kernel(k, static_cast<unsigned>(ntiles), c.block, 0, inst->stream, cr, bb, ee, nt, st,
-                     control, tot_dst);
-  }
+  const void* k     = range_scan_kernel<CR, Index, WithTotal>(ipt);
+  mx_launch_kernel(k, static_cast<unsigned>(ntiles), c.block, 0, inst->stream, cr, bb, ee, nt, st,
+                   control, tot_dst);
   if (copy_back) {
     KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
     std::memcpy(static_cast<void*>(total), tot_dst, sizeof(V));

@@ -30,6 +30,13 @@ struct ParallelReduceTag {};
 struct ParallelScanTag {};
 
 // ------------------------------------------------------------------ scratch
+// A team's scratch: two arenas, level 0 (LDS) and level 1 (the instance's HBM
+// pool), each carved front to back.  A request reserves one block per thread
+// sharing the arena (`copies`, set by PerThread use) and hands this thread the
+// block at position `slot`; an exhausted arena returns nullptr and keeps its
+// cursor.  The cursor is mutable so a member that is passed by const
+// reference can still carve, as the reference's interface requires
+// (core/src/Kokkos_ScratchSpace.hpp:83-140).
 template <class ExecSpace>
 class ScratchMemorySpace {
  public:
@@ -41,55 +48,45 @@ class ScratchMemorySpace {
   using size_type       = typename ExecSpace::size_type;
 
   KOKKOS_INLINE_FUNCTION ScratchMemorySpace() = default;
-  KOKKOS_INLINE_FUNCTION ScratchMemorySpace(void* l0, size_t l0_size, void* l1 = nullptr, size_t l1_size = 0)
-      : m_iter_L0(static_cast<char*>(l0)),
-        m_end_L0(static_cast<char*>(l0) + l0_size),
-        m_iter_L1(static_cast<char*>(l1)),
-        m_end_L1(static_cast<char*>(l1) + l1_size) {}
+  KOKKOS_INLINE_FUNCTION ScratchMemorySpace(void* lds, size_t lds_bytes, void* pool = nullptr,
+                                            size_t pool_bytes = 0)
+      : m_lds{static_cast<char*>(lds), static_cast<char*>(lds) + lds_bytes},
+        m_pool{static_cast<char*>(pool), static_cast<char*>(pool) + pool_bytes} {}
 
   template <class IntType>
   KOKKOS_INLINE_FUNCTION void* get_shmem(const IntType& size, int level = -1) const {
-    return common(static_cast<size_t>(size), 1, level);
+    return carve(static_cast<size_t>(size), 1, level);
   }
   template <class IntType>
   KOKKOS_INLINE_FUNCTION void* get_shmem_aligned(const IntType& size, const ptrdiff_t align,
                                                  int level = -1) const {
-    return common(static_cast<size_t>(size), static_cast<size_t>(align), level);
+    return carve(static_cast<size_t>(size), static_cast<size_t>(align), level);
   }
-  KOKKOS_INLINE_FUNCTION const ScratchMemorySpace& set_team_thread_mode(int level, int multiplier,
-                                                                        int offset) const {
-    m_default_level = level;
-    m_multiplier    = multiplier;
-    m_offset        = offset;
+  // Later requests without a level go to `level`; each reserves `copies`
+  // blocks and returns block `slot`.
+  KOKKOS_INLINE_FUNCTION const ScratchMemorySpace& set_team_thread_mode(int level, int copies, int slot) const {
+    m_level  = level;
+    m_copies = copies;
+    m_slot   = slot;
     return *this;
   }
 
  private:
-  KOKKOS_INLINE_FUNCTION void* common(size_t size, size_t align, int level) const {
-    if (level == -1) level = m_default_level;
-    char*& it      = (level == 0) ? m_iter_L0 : m_iter_L1;
-    char* const e  = (level == 0) ? m_end_L0 : m_end_L1;
-    char* old      = it;
-    if (align > 1) {
-      const size_t mis = reinterpret_cast<uintptr_t>(it) % align;
-      if (mis) it += align - mis;
-    }
-    void* p        = it + static_cast<size_t>(m_offset) * size;
-    const size_t inc = size * static_cast<size_t>(m_multiplier);
-    if (inc > static_cast<size_t>(e - it)) {
-      it = old;
-      return nullptr;
-    }
-    it += inc;
-    return p;
+  struct Arena {
+    char* next  = nullptr;
+    char* limit = nullptr;
+  };
+  KOKKOS_INLINE_FUNCTION void* carve(size_t bytes, size_t align, int level) const {
+    Arena& a             = ((level < 0 ? m_level : level) == 0) ? m_lds : m_pool;
+    const uintptr_t at   = reinterpret_cast<uintptr_t>(a.next);
+    const uintptr_t from = align > 1 ? (at + align - 1) / align * align : at;
+    const size_t span    = bytes * static_cast<size_t>(m_copies);
+    if (from + span > reinterpret_cast<uintptr_t>(a.limit)) return nullptr;
+    a.next = reinterpret_cast<char*>(from + span);
+    return reinterpret_cast<char*>(from) + static_cast<size_t>(m_slot) * bytes;
   }
-  mutable char* m_iter_L0   = nullptr;
-  char* m_end_L0            = nullptr;
-  mutable char* m_iter_L1   = nullptr;
-  char* m_end_L1            = nullptr;
-  mutable int m_multiplier  = 1;
-  mutable int m_offset      = 0;
-  mutable int m_default_level = 0;
+  mutable Arena m_lds{}, m_pool{};
+  mutable int m_level = 0, m_copies = 1, m_slot = 0;
 };
 
 namespace Impl {

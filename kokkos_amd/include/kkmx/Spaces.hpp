@@ -395,9 +395,8 @@ void tools_finalize();
 }  // namespace Impl
 }  // namespace Tools
 
-inline void fence(const std::string& name = "Kokkos::fence: Unnamed Global Fence") {
-  MI355X::impl_static_fence(name);
-}
+// Fences every registered execution space (defined with the registry).
+inline void fence(const std::string& name = "Kokkos::fence: Unnamed Global Fence");
 
 namespace Impl {
 inline void parse_kokkos_args(int& argc, char* argv[], InitializationSettings& s) {

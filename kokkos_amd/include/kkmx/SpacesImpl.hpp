@@ -9,6 +9,7 @@
 #ifndef KKMX_SPACES_IMPL_HPP
 #define KKMX_SPACES_IMPL_HPP
 
+#include <kkmx/ExecSpaceManager.hpp>
 #include <kkmx/Tools.hpp>
 
 namespace Kokkos {
@@ -86,6 +87,10 @@ inline void MI355X::impl_finalize() {
 inline void MI355X::print_configuration(std::ostream& os, bool verbose) const {
   const auto& p = Impl::mi355x_globals().props;
   os << "Device Execution Space:\n  KOKKOS_ENABLE_MI355X: yes\n";
+  if (!Impl::mi355x_globals().default_instance) {
+    os << "  (not initialized)\n";
+    return;
+  }
   os << "  Device " << p.device << ": " << p.name << " (" << p.arch << ")\n";
   os << "  CUs: " << p.cu_count << ", wavefront: " << p.wave_size << " lanes, XCDs: " << p.xcc_count << "\n";
   os << "  LDS per workgroup: " << p.lds_per_block << " B, LDS per CU: " << p.lds_per_cu << " B\n";
@@ -93,6 +98,15 @@ inline void MI355X::print_configuration(std::ostream& os, bool verbose) const {
      << " bit, memory clock " << p.mem_clock_khz << " kHz\n";
   if (verbose) os << "  max threads per block: " << p.max_threads_per_block << "\n";
 }
+
+namespace Impl {
+inline ExecSpaceManager::ExecSpaceManager() {
+  m_spaces["100_Serial"] = make_exec_space_hooks<Serial>();
+  m_spaces["150_MI355X"] = make_exec_space_hooks<MI355X>();
+}
+}  // namespace Impl
+
+inline void fence(const std::string& name) { Impl::ExecSpaceManager::get_instance().static_fence(name); }
 
 inline void initialize(const InitializationSettings& settings_in) {
   auto& g = Impl::global_state();
@@ -102,8 +116,7 @@ inline void initialize(const InitializationSettings& settings_in) {
   Impl::parse_kokkos_env(settings);
   g.tune_internals   = settings.has_tune_internals() && settings.get_tune_internals();
   g.disable_warnings = settings.has_disable_warnings() && settings.get_disable_warnings();
-  Serial::impl_initialize(settings);
-  MI355X::impl_initialize(settings);
+  Impl::ExecSpaceManager::get_instance().initialize_spaces(settings);
   g.state.store(1);
   Tools::Impl::tools_start(settings.has_tools_libs() ? settings.get_tools_libs() : std::string(),
                            settings.has_tools_args() ? settings.get_tools_args() : std::string());
@@ -117,15 +130,13 @@ inline void finalize() {
   for (auto it = g.finalize_hooks.rbegin(); it != g.finalize_hooks.rend(); ++it) (*it)();
   g.finalize_hooks.clear();
   Tools::Impl::tools_stop();
-  MI355X::impl_finalize();
-  Serial::impl_finalize();
+  Impl::ExecSpaceManager::get_instance().finalize_spaces();
   g.state.store(2);
 }
 
 inline void print_configuration(std::ostream& os, bool verbose) {
   os << "Kokkos Version: 4.6.99 (MI355X execution space)\n";
-  Serial().print_configuration(os, verbose);
-  if (Impl::mi355x_globals().default_instance) MI355X().print_configuration(os, verbose);
+  Impl::ExecSpaceManager::get_instance().print_configuration(os, verbose);
 }
 
 }  // namespace Kokkos

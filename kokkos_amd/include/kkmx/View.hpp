@@ -366,8 +366,20 @@ class View : public ViewTraits<DataType, Properties...> {
   using size_type              = typename traits::size_type;
   using pointer_type           = value_type*;
   using element_type           = value_type;
-  using reference_type         = std::conditional_t<memory_traits::is_atomic,
-                                            Impl::AtomicDataElement<value_type>, value_type&>;
+  // Read-only device Views of arithmetic values declared RandomAccess or
+  // Restrict promise that no thread writes their data during a kernel; their
+  // elements are read by value through the constant address space, whose
+  // loads the compiler may move across stores and merge
+  // (core/src/View/Kokkos_ViewMapping.hpp:2402-2423 gives Restrict Views a
+  // restrict-qualified handle; the reference's CUDA RandomAccess Views also
+  // return elements by value).
+  static constexpr bool kkmx_invariant_reads =
+      std::is_const<value_type>::value && std::is_arithmetic<typename traits::non_const_value_type>::value &&
+      !memory_traits::is_atomic && (memory_traits::is_random_access || memory_traits::is_restrict) &&
+      std::is_same<memory_space, MI355XSpace>::value;
+  using reference_type         = std::conditional_t<
+      memory_traits::is_atomic, Impl::AtomicDataElement<value_type>,
+      std::conditional_t<kkmx_invariant_reads, const_value_type, value_type&>>;
   using data_handle_type       = pointer_type;
   using type                   = View<typename traits::data_type, array_layout, device_type, memory_traits>;
   using const_type             = View<typename traits::const_data_type, array_layout, device_type, memory_traits>;
@@ -382,7 +394,7 @@ class View : public ViewTraits<DataType, Properties...> {
   static constexpr Impl::ViewRankConstant<traits::rank> rank                 = {};
   static constexpr Impl::ViewRankConstant<traits::rank_dynamic> rank_dynamic = {};
   static constexpr unsigned Rank         = rank;
-  static constexpr bool reference_type_is_lvalue_reference = !memory_traits::is_atomic;
+  static constexpr bool reference_type_is_lvalue_reference = !memory_traits::is_atomic && !kkmx_invariant_reads;
 
   // ------------------------------------------------------------ constructors
   KOKKOS_INLINE_FUNCTION View() : m_ptr(nullptr), m_rec(nullptr) {
@@ -659,8 +671,18 @@ class View : public ViewTraits<DataType, Properties...> {
   friend class View;
 
   KOKKOS_FORCEINLINE_FUNCTION reference_type ref(size_t off) const {
-    if constexpr (memory_traits::is_atomic) return reference_type(m_ptr + off);
-    else return m_ptr[off];
+    if constexpr (memory_traits::is_atomic) {
+      return reference_type(m_ptr + off);
+    } else if constexpr (kkmx_invariant_reads) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      using invariant_ptr = const __attribute__((address_space(4))) typename traits::non_const_value_type*;
+      return *(invariant_ptr)(m_ptr + off);
+#else
+      return m_ptr[off];
+#endif
+    } else {
+      return m_ptr[off];
+    }
   }
   KOKKOS_INLINE_FUNCTION void init_static() {
     for (unsigned r = 0; r < rank; ++r) if (static_ext(r)) m_ext[r] = static_ext(r);

@@ -11,6 +11,7 @@
 #include <Kokkos_ScatterView.hpp>
 #include <Kokkos_Sort.hpp>
 
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -1122,8 +1123,8 @@ KKMX_TEST(mdrange_6d) {
   md_both<6>({1, 1, 0, 0, 2, 0}, {5, 6, 4, 3, 5, 4}, {2, 2, 2, 1, 2, 2});
 }
 
-// mdrange_array_reduce (TestMDRange_c.hpp:30): a runtime-array value over an
-// MDRange is not supported here; the scalar and struct forms are.
+// A fixed-size struct value over an MDRange (the compile-time counterpart of
+// the runtime-length arrays below).
 struct MDScalarArray {
   struct value_type {
     int64_t v[4];
@@ -1150,6 +1151,126 @@ KKMX_TEST(mdrange_struct_reduce) {
   CHECK_EQ(r.v[1], s0 * N1);
   CHECK_EQ(r.v[2], s1 * N0);
   CHECK_EQ(r.v[3], s0 * s1);
+}
+
+// mdrange_array_reduce (TestMDRange_c.hpp:30, TestMDRange.hpp:31-215):
+// value_type = double[] with a runtime value_count, once with the default
+// init/join (zero, +) and once with the functor's own; the InitTag
+// parallel_for sets every element to 3, so the sums are 6*N and 3*N.
+struct MDArrayReduce {
+  using scalar_type = double;
+  using value_type  = scalar_type[];
+  Kokkos::View<int***, Exec> in;
+  unsigned value_count;
+  struct InitTag {};
+  KOKKOS_INLINE_FUNCTION void operator()(const InitTag&, const int i, const int j, const int k) const {
+    in(i, j, k) = 3;
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(const int i, const int j, const int k, value_type lsum) const {
+    lsum[0] += in(i, j, k) * 2;
+    lsum[1] += in(i, j, k);
+    for (unsigned c = 2; c < value_count; ++c) lsum[c] += double(i) * (c == 2) + double(j) * (c == 3) + k * (c > 3);
+  }
+};
+// Same functor with its own init and join (TestMDRange.hpp:114-126).
+struct MDArrayReduceOwnJoin : MDArrayReduce {
+  KOKKOS_INLINE_FUNCTION void init(scalar_type dst[]) const {
+    for (unsigned c = 0; c < value_count; ++c) dst[c] = 0.0;
+  }
+  KOKKOS_INLINE_FUNCTION void join(scalar_type dst[], const scalar_type src[]) const {
+    for (unsigned c = 0; c < value_count; ++c) dst[c] += src[c];
+  }
+};
+template <class F>
+void md_array_reduce_case(int N0, int N1, int N2, unsigned count, bool device_result) {
+  F f;
+  f.in          = Kokkos::View<int***, Exec>("in", N0, N1, N2);
+  f.value_count = count;
+  using Init    = Kokkos::MDRangePolicy<Exec, Kokkos::Rank<3>, Kokkos::IndexType<int>, typename F::InitTag>;
+  using Pol     = Kokkos::MDRangePolicy<Exec, Kokkos::Rank<3>, Kokkos::IndexType<int>>;
+  Kokkos::parallel_for(Init({0, 0, 0}, {N0, N1, N2}, {3, 3, 3}), f);
+  std::vector<double> sums(count, -1.0);
+  if (device_result) {
+    Kokkos::View<double*, Exec> r("r", count);
+    Kokkos::parallel_reduce(Pol({0, 0, 0}, {N0, N1, N2}), f, r);
+    auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), r);
+    for (unsigned c = 0; c < count; ++c) sums[c] = h(c);
+  } else {
+    Kokkos::parallel_reduce(Pol({0, 0, 0}, {N0, N1, N2}, {3, 3, 3}), f, sums.data());
+  }
+  const double n = double(N0) * N1 * N2;
+  CHECK_EQ(sums[0], 6 * n);
+  CHECK_EQ(sums[1], 3 * n);
+  const double s0 = double(N0) * (N0 - 1) / 2 * N1 * N2, s1 = double(N1) * (N1 - 1) / 2 * N0 * N2,
+               s2 = double(N2) * (N2 - 1) / 2 * N0 * N1;
+  for (unsigned c = 2; c < count; ++c) CHECK_EQ(sums[c], c == 2 ? s0 : c == 3 ? s1 : s2);
+}
+KKMX_TEST(mdrange_array_reduce) {
+  md_array_reduce_case<MDArrayReduce>(4, 5, 1, 2, false);
+  md_array_reduce_case<MDArrayReduceOwnJoin>(4, 5, 10, 2, false);
+  md_array_reduce_case<MDArrayReduce>(130, 67, 33, 7, false);
+  md_array_reduce_case<MDArrayReduceOwnJoin>(257, 31, 64, 5, true);
+  md_array_reduce_case<MDArrayReduce>(1, 1, 1, 40, true);
+}
+
+// team_array_reduce (TestTeam.hpp:297-333,399-433): TeamPolicy with
+// value_type = T[] and value_count = 3; each of the league*team threads takes
+// a chunk of [0, nwork) and adds 1, i+1 and nwork-i, so the sums are nwork,
+// nwork(nwork+1)/2 and the same.  Results into host arrays and device Views.
+template <class T>
+struct TeamArrayReduce {
+  using value_type = T[];
+  unsigned value_count = 3;
+  int64_t nwork;
+  KOKKOS_INLINE_FUNCTION void init(value_type dst) const {
+    for (unsigned i = 0; i < value_count; ++i) dst[i] = 0;
+  }
+  KOKKOS_INLINE_FUNCTION void join(value_type dst, const value_type src) const {
+    for (unsigned i = 0; i < value_count; ++i) dst[i] += src[i];
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(const Kokkos::TeamPolicy<Exec>::member_type& team, value_type dst) const {
+    const int64_t rank  = team.team_rank() + int64_t(team.team_size()) * team.league_rank();
+    const int64_t size  = int64_t(team.team_size()) * team.league_size();
+    const int64_t chunk = (nwork + size - 1) / size;
+    int64_t i           = chunk * rank;
+    const int64_t e     = i + chunk < nwork ? i + chunk : nwork;
+    for (; i < e; ++i) {
+      dst[0] += 1;
+      dst[1] += i + 1;
+      dst[2] += nwork - i;
+    }
+  }
+};
+template <class T>
+void team_array_reduce_case(int64_t nwork, bool device_result) {
+  using Pol = Kokkos::TeamPolicy<Exec>;
+  TeamArrayReduce<T> f;
+  f.nwork                = nwork;
+  const unsigned ts      = Pol(1, 1).team_size_recommended(f, Kokkos::ParallelReduceTag());
+  const int64_t league   = (nwork + ts - 1) / ts;
+  const uint64_t nw      = uint64_t(nwork);
+  const uint64_t nsum    = nw % 2 ? nw * ((nw + 1) / 2) : (nw / 2) * (nw + 1);
+  for (int rep = 0; rep < 3; ++rep) {
+    T r[3] = {T(-1), T(-1), T(-1)};
+    if (device_result) {
+      Kokkos::View<T*, Exec> d("d", 3);
+      Kokkos::parallel_reduce(Pol(league > 0 ? league : 1, ts), f, d);
+      Kokkos::deep_copy(Kokkos::View<T*, Kokkos::HostSpace, Kokkos::MemoryUnmanaged>(r, 3), d);
+    } else {
+      Kokkos::View<T*, Kokkos::HostSpace, Kokkos::MemoryUnmanaged> h(r, 3);
+      Kokkos::parallel_reduce(Pol(league > 0 ? league : 1, ts), f, h);
+    }
+    CHECK_EQ(uint64_t(r[0]), nw);
+    CHECK_EQ(uint64_t(r[1]), nsum);
+    CHECK_EQ(uint64_t(r[2]), nsum);
+  }
+}
+KKMX_TEST(team_array_reduce) {
+  for (int64_t n : {int64_t(0), int64_t(1), int64_t(1000), int64_t(10007), int64_t(1) << 20}) {
+    team_array_reduce_case<int64_t>(n, false);
+    team_array_reduce_case<double>(n, true);
+    team_array_reduce_case<long>(n, n % 2 == 0);
+  }
 }
 
 // =========================================================================
@@ -3037,6 +3158,56 @@ KKMX_TEST(team_combined_reducers) {
 }  // namespace
 
 // ---------------------------------------------------------------- C-ABI
+// =========================================================================
+// Backend registration (core/src/impl/Kokkos_ExecSpaceManager.hpp:157-163):
+// a host execution space registered from a namespace-scope initialiser is
+// initialised by Kokkos::initialize (when the registration came first),
+// fenced by Kokkos::fence and printed by Kokkos::print_configuration.
+// =========================================================================
+struct ProbeCounters {
+  static inline std::atomic<int> inits{0}, finals{0}, fences{0};
+};
+class SelftestProbeSpace {
+ public:
+  using execution_space      = SelftestProbeSpace;
+  using memory_space         = Kokkos::HostSpace;
+  using device_type          = Kokkos::Device<execution_space, memory_space>;
+  using array_layout         = Kokkos::LayoutRight;
+  using size_type            = size_t;
+  using scratch_memory_space = Kokkos::ScratchMemorySpace<Kokkos::Serial>;
+  static constexpr const char* name() { return "SelftestProbe"; }
+  int concurrency() const { return 1; }
+  void fence(const std::string& = "") const {}
+  static void impl_static_fence(const std::string&) { ++ProbeCounters::fences; }
+  void print_configuration(std::ostream& os, bool = false) const { os << "SelftestProbe execution space\n"; }
+  static void impl_initialize(const Kokkos::InitializationSettings&) { ++ProbeCounters::inits; }
+  static void impl_finalize() { ++ProbeCounters::finals; }
+  bool operator==(const SelftestProbeSpace&) const { return true; }
+  bool operator!=(const SelftestProbeSpace&) const { return false; }
+};
+const bool g_probe_registered_before_init = !Kokkos::is_initialized();
+const int g_probe_registered = Kokkos::Impl::initialize_space_factory<SelftestProbeSpace>("200_SelftestProbe");
+
+KKMX_TEST(exec_space_registration) {
+  CHECK_EQ(g_probe_registered, 1);
+  auto& m = Kokkos::Impl::ExecSpaceManager::get_instance();
+  CHECK(m.is_registered("100_Serial"));
+  CHECK(m.is_registered("150_MI355X"));
+  CHECK(m.is_registered("200_SelftestProbe"));
+  CHECK_EQ(ProbeCounters::inits.load(), g_probe_registered_before_init ? 1 : 0);
+  CHECK_EQ(ProbeCounters::finals.load(), 0);
+  const int f0 = ProbeCounters::fences.load();
+  Kokkos::fence("selftest: global fence reaches every registered space");
+  CHECK_EQ(ProbeCounters::fences.load(), f0 + 1);
+  std::ostringstream os;
+  Kokkos::print_configuration(os, false);
+  const std::string txt = os.str();
+  const size_t ps = txt.find("Host Serial Execution Space"), pm = txt.find("KOKKOS_ENABLE_MI355X"),
+               pp = txt.find("SelftestProbe execution space");
+  CHECK(ps != std::string::npos && pm != std::string::npos && pp != std::string::npos);
+  CHECK(ps < pm && pm < pp);  // in the order of the keys
+}
+
 extern "C" {
 
 int kkmx_selftest_count(void) { return static_cast<int>(registry().size()); }

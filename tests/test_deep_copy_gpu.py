@@ -2,7 +2,7 @@
 core/src/Kokkos_CopyViews.hpp:58-420): fills, contiguous copies and
 layout-changing copies, bit-exact against numpy on the same inputs.  The
 strided-window, rank-2/3/5 and stream-ordered cases run as C++ self-tests
-(deep_copy_* in csrc/selftest.hip)."""
+(deep_copy_* in tests/cpp/selftest.hip)."""
 import numpy as np
 import pytest
 

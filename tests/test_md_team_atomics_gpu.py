@@ -155,7 +155,8 @@ def test_zipf_keys_match_oracle(paths, oracle, n):
     assert np.array_equal(Kb.download(np.int32, n), oracle.zipf_keys(n, 1.1, 65536, 8))
 
 
-@pytest.mark.parametrize("mode,lds_bins", [(0, 0), (1, 32768), (1, 1024), (1, 65536 // 2 + 7)])
+@pytest.mark.parametrize("mode,lds_bins", [(0, 0), (1, 32768), (1, 1024), (1, 65536 // 2 + 7), (2, 32768),
+                                           (2, 100), (2, 65536 // 2 + 7)])
 def test_histogram_bit_exact(paths, oracle, mode, lds_bins):
     P = paths
     n, nb = 1 << 22, 65536
@@ -177,9 +178,27 @@ def test_histogram_full_size_config4(paths, oracle):
     run(P, P.kkmx_zipf_keys(Kb.ptr, n, 1.1, nb, 8))
     keys = oracle.zipf_keys(n, 1.1, nb, 8)
     assert oracle.checksum(Kb.download(np.int32, n)) == oracle.checksum(keys)
-    K.load("kkmx_rt").mx_memset_async(Bb.ptr, 0, nb * 4, P.kkmx_stream())
-    run(P, P.kkmx_histogram(Kb.ptr, n, Bb.ptr, nb, 1, 32768))
-    assert np.array_equal(Bb.download(np.int32, nb), oracle.histogram(keys, nb))
+    ref = oracle.histogram(keys, nb)
+    for mode in (1, 2):
+        K.load("kkmx_rt").mx_memset_async(Bb.ptr, 0, nb * 4, P.kkmx_stream())
+        run(P, P.kkmx_histogram(Kb.ptr, n, Bb.ptr, nb, mode, 32768))
+        assert np.array_equal(Bb.download(np.int32, nb), ref), f"mode {mode}"
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 4, 5, 1023, 4097, 1 << 16])
+def test_histogram_wide_ragged(paths, oracle, n):
+    """Mode 2 on extents that are not multiples of 4 keys or of a team's
+    share, and on keys at an offset of 4 bytes (not 16-byte aligned, so the
+    entry point runs mode 1)."""
+    P = paths
+    nb = 300
+    keys = oracle.fill_uniform_i32(n + 1, 81, 0, nb - 1)
+    Kb, Bb = dev(keys), K.DeviceBuffer(nb * 4)
+    for off in (0, 1):
+        K.load("kkmx_rt").mx_memset_async(Bb.ptr, 0, nb * 4, P.kkmx_stream())
+        run(P, P.kkmx_histogram(Kb.ptr + 4 * off, n, Bb.ptr, nb, 2, 128))
+        got = Bb.download(np.int32, nb)
+        assert np.array_equal(got, oracle.histogram(np.ascontiguousarray(keys[off:off + n]), nb)), off
 
 
 # ---------------------------------------------------------------- FP64 scatter

@@ -3,7 +3,7 @@ through the C ABI, against the oracle on the same seeded inputs: the
 float64 scatter-add within 1e-10 relative per bin (atomic order varies),
 min / max exact, the Zipf histogram bit-exact.  Reset, contribute into a
 separate View, prod and the integer forms run as C++ self-tests
-(scatter_view_* in csrc/selftest.hip)."""
+(scatter_view_* in tests/cpp/selftest.hip)."""
 import numpy as np
 import pytest
 

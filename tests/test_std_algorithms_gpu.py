@@ -3,7 +3,7 @@
 against numpy / the oracle on the same seeded inputs: integers bit-exact,
 float64 within 1e-12 relative of a compensated reference.  The iterator
 forms (sub-ranges, strided Views, returned iterators) run as C++ self-tests
-(std_* in csrc/selftest.hip)."""
+(std_* in tests/cpp/selftest.hip)."""
 import ctypes as C
 import math
 
