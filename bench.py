@@ -382,7 +382,8 @@ def leg_atomics(K, P, timer):
     nk, nb = 1 << 28, 65536
     Kb, Bb = K.DeviceBuffer(nk * 4), K.DeviceBuffer(nb * 4)
     chk(P.kkmx_zipf_keys(Kb.ptr, nk, 1.1, nb, 8))
-    h = _timeit(K, timer, lambda: chk(P.kkmx_histogram(Kb.ptr, nk, Bb.ptr, nb, 2, 32768)), reps=5)
+    h = _timeit(K, timer, lambda: chk(P.kkmx_histogram(Kb.ptr, nk, Bb.ptr, nb, 3, 32768)), reps=5)
+    h2 = _timeit(K, timer, lambda: chk(P.kkmx_histogram(Kb.ptr, nk, Bb.ptr, nb, 2, 32768)), reps=5)
     h1 = _timeit(K, timer, lambda: chk(P.kkmx_histogram(Kb.ptr, nk, Bb.ptr, nb, 1, 32768)), reps=5)
     hsv = _timeit(K, timer, lambda: chk(P.kkmx_scatter_view_histogram(Kb.ptr, nk, Bb.ptr, nb, 0)), reps=2, warm=1)
     hdup = _timeit(K, timer, lambda: chk(P.kkmx_scatter_view_histogram(Kb.ptr, nk, Bb.ptr, nb, 1)), reps=5)
@@ -398,7 +399,9 @@ def leg_atomics(K, P, timer):
         "gups_2^30_table_2^28_updates": {"ms": round(g[0], 3), "gups": round(U / g[0] / 1e6, 3)},
         "histogram_zipf_2^28_keys": {"ms": round(h[0], 3), "gbps": round(4 * nk / h[0] / 1e6, 1),
                                      "keys_per_s": round(nk / h[0] * 1e3, 1),
-                                     "kernel": "16-byte key loads, 256 hot bins replicated 16x in LDS"},
+                                     "kernel": "LDS bins with 256 hot bins replicated 16x, 16-byte key loads, "
+                                               "team rows summed by a second kernel (no device-scope atomics)"},
+        "histogram_zipf_2^28_keys_global_atomic_flush": {"ms": round(h2[0], 3), "gbps": round(4 * nk / h2[0] / 1e6, 1)},
         "histogram_zipf_2^28_keys_one_key_per_load": {"ms": round(h1[0], 3), "gbps": round(4 * nk / h1[0] / 1e6, 1)},
         "scatter_f64_2^26_into_2^20": {"ms": round(sc[0], 3), "updates_per_s": round(ns / sc[0] * 1e3, 1),
                                        "gbps": round(12 * ns / sc[0] / 1e6, 1)},
@@ -501,7 +504,99 @@ def leg_batched(K, P, timer, replays=1000, plain_steps=5):
     out["speedup_fused_graph_vs_plain"] = round(out["fused_graph"]["ops_per_s"] / out["plain"]["ops_per_s"], 2)
     if run is not None:
         run.close()
+    if model.exists():
+        from kokkos_amd import tuner as T
+
+        rep = json.loads(T.MODEL_META.read_text()).get("report", {}) if T.MODEL_META.exists() else {}
+        out["tuner"] = {
+            "batch_4096_ops": tuner_in_situ_batch(K, P, timer, wl),
+            "batch_4096_small_ops": tuner_in_situ_batch(K, P, timer, W.batch_workload(4096, 11, 4.0, 12.0)),
+            "offline_heldout_regret": rep.get("test"),
+        }
     return out
+
+
+def tuner_in_situ_range(K, P, calls, patterns, n, reps=5):
+    """Regret of the learned tuner and of the heuristic against exhaustive
+    search on the headline's own dispatches: each dispatch is timed under
+    the 16 launch-shape candidates (fixed with kkmx_set_range_config), under
+    the heuristic (no override) and with the tuning tool loaded; the tool's
+    choice is read back from it."""
+    import ctypes as C
+
+    from kokkos_amd import tuner as T
+
+    model = ROOT / "kokkos_amd" / "tuner" / "mlp_gfx950.bin"
+    if not model.exists():
+        return None
+    timer = K.EventTimer()
+    chk = lambda rc: K.check(rc, P)  # noqa: E731
+    out = {}
+    for (name, _), f, pat in zip(OPS, calls, patterns):
+        t_cand = []
+        for c in range(16):
+            b, it, mb = C.c_int(), C.c_int(), C.c_int()
+            chk(P.kkmx_range_candidate(pat, c, n, C.byref(b), C.byref(it), C.byref(mb)))
+            chk(P.kkmx_set_range_config(pat, b.value, it.value, mb.value))
+            t_cand.append(_timeit(K, timer, lambda: chk(f()), reps=reps, warm=1)[0])
+        chk(P.kkmx_set_range_config(pat, -1, 0, 0))
+        t_heur = _timeit(K, timer, lambda: chk(f()), reps=reps, warm=1)[0]
+        chk(P.kkmx_tools_load(str(K.lib_path("kkmx_tuner")).encode()))
+        try:
+            T.load_into_tool(model)
+            chk(f())
+            K.fence()
+            ans = T.last_answer()
+            t_tool = _timeit(K, timer, lambda: chk(f()), reps=reps, warm=1)[0]
+        finally:
+            chk(P.kkmx_tools_unload())
+        best = min(t_cand)
+        chosen = ans[2] if ans and ans[0] == 0 else None
+        heur_c = ans[1] if ans and ans[0] == 0 else None
+        out[name] = {
+            "best_candidate": int(np.argmin(t_cand)), "best_ms": round(best, 4),
+            "heuristic_candidate": heur_c, "heuristic_ms": round(t_heur, 4),
+            "tuner_candidate": chosen, "tuner_ms": round(t_tool, 4),
+            "regret_tuner": round(t_tool / best - 1, 4), "regret_heuristic": round(t_heur / best - 1, 4),
+        }
+    rt = [v["regret_tuner"] for v in out.values()]
+    rh = [v["regret_heuristic"] for v in out.values()]
+    return {"dispatches": out, "mean_regret_tuner": round(float(np.mean(rt)), 4),
+            "mean_regret_heuristic": round(float(np.mean(rh)), 4)}
+
+
+def tuner_in_situ_batch(K, P, timer, wl, replays=100):
+    """Exhaustive search over the fused batch kernel's 8 tile shapes for one
+    workload (graph replays), against the heuristic's shape (5) and the shape
+    the learned tuner picks at plan time."""
+    from kokkos_amd import batch as B
+    from kokkos_amd import tuner as T
+
+    def step_ms(run):
+        run.run(B.MODE_FUSED_GRAPH, 1)
+        K.fence()
+        timer.start()
+        run.run(B.MODE_FUSED_GRAPH, replays)
+        return timer.stop() / replays
+
+    t = []
+    for shape in range(8):
+        run = B.BatchRun(wl, shape)
+        t.append(step_ms(run))
+        run.close()
+    K.check(P.kkmx_tools_load(str(K.lib_path("kkmx_tuner")).encode()), P)
+    try:
+        T.load_into_tool(ROOT / "kokkos_amd" / "tuner" / "mlp_gfx950.bin")
+        run = B.BatchRun(wl)
+        chosen = run.shape
+        t_tool = step_ms(run)
+        run.close()
+    finally:
+        K.check(P.kkmx_tools_unload(), P)
+    best = min(t)
+    return {"ms_per_shape": [round(x, 4) for x in t], "best_shape": int(np.argmin(t)), "heuristic_shape": 5,
+            "tuner_shape": chosen, "tuner_ms": round(t_tool, 4),
+            "regret_tuner": round(t_tool / best - 1, 4), "regret_heuristic": round(t[5] / best - 1, 4)}
 
 
 def main():
@@ -604,6 +699,7 @@ def main():
     checks = verify_step(K, P, calls, a, b, c, x, s, so, res)
     ok = all(checks.values())
 
+    tuner_range = None if args.no_extra else tuner_in_situ_range(K, P, calls, [0, 0, 0, 0, 1, 1, 1, 2], N)
     peak = measured_copy_peak(K, a.ptr, b.ptr, N * 8, K.EventTimer())
     del a, b, c, x, s, so
 
@@ -679,6 +775,8 @@ def main():
     if extra:
         line["workloads"] = extra
         bt = extra["batched"]
+        if tuner_range is not None and "tuner" in bt:
+            bt["tuner"]["headline_range_dispatches"] = tuner_range
         line["batched_ops_per_s"] = {
             "fused_graph": bt["fused_graph"]["ops_per_s"],
             "plain": bt["plain"]["ops_per_s"],

@@ -144,7 +144,13 @@ int kkmx_spmv_csr(const int64_t* rowptr_dev, const int32_t* col_dev, const doubl
  * kkmx_histogram: mode 0 = global atomics, 1 = bins [0, lds_bins) privatised
  *   in team scratch (LDS atomics), the rest global; 2 = as 1 with 16-byte key
  *   loads and the lowest 256 bins replicated 16 times in LDS (keys 16-byte
- *   aligned; otherwise mode 1 runs).
+ *   aligned; otherwise mode 1 runs); 3 = 16-byte key loads, the lowest 128
+ *   bins replicated 64 times (one copy per lane of a wavefront), bins at or
+ *   above lds_bins counted in a per-team row of a workspace and the team's
+ *   LDS counts stored into the same row, the rows then summed into bins by a
+ *   second kernel (no device-scope atomics; the workspace, teams x nbins
+ *   int32, is kept between calls and released by kkmx_finalize).  Modes 2
+ *   and 3 lower lds_bins to what fits in LDS beside the replicas.
  * kkmx_scatter_add_f64: bins[idx[i]] += vals[i] with FP64 atomic_add. */
 int kkmx_xorshift64_stream(uint64_t* out_dev, int64_t n, uint64_t seed);
 int kkmx_gups(int64_t* table_dev, int64_t table_size, const uint64_t* stream_dev, int64_t nupdates);
@@ -203,6 +209,11 @@ int kkmx_tools_unload(void);
 
 /* ---- launch-shape overrides (what the tuning hooks set), -1 = heuristic ---- */
 int kkmx_set_range_config(int pattern, int block, int items, int max_blocks);
+/* The launch shape of tuning candidate `candidate` (0-15) of the Range
+ * launch-shape variable for a dispatch of n indices (pattern 0 for, 1
+ * reduce, 2 scan): what a tuning tool's answer selects, and what
+ * kkmx_set_range_config then fixes for an exhaustive timing. */
+int kkmx_range_candidate(int pattern, int candidate, int64_t n, int* block, int* items, int* max_blocks);
 /* MDRange parallel_for coarsening: tiles per workgroup along the slowest tile
  * dimension (0 = heuristic). */
 int kkmx_set_mdrange_depth(int depth);

@@ -111,6 +111,7 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_scan_excl_i64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_scan_excl_f64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_set_range_config", [C.c_int, C.c_int, C.c_int, C.c_int])
+        _sig(lib, "kkmx_range_candidate", [C.c_int, C.c_int, i64] + [C.POINTER(C.c_int)] * 3)
         _sig(lib, "kkmx_stencil7", [vp, vp, i64, f64, f64, C.c_int, C.c_int, C.c_int])
         _sig(lib, "kkmx_stencil7_sweeps", [vp, vp, i64, C.c_int, f64, f64, C.c_int, C.c_int, C.c_int])
         _sig(lib, "kkmx_csr_generate", [vp, vp, vp, i64, i64, i64, i64, u64, C.POINTER(i64)])

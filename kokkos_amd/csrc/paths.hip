@@ -422,6 +422,122 @@ struct HistWide {
     });
   }
 };
+// Mode 3 of the histogram: no device-scope atomics.  Each team owns one row
+// of a [teams x nbins] table in HBM.  It counts bins [0, lds_bins) in LDS as
+// mode 2 does (with one replica of the hot bins per lane instead of 16
+// replicas) and the remaining bins straight into its own row with
+// workgroup-scope atomics (desul::MemoryScopeCore: no other team touches the
+// row, so the update resolves in the XCD's L2), then stores its LDS counts
+// into the row.  HistRowSum adds the column of every bin into `bins`.
+struct HistRows {
+  // One replica of the hot bins per lane of a wavefront: lanes never add to
+  // the same LDS word for a hot key.
+  static constexpr int kHotCopies = 64;
+  static constexpr int kHotBins   = 128;
+  static size_t scratch_bytes(int32_t lds_bins, int32_t hot) {
+    return (size_t(lds_bins) + size_t(kHotCopies) * (hot + 1)) * 4;
+  }
+  DevView<const int32_t> keys;
+  Kokkos::View<int32_t**, Kokkos::LayoutRight, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>> rows;
+  int64_t n;
+  int32_t nbins;
+  int32_t lds_bins;
+  int32_t hot;
+  using member_type = Kokkos::TeamPolicy<Kokkos::MI355X>::member_type;
+  using Lds         = HistWide::Lds;
+  KOKKOS_INLINE_FUNCTION void count(const Lds& h, int32_t* row, int32_t copy_base, int32_t k) const {
+    if (k < hot) Kokkos::atomic_add(&h(lds_bins + copy_base + k), 1);
+    else if (k < lds_bins) Kokkos::atomic_add(&h(k), 1);
+    else desul::atomic_add(row + k, 1, desul::MemoryOrderRelaxed(), desul::MemoryScopeCore());
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(const member_type& m) const {
+    constexpr int kCopies       = kHotCopies;
+    constexpr int kLoads        = HistWide::kWideLoads;
+    const int32_t stride_copies = hot + 1;
+    const int32_t words         = lds_bins + kCopies * stride_copies;
+    int32_t* row                = &rows(m.league_rank(), 0);
+    Lds h(m.team_scratch(0), words);
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, words), [&](const int b) { h(b) = 0; });
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, lds_bins, nbins), [&](const int b) { row[b] = 0; });
+    m.team_barrier();
+    const int lane          = m.team_rank();
+    const int T             = m.team_size();
+    const int32_t copy_base = (lane % kCopies) * stride_copies;
+    const int64_t groups    = n / 4;
+    const int64_t nteams    = m.league_size();
+    const int64_t chunk     = (groups + nteams - 1) / nteams;
+    const int64_t g0        = chunk * m.league_rank();
+    const int64_t g1        = g0 + chunk < groups ? g0 + chunk : groups;
+    const int4* kv          = reinterpret_cast<const int4*>(keys.data());
+    for (int64_t g = g0 + lane; g < g1; g += int64_t(kLoads) * T) {
+      int4 q[kLoads];
+#pragma unroll
+      for (int u = 0; u < kLoads; ++u) {
+        const int64_t gi = g + int64_t(u) * T;
+        q[u]             = gi < g1 ? kv[gi] : int4{0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int u = 0; u < kLoads; ++u) {
+        if (g + int64_t(u) * T >= g1) continue;
+        count(h, row, copy_base, q[u].x);
+        count(h, row, copy_base, q[u].y);
+        count(h, row, copy_base, q[u].z);
+        count(h, row, copy_base, q[u].w);
+      }
+    }
+    if (m.league_rank() == 0 && lane < n - groups * 4) count(h, row, copy_base, keys(groups * 4 + lane));
+    m.team_barrier();
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, lds_bins), [&](const int b) {
+      int32_t c = h(b);
+      if (b < hot)
+        for (int r = 0; r < kCopies; ++r) c += h(lds_bins + r * stride_copies + b);
+      row[b] = c;
+    });
+  }
+};
+// One team of 8 wavefronts per 64 consecutive bins: wavefront w sums rows
+// w, w+8, ... of the team's bins (64 lanes = 64 consecutive bins, so each
+// load is one contiguous 256-byte run), and the 8 partial sums meet in LDS.
+struct HistRowSum {
+  static constexpr int kBins = 64, kWaves = 8;
+  Kokkos::View<const int32_t**, Kokkos::LayoutRight, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>> rows;
+  DevView<int32_t> bins;
+  using member_type = Kokkos::TeamPolicy<Kokkos::MI355X>::member_type;
+  KOKKOS_INLINE_FUNCTION void operator()(const member_type& m) const {
+    Kokkos::View<int32_t**, Kokkos::MI355X::scratch_memory_space, Kokkos::MemoryTraits<Kokkos::Unmanaged>> part(
+        m.team_scratch(0), kWaves, kBins);
+    const int w = m.team_rank() / kBins, l = m.team_rank() % kBins;
+    const int64_t b = int64_t(m.league_rank()) * kBins + l;
+    const int64_t nrows = int64_t(rows.extent(0));
+    int32_t c = 0;
+    if (b < int64_t(bins.extent(0))) {
+#pragma unroll 8
+      for (int64_t r = w; r < nrows; r += kWaves) c += rows(r, b);
+    }
+    part(w, l) = c;
+    m.team_barrier();
+    if (w == 0 && b < int64_t(bins.extent(0))) {
+      int32_t t = 0;
+      for (int k = 0; k < kWaves; ++k) t += part(k, l);
+      bins(b) += t;
+    }
+  }
+};
+// Grow-only table of team rows for mode 3, released by Kokkos::finalize.
+int32_t* hist_rows_space(size_t elems) {
+  static Kokkos::View<int32_t*, Kokkos::MI355XSpace>* rows = nullptr;
+  if (!rows) {
+    rows = new Kokkos::View<int32_t*, Kokkos::MI355XSpace>();
+    Kokkos::push_finalize_hook([] {
+      delete rows;
+      rows = nullptr;
+    });
+  }
+  if (rows->extent(0) < elems)
+    *rows = Kokkos::View<int32_t*, Kokkos::MI355XSpace>(
+        Kokkos::view_alloc(Kokkos::WithoutInitializing, std::string("kkmx_hist_rows")), elems);
+  return rows->data();
+}
 struct ScatterAdd {
   DevView<const double> vals;
   DevView<const int32_t> idx;
@@ -937,7 +1053,7 @@ int kkmx_zipf_keys(int32_t* keys, int64_t n, double s, int32_t nbins, uint64_t s
   });
 }
 int kkmx_histogram(const int32_t* keys, int64_t n, int32_t* bins, int32_t nbins, int mode, int lds_bins) {
-  if (n < 0 || nbins < 1 || !bins || (n && !keys) || mode < 0 || mode > 2)
+  if (n < 0 || nbins < 1 || !bins || (n && !keys) || mode < 0 || mode > 3)
     return reject("kkmx_histogram: bad arguments");
   return guarded([&] {
     DevView<const int32_t> K(keys, n);
@@ -949,7 +1065,32 @@ int kkmx_histogram(const int32_t* keys, int64_t n, int32_t* bins, int32_t nbins,
     int lb = lds_bins > 0 ? lds_bins : 32768;
     if (lb > nbins) lb = nbins;
     const int cus = Kokkos::MI355X::impl_cu_count();
-    if (mode == 2 && reinterpret_cast<uintptr_t>(keys) % 16 == 0) {
+    if (mode >= 2) {
+      // Modes 2 and 3 keep as many bins in LDS as fit beside the hot copies.
+      const int32_t hot0  = lb < 256 ? lb : 256;
+      const size_t budget = Kokkos::TeamPolicy<Kokkos::MI355X>(cus, 1024).scratch_size_max(0);
+      const int64_t fit   = int64_t(budget / 4) - int64_t(HistWide::kHotCopies) * (hot0 + 1);
+      if (lb > fit) lb = int32_t(fit);
+    }
+    if (mode == 3 && reinterpret_cast<uintptr_t>(keys) % 16 == 0) {
+      int32_t lb3       = lds_bins > 0 ? lds_bins : 32768;
+      if (lb3 > nbins) lb3 = nbins;
+      const int32_t hot = lb3 < HistRows::kHotBins ? lb3 : HistRows::kHotBins;
+      const size_t budget = Kokkos::TeamPolicy<Kokkos::MI355X>(cus, 1024).scratch_size_max(0);
+      const int64_t fit   = int64_t(budget / 4) - int64_t(HistRows::kHotCopies) * (hot + 1);
+      if (lb3 > fit) lb3 = int32_t(fit);
+      int32_t* rows = hist_rows_space(size_t(cus) * nbins);
+      using Rows = Kokkos::View<int32_t**, Kokkos::LayoutRight, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+      auto pol = Kokkos::TeamPolicy<Kokkos::MI355X>(cus, 1024)
+                     .set_scratch_size(0, Kokkos::PerTeam(HistRows::scratch_bytes(lb3, hot)));
+      Kokkos::parallel_for("kkmx::hist_rows", pol, HistRows{K, Rows(rows, cus, nbins), n, nbins, lb3, hot});
+      const int sum_teams = (nbins + HistRowSum::kBins - 1) / HistRowSum::kBins;
+      auto sum_pol = Kokkos::TeamPolicy<Kokkos::MI355X>(sum_teams, HistRowSum::kBins * HistRowSum::kWaves)
+                         .set_scratch_size(0, Kokkos::PerTeam(size_t(HistRowSum::kBins) * HistRowSum::kWaves * 4));
+      Kokkos::parallel_for("kkmx::hist_row_sum", sum_pol, HistRowSum{decltype(HistRowSum::rows)(rows, cus, nbins), B});
+      return;
+    }
+    if (mode >= 2 && reinterpret_cast<uintptr_t>(keys) % 16 == 0) {
       const int32_t hot = lb < 256 ? lb : 256;
       auto pol = Kokkos::TeamPolicy<Kokkos::MI355X>(cus, 1024)
                      .set_scratch_size(0, Kokkos::PerTeam(HistWide::scratch_bytes(lb, hot)));
@@ -997,6 +1138,17 @@ int kkmx_set_mdrange_depth(int depth) {
   return 0;
 }
 
+int kkmx_range_candidate(int pattern, int candidate, int64_t n, int* block, int* items, int* max_blocks) {
+  if (pattern < 0 || pattern > 2 || candidate < 0 || candidate >= Kokkos::Impl::kRangeCandidates || n < 0 ||
+      !block || !items || !max_blocks)
+    return reject("kkmx_range_candidate: bad arguments");
+  return guarded([&] {
+    const auto c = Kokkos::Impl::range_candidate(pattern, candidate, n);
+    *block       = c.block;
+    *items       = c.items;
+    *max_blocks  = c.max_blocks;
+  });
+}
 int kkmx_set_range_config(int pattern, int block, int items, int max_blocks) {
   if (pattern < 0 || pattern > 2) return reject("kkmx_set_range_config: pattern must be 0, 1 or 2");
   auto& o = Kokkos::Impl::range_overrides();

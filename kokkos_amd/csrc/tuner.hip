@@ -249,6 +249,9 @@ struct ToolState {
   DevBuf* dev_params = nullptr;
   mx_stream_t stream = nullptr;
   uint64_t requests = 0, answered = 0, inferences = 0;
+  // The last request answered: variable (0-3), the runtime's default value
+  // (the heuristic's candidate) and the value the model gave.
+  int last_var = -1, last_default = -1, last_value = -1;
 };
 ToolState& tool() {
   static ToolState s;
@@ -474,6 +477,19 @@ void kkmx_tuner_stats(uint64_t* requests, uint64_t* answered, uint64_t* inferenc
   if (inferences) *inferences = s.inferences;
 }
 
+// The last answered request: tuning variable (0 range launch, 1 team shape,
+// 2 MDRange tile, 3 batch tile shape), the heuristic's candidate the runtime
+// offered as default, and the candidate the model chose.  Returns 1 if no
+// request has been answered yet.
+int kkmx_tuner_last_answer(int* variable, int* heuristic, int* chosen) {
+  auto& s = tool();
+  std::lock_guard<std::mutex> lk(s.mtx);
+  if (variable) *variable = s.last_var;
+  if (heuristic) *heuristic = s.last_default;
+  if (chosen) *chosen = s.last_value;
+  return s.last_var < 0 ? 1 : 0;
+}
+
 // ---------------------------------------------------------------- Kokkos Tools callbacks
 void kokkosp_init_library(const int, const uint64_t, const uint32_t, Kokkos_Profiling_KokkosPDeviceInfo*) {
   auto& s = tool();
@@ -566,6 +582,9 @@ void kokkosp_request_values(const size_t, const size_t n_in, const Kokkos_Tools_
       s.cache[key] = c;
     }
     if (c >= 0 && c < s.ncand[var]) {
+      s.last_var             = var;
+      s.last_default         = int(out[o].value.int_value);
+      s.last_value           = c;
       out[o].value.int_value = c;
       ++s.answered;
     }

@@ -169,11 +169,13 @@ KOKKOS_INLINE_FUNCTION T fetch_oper(T* p, const T& val, Op op) {
     uint64_t pending    = __builtin_amdgcn_ballot_w64(true);
     while (pending) {
       if (lane == unsigned(__builtin_ctzll(pending))) {
-        while (__hip_atomic_exchange(lock, 1, __ATOMIC_ACQUIRE, S) != 0) {
+        // The lock array is shared by every workgroup, so the lock itself is
+        // taken at agent scope whatever scope the caller asked for.
+        while (__hip_atomic_exchange(lock, 1, __ATOMIC_ACQUIRE, kScopeAgent) != 0) {
         }
         result = *p;
         *p     = op(result, val);
-        __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, S);
+        __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, kScopeAgent);
       }
       pending &= pending - 1;
     }

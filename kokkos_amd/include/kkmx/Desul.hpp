@@ -7,9 +7,10 @@
 //
 // On MI355X every call lands on the scoped operations of kkmx/Atomic.hpp:
 //   MemoryScopeCaller        plain loads and stores (no other thread races)
-//   MemoryScopeCore          agent scope (a CU-local scope would be weaker
-//                            than what a workgroup spread over the chip needs;
-//                            agent scope is always correct)
+//   MemoryScopeCore          workgroup scope, as the reference maps it
+//                            (tpls/desul/include/desul/atomics/Adapt_HIP.hpp:55):
+//                            only threads of one workgroup (one CU) touch the
+//                            address, so the atomic resolves in the XCD's L2
 //   MemoryScopeDevice        agent scope: native L2 atomics across the 8 XCDs
 //   MemoryScopeNode/System   system scope (host-coherent memory, peer GPUs)
 // Memory orders other than relaxed add fences of the same scope around the
@@ -51,7 +52,7 @@ struct hip_scope<MemoryScopeCaller> {
 };
 template <>
 struct hip_scope<MemoryScopeCore> {
-  static constexpr int value = Kokkos::Impl::kScopeAgent;
+  static constexpr int value = Kokkos::Impl::kScopeWorkgroup;
 };
 template <>
 struct hip_scope<MemoryScopeDevice> {

@@ -469,25 +469,21 @@ __global__ __launch_bounds__(1024) void k_range_scan(
 // cache.  Lane l of wavefront w holds index base + r*64 for r < IPT (one
 // contiguous 64-index run per round).  Used while IPT values fit in about
 // 256 bytes of registers per lane; larger IPT takes the two-pass tile above.
+// One tile of the register-resident scan, run by the whole workgroup: pass 1
+// with the values kept in registers, aggregate published, look-back,
+// inclusive prefix published, final calls.  The thread that runs tile
+// ntiles-1 writes the total.  Uses s_wave, s_red and s_pos; the caller puts a
+// barrier between two calls.
 template <class CR, class Index, int IPT, bool WithTotal>
-__global__ __launch_bounds__(1024) void k_range_scan_rr(
-    const CR cr, const int64_t begin, const int64_t end, const int64_t ntiles,
-    ScanTileState<typename CR::value_type> st, unsigned* control,
-    typename CR::value_type* total_out) {
-  using V   = typename CR::value_type;
-  using Tag = typename CR::work_tag;
-  __shared__ LdsSlots16<V> s_wave;
-  __shared__ LdsSlots16<V> s_red;
-  __shared__ uint64_t s_claim;
-  __shared__ int s_pos;
-  const auto join      = [&cr](V& a, const V& b) { cr.join(a, b); };
-  uint64_t* claim_word = reinterpret_cast<uint64_t*>(control + 2);
-  if (threadIdx.x == 0)
-    s_claim = __hip_atomic_fetch_add(claim_word, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const uint64_t claim = s_claim;
-  const int64_t tile   = static_cast<int64_t>(static_cast<uint32_t>(claim));
-  const uint32_t epoch = static_cast<uint32_t>(claim >> 32);
+__device__ __forceinline__ void scan_tile_rr(const CR& cr, const int64_t begin, const int64_t end,
+                                             const int64_t ntiles, const ScanTileState<typename CR::value_type>& st,
+                                             const int64_t tile, const uint32_t epoch,
+                                             typename CR::value_type* total_out,
+                                             typename CR::value_type* s_wave, typename CR::value_type* s_red,
+                                             int* s_pos) {
+  using V         = typename CR::value_type;
+  using Tag       = typename CR::work_tag;
+  const auto join = [&cr](V& a, const V& b) { cr.join(a, b); };
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6, l = wave::lane();
   V identity;
   cr.init(identity);
@@ -519,9 +515,9 @@ __global__ __launch_bounds__(1024) void k_range_scan_rr(
     inc[r] = wave::inclusive_scan(v[r], identity, join);
     join(wave_total, wave::readlane(inc[r], 63));
   }
-  if (l == 0) s_wave.get()[w] = wave_total;
+  if (l == 0) s_wave[w] = wave_total;
   __syncthreads();
-  const V x    = (l < nw) ? s_wave.get()[l] : identity;
+  const V x    = (l < nw) ? s_wave[l] : identity;
   const V xs   = wave::inclusive_scan(x, identity, join);
   const V tagg = wave::readlane(xs, nw - 1);
   const V wpre = (w == 0) ? identity : wave::readlane(xs, w - 1);
@@ -531,21 +527,19 @@ __global__ __launch_bounds__(1024) void k_range_scan_rr(
     if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanIncl), tagg);
   } else {
     if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanAgg), tagg);
-    prefix = scan_lookback(st, tile, epoch, identity, join, s_red.get(), &s_pos);
+    prefix = scan_lookback(st, tile, epoch, identity, join, s_red, s_pos);
     if (threadIdx.x == 0) {
       V t = prefix;
       join(t, tagg);
       st.publish(tile, scan_flag(epoch, kScanIncl), t);
     }
   }
-  if (threadIdx.x == 0 && tile == ntiles - 1) {
-    if constexpr (WithTotal) {
+  if constexpr (WithTotal) {
+    if (threadIdx.x == 0 && tile == ntiles - 1) {
       V tot = prefix;
       join(tot, tagg);
       *total_out = tot;
     }
-    __hip_atomic_store(claim_word, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
   }
   V running = prefix;
   join(running, wpre);
@@ -567,6 +561,35 @@ __global__ __launch_bounds__(1024) void k_range_scan_rr(
       if (i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), u, true);
     }
   }
+}
+
+// One tile per workgroup, claimed in launch order from control[2..3] (one
+// 64-bit word {tile counter (low), epoch (high)}: a single relaxed fetch_add
+// returns a consistent pair, so the claim needs no fence).  The workgroup of
+// the last tile returns the word to {0, epoch + 1}: every workgroup has
+// claimed by then.
+template <class CR, class Index, int IPT, bool WithTotal>
+__global__ __launch_bounds__(1024) void k_range_scan_rr(
+    const CR cr, const int64_t begin, const int64_t end, const int64_t ntiles,
+    ScanTileState<typename CR::value_type> st, unsigned* control,
+    typename CR::value_type* total_out) {
+  using V = typename CR::value_type;
+  __shared__ LdsSlots16<V> s_wave;
+  __shared__ LdsSlots16<V> s_red;
+  __shared__ uint64_t s_claim;
+  __shared__ int s_pos;
+  uint64_t* claim_word = reinterpret_cast<uint64_t*>(control + 2);
+  if (threadIdx.x == 0)
+    s_claim = __hip_atomic_fetch_add(claim_word, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const uint64_t claim = s_claim;
+  const int64_t tile   = static_cast<int64_t>(static_cast<uint32_t>(claim));
+  const uint32_t epoch = static_cast<uint32_t>(claim >> 32);
+  if (threadIdx.x == 0 && tile == ntiles - 1)
+    __hip_atomic_store(claim_word, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  scan_tile_rr<CR, Index, IPT, WithTotal>(cr, begin, end, ntiles, st, tile, epoch, total_out, s_wave.get(),
+                                          s_red.get(), &s_pos);
 }
 
 // ------------------------------------------------------------------ host helpers
@@ -1012,8 +1035,8 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
   int64_t bb = b, ee = e, nt = ntiles;
   unsigned* control = inst->control;
   const void* k     = range_scan_kernel<CR, Index, WithTotal>(ipt);
-  mx_launch_kernel(k, static_cast<unsigned>(ntiles), c.block, 0, inst->stream, cr, bb, ee, nt, st,
-                   control, tot_dst);
+  mx_launch_kernel(k, static_cast<unsigned>(ntiles), c.block, 0, inst->stream, cr, bb, ee, nt, st, control,
+                   tot_dst);
   if (copy_back) {
     KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
     std::memcpy(static_cast<void*>(total), tot_dst, sizeof(V));

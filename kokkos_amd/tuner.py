@@ -50,6 +50,7 @@ def _libs():
     T.kkmx_tuner_load_model.argtypes = [C.c_char_p]
     T.kkmx_tuner_stats.argtypes = [C.POINTER(C.c_uint64)] * 3
     T.kkmx_tuner_last_error.restype = C.c_char_p
+    T.kkmx_tuner_last_answer.argtypes = [C.POINTER(C.c_int)] * 3
     return S, T
 
 
@@ -217,6 +218,16 @@ def load_into_tool(path: Path = MODEL_PATH) -> None:
     _, T = _libs()
     if T.kkmx_tuner_load_model(str(path).encode()) != 0:
         raise KkmxError(T.kkmx_tuner_last_error().decode())
+
+
+def last_answer():
+    """(variable, heuristic's candidate, model's candidate) of the last
+    tuning request the tool answered, or None."""
+    _, T = _libs()
+    v, h, c = C.c_int(), C.c_int(), C.c_int()
+    if T.kkmx_tuner_last_answer(C.byref(v), C.byref(h), C.byref(c)) != 0:
+        return None
+    return v.value, h.value, c.value
 
 
 def tool_stats():

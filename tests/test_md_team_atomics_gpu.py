@@ -156,7 +156,7 @@ def test_zipf_keys_match_oracle(paths, oracle, n):
 
 
 @pytest.mark.parametrize("mode,lds_bins", [(0, 0), (1, 32768), (1, 1024), (1, 65536 // 2 + 7), (2, 32768),
-                                           (2, 100), (2, 65536 // 2 + 7)])
+                                           (2, 100), (2, 65536 // 2 + 7), (3, 32768), (3, 100), (3, 65536)])
 def test_histogram_bit_exact(paths, oracle, mode, lds_bins):
     P = paths
     n, nb = 1 << 22, 65536
@@ -179,7 +179,7 @@ def test_histogram_full_size_config4(paths, oracle):
     keys = oracle.zipf_keys(n, 1.1, nb, 8)
     assert oracle.checksum(Kb.download(np.int32, n)) == oracle.checksum(keys)
     ref = oracle.histogram(keys, nb)
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         K.load("kkmx_rt").mx_memset_async(Bb.ptr, 0, nb * 4, P.kkmx_stream())
         run(P, P.kkmx_histogram(Kb.ptr, n, Bb.ptr, nb, mode, 32768))
         assert np.array_equal(Bb.download(np.int32, nb), ref), f"mode {mode}"
@@ -187,18 +187,34 @@ def test_histogram_full_size_config4(paths, oracle):
 
 @pytest.mark.parametrize("n", [0, 1, 3, 4, 5, 1023, 4097, 1 << 16])
 def test_histogram_wide_ragged(paths, oracle, n):
-    """Mode 2 on extents that are not multiples of 4 keys or of a team's
+    """Modes 2 and 3 on extents that are not multiples of 4 keys or of a team's
     share, and on keys at an offset of 4 bytes (not 16-byte aligned, so the
     entry point runs mode 1)."""
     P = paths
     nb = 300
     keys = oracle.fill_uniform_i32(n + 1, 81, 0, nb - 1)
     Kb, Bb = dev(keys), K.DeviceBuffer(nb * 4)
-    for off in (0, 1):
+    for mode in (2, 3):
+        for off in (0, 1):
+            K.load("kkmx_rt").mx_memset_async(Bb.ptr, 0, nb * 4, P.kkmx_stream())
+            run(P, P.kkmx_histogram(Kb.ptr + 4 * off, n, Bb.ptr, nb, mode, 128))
+            got = Bb.download(np.int32, nb)
+            ref = oracle.histogram(np.ascontiguousarray(keys[off:off + n]), nb)
+            assert np.array_equal(got, ref), (mode, off)
+
+
+def test_histogram_rows_adversarial_keys(paths, oracle):
+    """Mode 3 with every key in one bin above the LDS range (one team-row
+    counter takes a whole team's share) and with all keys in bin 0."""
+    P = paths
+    n, nb = 1 << 22, 65536
+    Bb = K.DeviceBuffer(nb * 4)
+    for key in (nb - 1, 0):
+        keys = np.full(n, key, np.int32)
+        Kb = dev(keys)
         K.load("kkmx_rt").mx_memset_async(Bb.ptr, 0, nb * 4, P.kkmx_stream())
-        run(P, P.kkmx_histogram(Kb.ptr + 4 * off, n, Bb.ptr, nb, 2, 128))
-        got = Bb.download(np.int32, nb)
-        assert np.array_equal(got, oracle.histogram(np.ascontiguousarray(keys[off:off + n]), nb)), off
+        run(P, P.kkmx_histogram(Kb.ptr, n, Bb.ptr, nb, 3, 32768))
+        assert np.array_equal(Bb.download(np.int32, nb), oracle.histogram(keys, nb)), key
 
 
 # ---------------------------------------------------------------- FP64 scatter
