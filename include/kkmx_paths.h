@@ -109,6 +109,15 @@ int kkmx_reduce_minmaxsum_i64(const int64_t* x_dev, int64_t n, int64_t* out_dev)
 int kkmx_scan_excl_i32_i64(const int32_t* in_dev, int64_t* out_dev, int64_t n, int64_t* total_dev);
 /* Inclusive prefix sum of int32 input in int32. */
 int kkmx_scan_incl_i32(const int32_t* in_dev, int32_t* out_dev, int64_t n);
+/* kkmx_scan_incl_i32 without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Its scratch (a 256-byte control block and
+ * one 8-byte status word per tile of 32768 elements) is a fixed device array
+ * of 512 KiB + 256 B per GPU, held until the process ends; n above 2^31 is
+ * rejected (status -1).  Call kkmx_init first.  Each call clears the part of
+ * the array it uses, so the previous call's contents do not matter; calls
+ * that overlap on one GPU share the array and race. */
+int kkmx_scan_incl_i32_noalloc(const int32_t* in_dev, int32_t* out_dev, int64_t n);
 int kkmx_scan_excl_i64(const int64_t* in_dev, int64_t* out_dev, int64_t n, int64_t* total_dev);
 int kkmx_scan_excl_f64(const double* in_dev, double* out_dev, int64_t n, double* total_dev);
 

@@ -108,6 +108,7 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_reduce_minmaxsum_i64", [vp, i64, vp])
         _sig(lib, "kkmx_scan_excl_i32_i64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_scan_incl_i32", [vp, vp, i64])
+        _sig(lib, "kkmx_scan_incl_i32_noalloc", [vp, vp, i64])
         _sig(lib, "kkmx_scan_excl_i64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_scan_excl_f64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_set_range_config", [C.c_int, C.c_int, C.c_int, C.c_int])

@@ -653,12 +653,63 @@ void sort_by_key_typed(void* keys, void* values, int64_t n, int value_bytes, int
 }
 }  // namespace
 
+// ---------------------------------------------------------------- no-allocation scan
+// kkmx_scan_incl_i32_noalloc is kkmx_scan_incl_i32 for callers that cannot
+// allocate (graph capture, tight serving loops).  The scan's scratch -- a
+// 256-byte control block holding the tile-claim word, then one 8-byte status
+// word per tile -- lives in the fixed device array `scratch` below instead of
+// the instance's grow-only buffers.  Each GPU holds its own copy of the array
+// until the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (kkmx_init looks it up,
+// or the first call on a device).  Every call clears the control block and
+// the status words it will use with a small kernel (no memset, so the call
+// may be captured into a graph) and then runs the same kernel as the
+// instance path for extents of 2^22 and more (1024 lanes x 32 values per
+// tile).  Calls that overlap on one GPU share the array and race.
+namespace {
+constexpr int64_t kNoAllocTileElems   = 1024 * 32;
+constexpr int64_t kNoAllocMaxTiles    = int64_t(1) << 16;  // 2^31 elements
+constexpr size_t kNoAllocControlBytes = 256;
+constexpr size_t kNoAllocScratchBytes = kNoAllocControlBytes + size_t(kNoAllocMaxTiles) * 8;
+constexpr int kNoAllocMaxDevices      = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mtx;
+unsigned char* g_scratch_dev[kNoAllocMaxDevices] = {};
+
+// This device's address of `scratch`; call with the device current.
+int noalloc_scratch_setup(int dev) {
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return reject("kkmx_scan_incl_i32_noalloc: device id out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mtx);
+  if (g_scratch_dev[dev]) return 0;
+  int cur = -1;
+  if (hipGetDevice(&cur) != hipSuccess || cur != dev)
+    return reject("kkmx_scan_incl_i32_noalloc: the Kokkos device is not current on this thread");
+  void* p              = nullptr;
+  const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (err != hipSuccess || p == nullptr) {
+    g_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(err);
+    return 1;
+  }
+  g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+  return 0;
+}
+
+__global__ __launch_bounds__(256) void k_noalloc_clear(uint64_t* words, int64_t nwords) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < nwords) words[i] = 0;
+}
+}  // namespace
+
 extern "C" {
 
 int kkmx_init(int device) {
   try {
     if (!Kokkos::is_initialized()) Kokkos::initialize(Kokkos::InitializationSettings().set_device_id(device));
-    return 0;
+    return noalloc_scratch_setup(Kokkos::Impl::mi355x_globals().props.device);
   } catch (const std::exception& e) {
     g_err = e.what();
     return 1;
@@ -909,6 +960,39 @@ int kkmx_scan_incl_i32(const int32_t* in, int32_t* out, int64_t n) {
     Kokkos::parallel_scan("kkmx::scan_incl_i32", Range64(0, n),
                           InclusiveScan<int32_t>{DevView<const int32_t>(in, n), DevView<int32_t>(out, n)});
   });
+}
+int kkmx_scan_incl_i32_noalloc(const int32_t* in, int32_t* out, int64_t n) {
+  if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_scan_incl_i32_noalloc: bad arguments");
+  if (n > kNoAllocMaxTiles * kNoAllocTileElems)
+    return reject("kkmx_scan_incl_i32_noalloc: n exceeds 2^31 elements, the most its fixed scratch "
+                  "(65536 tile-status words) covers");
+  if (!Kokkos::is_initialized()) return reject("kkmx_scan_incl_i32_noalloc: call kkmx_init first");
+  if (n == 0) return 0;
+  try {
+    const int dev = Kokkos::Impl::mi355x_globals().props.device;
+    if (int rc = noalloc_scratch_setup(dev)) return rc;
+    unsigned char* base = g_scratch_dev[dev];
+    using F             = InclusiveScan<int32_t>;
+    using CR            = Kokkos::Impl::ScanTypes<Range64, F, void>::CR;
+    CR cr(F{DevView<const int32_t>(in, n), DevView<int32_t>(out, n)}, typename CR::reducer_store{}, 1);
+    int64_t begin = 0, end = n, ntiles = (n + kNoAllocTileElems - 1) / kNoAllocTileElems;
+    int64_t nwords = int64_t(kNoAllocControlBytes / 8) + ntiles;
+    uint64_t* words = reinterpret_cast<uint64_t*>(base);
+    mx_stream_t stream = Kokkos::MI355X().stream();
+    Kokkos::Impl::mx_launch_kernel(reinterpret_cast<const void*>(&k_noalloc_clear),
+                                   static_cast<unsigned>((nwords + 255) / 256), 256, 0, stream, words, nwords);
+    Kokkos::Impl::ScanTileState<int32_t> st;
+    st.bind(base + kNoAllocControlBytes, nullptr);
+    unsigned* control = reinterpret_cast<unsigned*>(base);
+    int32_t* total    = nullptr;
+    const void* k     = Kokkos::Impl::range_scan_kernel<CR, int64_t, false>(32);
+    Kokkos::Impl::mx_launch_kernel(k, static_cast<unsigned>(ntiles), 1024, 0, stream, cr, begin, end, ntiles, st,
+                                   control, total);
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
 }
 int kkmx_scan_excl_i64(const int64_t* in, int64_t* out, int64_t n, int64_t* total) {
   if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_scan_excl_i64: bad arguments");
