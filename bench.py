@@ -399,8 +399,8 @@ def leg_atomics(K, P, timer):
         "gups_2^30_table_2^28_updates": {"ms": round(g[0], 3), "gups": round(U / g[0] / 1e6, 3)},
         "histogram_zipf_2^28_keys": {"ms": round(h[0], 3), "gbps": round(4 * nk / h[0] / 1e6, 1),
                                      "keys_per_s": round(nk / h[0] * 1e3, 1),
-                                     "kernel": "LDS bins with 256 hot bins replicated 16x, 16-byte key loads, "
-                                               "team rows summed by a second kernel (no device-scope atomics)"},
+                                     "kernel": "LDS bins with the 256 hottest replicated 16x, 16-byte key loads, team "
+                                               "rows summed by a second kernel (no device-scope atomics)"},
         "histogram_zipf_2^28_keys_global_atomic_flush": {"ms": round(h2[0], 3), "gbps": round(4 * nk / h2[0] / 1e6, 1)},
         "histogram_zipf_2^28_keys_one_key_per_load": {"ms": round(h1[0], 3), "gbps": round(4 * nk / h1[0] / 1e6, 1)},
         "scatter_f64_2^26_into_2^20": {"ms": round(sc[0], 3), "updates_per_s": round(ns / sc[0] * 1e3, 1),
@@ -609,6 +609,8 @@ def main():
     ap.add_argument("--only", choices=["small_reduce_scan", "mdrange_team", "atomics", "deep_copy", "std_algorithms",
                                        "sort", "batched"],
                     help="run one secondary workload alone and print its JSON (for profiling); no headline line")
+    ap.add_argument("--scan-shape", type=int, nargs=2, metavar=("BLOCK", "ITEMS"),
+                    help="time only the 2^28 int32 inclusive scan under this launch shape (for profiling); no headline line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -623,6 +625,14 @@ def main():
     import kokkos_amd as K
 
     P = K.init(0)
+    if args.scan_shape:
+        S, O = K.DeviceBuffer(N * 4), K.DeviceBuffer(N * 4)
+        K.check(P.kkmx_fill_uniform_i32(S.ptr, N, 4, 0, 7), P)
+        K.check(P.kkmx_set_range_config(2, args.scan_shape[0], args.scan_shape[1], 0), P)
+        t = _timeit(K, K.EventTimer(), lambda: K.check(P.kkmx_scan_incl_i32(S.ptr, O.ptr, N), P), reps=3, warm=1)
+        print(json.dumps({"scan_incl_i32_2^28": {"block": args.scan_shape[0], "items": args.scan_shape[1],
+                                                 "ms": round(t[0], 4), "gbps": round(8 * N / t[0] / 1e6, 1)}}))
+        return
     if args.only:
         legs = {"small_reduce_scan": leg_small, "mdrange_team": leg_md_team, "atomics": leg_atomics,
                 "deep_copy": leg_deep_copy, "std_algorithms": leg_std_algorithms, "sort": leg_sort,

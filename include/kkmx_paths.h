@@ -153,9 +153,8 @@ int kkmx_spmv_csr(const int64_t* rowptr_dev, const int32_t* col_dev, const doubl
  * kkmx_histogram: mode 0 = global atomics, 1 = bins [0, lds_bins) privatised
  *   in team scratch (LDS atomics), the rest global; 2 = as 1 with 16-byte key
  *   loads and the lowest 256 bins replicated 16 times in LDS (keys 16-byte
- *   aligned; otherwise mode 1 runs); 3 = 16-byte key loads, the lowest 128
- *   bins replicated 64 times (one copy per lane of a wavefront), bins at or
- *   above lds_bins counted in a per-team row of a workspace and the team's
+ *   aligned; otherwise mode 1 runs); 3 = as 2 for the LDS part, with bins at
+ *   or above lds_bins counted in a per-team row of a workspace and the team's
  *   LDS counts stored into the same row, the rows then summed into bins by a
  *   second kernel (no device-scope atomics; the workspace, teams x nbins
  *   int32, is kept between calls and released by kkmx_finalize).  Modes 2

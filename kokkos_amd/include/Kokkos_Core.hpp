@@ -36,6 +36,7 @@
 #include <kkmx/LocalDeepCopy.hpp>
 #include <kkmx/KokkosMalloc.hpp>
 #include <kkmx/Graph.hpp>
+#include <kkmx/Tuners.hpp>
 #include <kkmx/SpacesImpl.hpp>
 
 namespace Kokkos {

@@ -324,11 +324,23 @@ class RangePolicy : public Impl::PolicyTraitsWithDefaults<Properties...> {
   }
   template <class... OtherProps>
   RangePolicy(const RangePolicy<OtherProps...>& o)
-      : m_space(o.space()), m_begin(o.begin()), m_end(o.end()), m_chunk(o.chunk_size()) {}
+      : m_space(o.space()), m_begin(o.begin()), m_end(o.end()), m_chunk(o.chunk_size()),
+        m_occupancy(o.impl_get_desired_occupancy().value) {}
+  // Desired occupancy in percent, as RangePolicyOccupancyTuner sets it.  The
+  // MI355X launches size their grids from the tile count, so the value is
+  // kept for the tuner's bookkeeping and does not change the launch.
+  RangePolicy& impl_set_desired_occupancy(const Experimental::DesiredOccupancy& o) {
+    m_occupancy = o.value;
+    return *this;
+  }
+  Experimental::DesiredOccupancy impl_get_desired_occupancy() const {
+    return Experimental::DesiredOccupancy{m_occupancy};
+  }
 
  private:
   execution_space m_space;
   member_type m_begin, m_end, m_chunk;
+  int m_occupancy = 100;
 
   void check_bounds() {
     if (m_end < m_begin) {

@@ -328,6 +328,24 @@ class MI355X {
   uint32_t impl_instance_id() const noexcept { return m_inst->id; }
   static int impl_cu_count() { return Impl::mi355x_globals().props.cu_count; }
   static const mx_device_props& impl_props() { return Impl::mi355x_globals().props; }
+  // Slot of the calling lane among all lanes the GPU can hold at once: the
+  // wavefront's hardware position (XCD from the XCC_ID register; shader
+  // engine, shader array, CU, SIMD and wave slot from HW_ID) times 64 plus
+  // the lane.  Unique among lanes that run at the same time and below
+  // impl_max_hardware_threads(); 0 on the host.
+  KOKKOS_INLINE_FUNCTION static unsigned impl_hardware_thread_id() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned hw   = __builtin_amdgcn_s_getreg((31 << 11) | 4);         // HW_ID, bits 31:0
+    const unsigned xcc  = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15u;  // XCC_ID, bits 3:0
+    const unsigned wave = hw & 15u, simd = (hw >> 4) & 3u, cu = (hw >> 8) & 15u, sh = (hw >> 12) & 1u,
+                   se = (hw >> 13) & 7u;
+    const unsigned slot = ((((xcc * 8u + se) * 2u + sh) * 16u + cu) * 4u + simd) * 16u + wave;
+    return slot * 64u + static_cast<unsigned>(__lane_id());
+#else
+    return 0;
+#endif
+  }
+  KOKKOS_INLINE_FUNCTION static unsigned impl_max_hardware_threads() { return 16u * 8u * 2u * 16u * 4u * 16u * 64u; }
   bool in_parallel() const { return false; }
 
  private:

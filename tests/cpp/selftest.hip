@@ -3159,6 +3159,95 @@ KKMX_TEST(team_combined_reducers) {
 
 // ---------------------------------------------------------------- C-ABI
 // =========================================================================
+// Built-in tuners (core/unit_test/tools/TestBuiltinTuners.cpp:29-118,
+// TestCategoricalTuner.cpp): the sparse problem maps coordinates in [0, 1)
+// to points; with no tuning tool loaded begin() returns the first point and
+// tune() leaves the policy as it was.
+// =========================================================================
+struct TunerTeamF {
+  KOKKOS_INLINE_FUNCTION void operator()(const Kokkos::TeamPolicy<Exec>::member_type&) const {}
+};
+struct TunerMDF {
+  KOKKOS_INLINE_FUNCTION void operator()(const int, const int) const {}
+};
+KKMX_TEST(builtin_tuners_points_and_defaults) {
+  namespace KTE = Kokkos::Tools::Experimental;
+  Kokkos::TeamPolicy<Exec> teamp(1, Kokkos::AUTO, Kokkos::AUTO);
+  Kokkos::MDRangePolicy<Exec, Kokkos::Rank<2>> mdp({0, 0}, {1, 1});
+  KTE::TeamSizeTuner team_tuner("team_tuner", teamp, TunerTeamF{}, Kokkos::ParallelForTag{},
+                                KTE::Impl::Impl::SimpleTeamSizeCalculator{});
+  KTE::MDRangeTuner<2> md_tuner("md_tuner", mdp, TunerMDF{}, Kokkos::ParallelForTag{},
+                                KTE::Impl::Impl::SimpleTeamSizeCalculator{});
+  std::vector<int> options{1, 2, 3, 4, 5};
+  auto team_opts = team_tuner.combine("options", options);
+  auto md_opts   = md_tuner.combine("options", options);
+  CHECK_EQ(std::get<0>(team_opts.get_point(0.0, 0.0, 0.0)), 1);
+  CHECK_EQ(std::get<0>(team_opts.get_point(0.9, 0.0, 0.0)), 5);
+  CHECK_EQ(std::get<0>(team_opts.get_point(0.45, 0.0, 0.0)), 3);
+  // Vector lengths are listed in increasing key order: 0.0 picks the
+  // smallest, 0.999 the largest (64); team sizes at that length count down
+  // from the maximum.
+  const auto low  = team_tuner.get_point(0.0, 0.0);
+  const auto high = team_tuner.get_point(0.999, 0.0);
+  Kokkos::TeamPolicy<Exec> p1(teamp), p64(teamp);
+  p1.impl_set_vector_length(1);
+  p64.impl_set_vector_length(64);
+  CHECK_EQ(std::get<0>(low), int64_t(1));
+  CHECK_EQ(std::get<1>(low), int64_t(p1.team_size_max(TunerTeamF{}, Kokkos::ParallelForTag{})));
+  CHECK_EQ(std::get<0>(high), int64_t(64));
+  CHECK_EQ(std::get<1>(high), int64_t(p64.team_size_max(TunerTeamF{}, Kokkos::ParallelForTag{})));
+  // MD tiles: powers of two whose product stays below max_tile_size_product.
+  const auto t0 = md_tuner.get_point(0.0, 0.0);
+  CHECK_EQ(std::get<0>(t0), 1);
+  CHECK_EQ(std::get<1>(t0), 1);
+  const auto t1 = md_tuner.get_point(0.999, 0.999);
+  CHECK(int64_t(std::get<0>(t1)) * std::get<1>(t1) < 1024);
+  // No tool: begin() answers the first point, tune() keeps the policy.
+  auto cfg = md_opts.begin();
+  CHECK_EQ(std::get<0>(cfg), 1);
+  md_opts.end();
+  auto tuned = md_tuner.tune(mdp);
+  md_tuner.end();
+  Kokkos::parallel_for(tuned, TunerMDF{});
+  auto cat = KTE::make_categorical_tuner("cat", std::vector<std::string>{"a", "b", "c"});
+  CHECK(cat.begin() == "a");
+  cat.end();
+  KTE::SingleDimensionalRangeTuner<int64_t> r("rng", KTE::StatisticalCategory::kokkos_value_ratio, int64_t(42),
+                                              int64_t(5), int64_t(100), int64_t(5));
+  CHECK_EQ(r.begin(), int64_t(42));
+  r.end();
+}
+
+// impl_hardware_thread_id: every lane's slot is below
+// impl_max_hardware_threads, the lane field is the lane, and a launch that
+// fills the GPU touches all 8 XCDs and many wave slots.
+KKMX_TEST(hardware_thread_ids) {
+  const int64_t n = int64_t(1) << 22;
+  Kokkos::View<unsigned*, Exec> ids("ids", n);
+  Kokkos::View<unsigned*, Exec> lanes("lanes", n);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(0, n), KOKKOS_LAMBDA(const int64_t i) {
+    ids(i)   = Exec::impl_hardware_thread_id();
+    lanes(i) = static_cast<unsigned>(__lane_id());
+  });
+  auto h  = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), ids);
+  auto hl = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), lanes);
+  const unsigned mx = Exec::impl_max_hardware_threads();
+  std::vector<char> xcd(16, 0);
+  std::vector<char> seen(mx / 64, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    if (h(i) >= mx) CHECK(h(i) < mx);
+    if (h(i) % 64 != hl(i)) CHECK_EQ(h(i) % 64, hl(i));
+    seen[h(i) / 64] = 1;
+    xcd[(h(i) / 64) >> 14] = 1;  // slot = xcc * 2^14 + (se, sh, cu, simd, wave)
+  }
+  int nx = 0, nslots = 0;
+  for (char c : xcd) nx += c;
+  for (char c : seen) nslots += c;
+  CHECK_EQ(nx, 8);
+  CHECK(nslots >= 1024);
+}
+
+// =========================================================================
 // Backend registration (core/src/impl/Kokkos_ExecSpaceManager.hpp:157-163):
 // a host execution space registered from a namespace-scope initialiser is
 // initialised by Kokkos::initialize (when the registration came first),

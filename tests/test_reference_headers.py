@@ -41,7 +41,8 @@ TestJoinBackwardCompatibility.hpp TestLocalDeepCopy.hpp TestMDRange.hpp TestMDRa
 TestMDRangeReduce.hpp TestMDRange_a.hpp TestMDRange_b.hpp TestMDRange_c.hpp TestMDRange_d.hpp
 TestMDRange_e.hpp TestMDRange_f.hpp TestMDRange_g.hpp TestMDSpan.hpp TestMathematicalConstants.hpp
 TestMultiGPU.hpp TestNonTrivialScalarTypes.hpp TestOccupancyControlTrait.hpp TestParallelScanRangePolicy.hpp
-TestPrintf.hpp TestQuadPrecisionMath.hpp TestRangePolicyConstructors.hpp TestRealloc.hpp TestReduce.hpp
+TestPrintf.hpp TestQuadPrecisionMath.hpp TestRange.hpp TestRangePolicy.hpp TestRangePolicyConstructors.hpp
+TestRangePolicyRequire.hpp TestRealloc.hpp TestReduce.hpp
 TestReduceCombinatorical.hpp TestReductions.hpp TestReductions_DeviceView.hpp TestResize.hpp TestSwap.hpp
 TestTaskScheduler.hpp TestTeam.hpp TestTeamBasic.hpp TestTeamCombinedReducers.hpp TestTeamMDRange.hpp
 TestTeamPolicyConstructors.hpp TestTeamReductionScan.hpp TestTeamScan.hpp TestTeamScratch.hpp
@@ -51,6 +52,10 @@ TestViewCopy_c.hpp TestViewCtorDimMatch.hpp TestViewEmptyRuntimeUnmanaged.hpp
 TestViewLayoutStrideAssignment.hpp TestViewMemoryAccessViolation.hpp TestViewOfClass.hpp
 TestViewSpaceAssign.hpp TestView_64bit.hpp Test_InterOp_Streams.hpp
 """.split())
+
+# Whole test programs of core/unit_test/tools/ that compile unchanged (the
+# built-in tuners and the tuning interface).
+TOOLS_PASS = ["TestBuiltinTuners.cpp", "TestCategoricalTuner.cpp", "TestTuning.cpp"]
 
 SOURCE = """#include <gtest/gtest.h>
 #define TEST_CATEGORY mi355x
@@ -78,6 +83,18 @@ def compile_header(header: str) -> tuple[str, bool, str]:
         return header, r.returncode == 0, first
 
 
+def compile_tools_program(name: str) -> tuple[str, bool, str]:
+    cmd = [
+        HIPCC, "-std=c++17", "-fsyntax-only", "--offload-arch=gfx950", "-x", "hip",
+        "-I", str(ROOT / "kokkos_amd" / "include"), "-I", str(ROOT / "include"),
+        "-I", str(UNIT), "-I", str(REF / "tpls" / "gtest"),
+        "-Wno-unused-command-line-argument", "-ferror-limit=3", str(UNIT / "tools" / name),
+    ]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    first = next((ln for ln in (r.stdout + r.stderr).splitlines() if "error:" in ln), "")
+    return name, r.returncode == 0, first
+
+
 def _sweep(headers, jobs):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         return list(ex.map(compile_header, headers))
@@ -91,6 +108,14 @@ def test_pinned_reference_headers_compile():
     missing = [h for h in PASS if not (UNIT / h).is_file()]
     assert not missing, missing
     res = _sweep(PASS, min(8, os.cpu_count() or 1))
+    bad = [(h, err) for h, ok, err in res if not ok]
+    assert not bad, bad
+
+
+@needs_reference
+def test_pinned_tools_programs_compile():
+    with cf.ThreadPoolExecutor(max_workers=3) as ex:
+        res = list(ex.map(compile_tools_program, TOOLS_PASS))
     bad = [(h, err) for h, ok, err in res if not ok]
     assert not bad, bad
 
