@@ -1313,7 +1313,10 @@ int kkmx_batch_run(kkmx_batch* b, int mode, int steps) {
           KKMX_SAFE_CALL(mx_graph_capture_end(ex.stream(), &b->fused_graph));
           KKMX_SAFE_CALL(mx_graph_instantiate(&b->fused_exec, b->fused_graph));
         }
-        for (int s = 0; s < steps; ++s) KKMX_SAFE_CALL(mx_graph_launch(b->fused_exec, ex.stream()));
+        for (int s = 0; s < steps; ++s) {
+          b->batch.impl_note_replays(ex, 1);
+          KKMX_SAFE_CALL(mx_graph_launch(b->fused_exec, ex.stream()));
+        }
         break;
       case 3:
         if (!b->plain_graph) {

@@ -565,10 +565,24 @@ class BatchDispatch {
     int64_t total              = m_tiles;
     uint64_t* status           = static_cast<uint64_t*>(m_status);
     unsigned* control          = static_cast<unsigned*>(m_control);
+    if (!Impl::g_size_only_dispatch) impl_note_replays(ex, 1);
     Impl::mx_launch_kernel(Impl::batch_kernel(m_shape), m_grid, Impl::kBatchShape[m_shape].block, 0,
                            ex.stream(), ops, tile_op, total, status, control);
   }
   void submit() const { submit(m_exec); }
+
+  // Counts `k` runs of the fused kernel about to be queued on ex's stream
+  // that submit() did not launch itself (replays of a graph that captured
+  // submit()); the tile status words, which keep 30 bits of the launch
+  // epoch, are zeroed first when 2^29 runs have gone by since the last clear.
+  void impl_note_replays(const MI355X& ex, uint64_t k) const {
+    if (m_ops.empty()) return;
+    if (m_since_clear + k > Impl::kScanClearPeriod) {
+      Impl::clear_words_async(m_status, 16 * size_t(m_tiles), ex.stream());
+      m_since_clear = 0;
+    }
+    m_since_clear += k;
+  }
 
   // The same ops as one parallel_reduce / parallel_scan / parallel_for each.
   void submit_plain(const MI355X& ex) const {
@@ -663,6 +677,7 @@ class BatchDispatch {
   void* m_tile_op    = nullptr;
   void* m_status     = nullptr;
   void* m_control    = nullptr;
+  mutable uint64_t m_since_clear = 0;
 };
 
 }  // namespace Experimental

@@ -18,7 +18,9 @@
 //    to the origin before the capture ends.  No allocation or memset is ever
 //    recorded; the kernels' self-cleaning counters and scan epochs make every
 //    replay start from a clean state;
-//  * submit() is one hipGraphLaunch on the submitting instance's stream.
+//  * submit() is one hipGraphLaunch on the submitting instance's stream,
+//    preceded once every 2^29 scan replays by a kernel that zeroes the scan
+//    nodes' status words (their epochs keep 30 bits).
 // Reductions inside a graph must write to device-accessible Views.
 #ifndef KKMX_GRAPH_HPP
 #define KKMX_GRAPH_HPP
@@ -156,10 +158,19 @@ class GraphImpl<MI355X> {
       if (node_stream[i] != origin) KKMX_SAFE_CALL(mx_stream_wait_event(origin, node_done[i]));
     KKMX_SAFE_CALL(mx_graph_capture_end(origin, &m_graph));
     KKMX_SAFE_CALL(mx_graph_instantiate(&m_exec_graph, m_graph));
+    // Scan launches each replay makes per node instance; they are counted
+    // at submit, where the status arrays are zeroed when the period is due.
+    for (auto& nd : m_nodes) {
+      if (!nd->instance) continue;
+      auto* inst = static_cast<MI355XInstance*>(nd->instance.get());
+      if (inst->scan_since_clear) m_scan_nodes.emplace_back(inst, inst->scan_since_clear);
+      inst->scan_since_clear = 0;
+    }
   }
 
   void submit(const MI355X& ex) {
     if (!m_exec_graph) instantiate();
+    for (auto& sn : m_scan_nodes) note_scan_launches(sn.first, sn.second, ex.stream());
     KKMX_SAFE_CALL(mx_graph_launch(m_exec_graph, ex.stream()));
   }
   mx_graph_t native_graph() const { return m_graph; }
@@ -171,6 +182,7 @@ class GraphImpl<MI355X> {
   std::vector<std::unique_ptr<node_t>> m_nodes;
   std::vector<mx_stream_t> m_streams;
   std::vector<mx_event_t> m_events;
+  std::vector<std::pair<MI355XInstance*, uint64_t>> m_scan_nodes;
   mx_graph_t m_graph           = nullptr;
   mx_graph_exec_t m_exec_graph = nullptr;
 };

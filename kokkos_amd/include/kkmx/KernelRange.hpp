@@ -22,7 +22,8 @@
 //                     f(i, v, false) once per index and keeps the per-round
 //                     inclusive wave scans in registers, pass 2 calls
 //                     f(i, prefix, true) once per index.  Tile status words
-//                     carry a launch epoch, so they never need clearing.
+//                     carry a launch epoch, so they are cleared only once
+//                     every 2^29 launches (kScanClearPeriod).
 //
 // Reference closures these specialise: core/src/HIP/Kokkos_HIP_ParallelFor_Range.hpp:28-101,
 // Kokkos_HIP_ParallelReduce_Range.hpp:30-337, Kokkos_HIP_ParallelScan_Range.hpp:29-388.
@@ -210,6 +211,41 @@ constexpr uint32_t kScanAgg  = 1;
 constexpr uint32_t kScanIncl = 2;
 KOKKOS_INLINE_FUNCTION uint32_t scan_flag(uint32_t epoch, uint32_t st) {
   return ((epoch & 0x3fffffffu) << 2) | st;
+}
+
+// A status word keeps the low 30 bits of its launch's epoch, so a word left
+// 2^30 launches earlier would pass for a current one.  The host counts the
+// launches that use one status array and zeroes the array, in stream order
+// and with a kernel (so the step may be stream-captured), before that count
+// since the last clear exceeds 2^29: every word a launch can then see was
+// written by one of the last 2^29 launches, whose epochs differ modulo 2^30.
+inline constexpr uint64_t kScanClearPeriod = uint64_t(1) << 29;
+
+template <class Unused = void>
+__global__ __launch_bounds__(256) void k_clear_words(uint64_t* __restrict__ p, const int64_t n) {
+  const int64_t stride = int64_t(gridDim.x) * 256;
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+// Zeroes `bytes` (a multiple of 8) at p on stream s.
+inline void clear_words_async(void* p, size_t bytes, mx_stream_t s) {
+  if (!p || bytes < 8) return;
+  uint64_t* w     = static_cast<uint64_t*>(p);
+  int64_t n       = static_cast<int64_t>(bytes / 8);
+  const int64_t g = (n + 255) / 256;
+  mx_launch_kernel(reinterpret_cast<const void*>(&k_clear_words<>), static_cast<unsigned>(g < 2048 ? g : 2048),
+                   256, 0, s, w, n);
+}
+
+// Accounts for `k` scan launches about to be queued on stream s that use
+// inst's status arrays, zeroing the arrays first when the period is due.
+inline void note_scan_launches(MI355XInstance* inst, uint64_t k, mx_stream_t s) {
+  if (inst->scan_since_clear + k > kScanClearPeriod) {
+    clear_words_async(inst->scan_status, inst->scan_status_bytes, s);
+    clear_words_async(inst->scan_flags, inst->scan_flags_bytes, s);
+    inst->scan_since_clear = 0;
+  }
+  inst->scan_since_clear += k;
 }
 
 template <class T, int Kind = (sizeof(T) == 4 ? 1 : (sizeof(T) == 8 ? 2 : 0))>
@@ -1035,6 +1071,7 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
   int64_t bb = b, ee = e, nt = ntiles;
   unsigned* control = inst->control;
   const void* k     = range_scan_kernel<CR, Index, WithTotal>(ipt);
+  if (!g_size_only_dispatch) note_scan_launches(inst, 1, inst->stream);
   mx_launch_kernel(k, static_cast<unsigned>(ntiles), c.block, 0, inst->stream, cr, bb, ee, nt, st, control,
                    tot_dst);
   if (copy_back) {

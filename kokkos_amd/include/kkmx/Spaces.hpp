@@ -212,6 +212,9 @@ struct MI355XInstance {
   size_t kernarg_stage_bytes = 0;
   void* team_l1           = nullptr;  // level-1 team scratch pool
   size_t team_l1_bytes    = 0;
+  // Scan launches made against scan_status / scan_flags since they were last
+  // zeroed (tile status words keep only 30 bits of the launch epoch).
+  uint64_t scan_since_clear = 0;
 
   explicit MI355XInstance(mx_stream_t s, bool owns, uint32_t ident)
       : stream(s), owns_stream(owns), id(ident) {

@@ -967,6 +967,60 @@ KKMX_TEST(scan_range_policy_float) { ScanRangeFunctor<float>{}.all({13, 34}); }
 KKMX_TEST(scan_range_policy_double) { ScanRangeFunctor<double>{}.all({17, 59}); }
 KKMX_TEST(scan_range_policy_large) { ScanRangeFunctor<int64_t>{}.all({(size_t(1) << 22) + 12345, 10000019}); }
 
+// Tile status words keep 30 bits of the launch epoch, and the instance zeroes
+// them once every 2^29 scan launches.  The next launch's epoch is set to
+// alias the previous launch's (previous + 2^30) and the launch count to the
+// period: the scan over new data must still be exact, and only its own tiles
+// may hold status words afterwards.
+struct ScanEpochProbe {
+  Kokkos::View<int*, Exec> a, out;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int& u, const bool final) const {
+    if (final) out(i) = u;
+    u += a(i);
+  }
+};
+KKMX_TEST(scan_epoch_wrap_clears_status) {
+  Exec ex;
+  Kokkos::Impl::MI355XInstance* inst = ex.impl_internal_space_instance();
+  const int64_t nbig = int64_t(1) << 22, nsmall = 4096;
+  Kokkos::View<int*, Exec> a("a", nbig), out("out", nbig);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(ex, 0, nbig), KOKKOS_LAMBDA(const int64_t i) { a(i) = int(i % 7) - 3; });
+  Kokkos::parallel_scan(Kokkos::RangePolicy<Exec>(ex, 0, nbig), ScanEpochProbe{a, out});
+  ex.fence();
+  auto nonzero_words = [&]() {
+    std::vector<uint64_t> w(inst->scan_status_bytes / 8);
+    KKMX_SAFE_CALL(mx_memcpy(w.data(), inst->scan_status, w.size() * 8));
+    int64_t c = 0;
+    for (uint64_t x : w) c += (x != 0);
+    return c;
+  };
+  const int64_t before = nonzero_words();
+  CHECK(before > 64);
+  unsigned next_epoch = 0;
+  KKMX_SAFE_CALL(mx_memcpy(&next_epoch, inst->control + 3, sizeof(unsigned)));
+  CHECK(next_epoch >= 2u);
+  const unsigned alias = (next_epoch - 1u) + (1u << 30);
+  KKMX_SAFE_CALL(mx_memcpy(inst->control + 3, &alias, sizeof(unsigned)));
+  inst->scan_since_clear = Kokkos::Impl::kScanClearPeriod;
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec>(ex, 0, nsmall), KOKKOS_LAMBDA(const int64_t i) { a(i) = int(i % 5) + 1; });
+  Kokkos::parallel_scan(Kokkos::RangePolicy<Exec>(ex, 0, nsmall), ScanEpochProbe{a, out});
+  ex.fence();
+  CHECK_EQ(inst->scan_since_clear, uint64_t(1));
+  auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+  int64_t bad = 0, acc = 0;
+  for (int64_t i = 0; i < nsmall; ++i) {
+    bad += (h(i) != acc);
+    acc += i % 5 + 1;
+  }
+  CHECK_EQ(bad, int64_t(0));
+  const int64_t after = nonzero_words();
+  CHECK(after >= 1);
+  CHECK(after <= nsmall / 512);  // tiles hold at least 64 lanes x 8 items
+  unsigned epoch_now = 0;
+  KKMX_SAFE_CALL(mx_memcpy(&epoch_now, inst->control + 3, sizeof(unsigned)));
+  CHECK_EQ(epoch_now, alias + 1u);
+}
+
 // A scan whose value is a struct (generic tile state) and a scan with a tag
 // and a non-zero begin.
 struct Pair64 {
