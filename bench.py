@@ -195,6 +195,29 @@ def _timeit(K, timer, fn, reps=5, warm=2):
     return min(ts), float(np.median(ts))
 
 
+def scan_view_traits(K, P, s, so, n):
+    """The 2^28 int32 inclusive scan with its input read through a plain,
+    a RandomAccess and a Restrict View (kkmx_scan_incl_i32_traits), each after
+    its functor type's tile-body trials: GB/s (8 bytes per element), the body
+    the runtime kept, and exactness against the headline scan's output."""
+    import ctypes as C
+    timer = K.EventTimer()
+    ref = so.download(np.int32, n)
+    out = K.DeviceBuffer(n * 4)
+    res = {}
+    for traits, name in [(0, "plain"), (1, "random_access"), (2, "restrict")]:
+        for _ in range(8):
+            K.check(P.kkmx_scan_incl_i32_traits(s.ptr, out.ptr, n, traits), P)
+            K.fence()
+        mn, med = _timeit(K, timer, lambda: K.check(P.kkmx_scan_incl_i32_traits(s.ptr, out.ptr, n, traits), P),
+                          reps=10, warm=1)
+        res[name] = {"ms": round(mn, 4), "gbps": round(8 * n / mn / 1e6, 1), "median_ms": round(med, 4),
+                     "tile_body": int(P.kkmx_scan_incl_i32_body(traits)),
+                     "exact": bool(np.array_equal(out.download(np.int32, n), ref))}
+    del out
+    return res
+
+
 def leg_small(K, P, timer):
     """Plain-method pair: reduce sum of 2^24 U[0,1) doubles (seed 0) and the
     exclusive scan of 2^24 U[0,15] int32 (seed 1) into int64."""
@@ -611,6 +634,10 @@ def main():
                     help="run one secondary workload alone and print its JSON (for profiling); no headline line")
     ap.add_argument("--scan-shape", type=int, nargs=2, metavar=("BLOCK", "ITEMS"),
                     help="time only the 2^28 int32 inclusive scan under this launch shape (for profiling); no headline line")
+    ap.add_argument("--scan-body", type=int, default=0, choices=[0, 1],
+                    help="with --scan-shape: the scan tile body (0 keeps wave scans, 1 keeps contributions)")
+    ap.add_argument("--scan-traits", type=int, default=1, choices=[0, 1, 2],
+                    help="with --scan-shape: the input View's traits (0 plain, 1 RandomAccess, 2 Restrict)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -629,8 +656,11 @@ def main():
         S, O = K.DeviceBuffer(N * 4), K.DeviceBuffer(N * 4)
         K.check(P.kkmx_fill_uniform_i32(S.ptr, N, 4, 0, 7), P)
         K.check(P.kkmx_set_range_config(2, args.scan_shape[0], args.scan_shape[1], 0), P)
-        t = _timeit(K, K.EventTimer(), lambda: K.check(P.kkmx_scan_incl_i32(S.ptr, O.ptr, N), P), reps=3, warm=1)
+        K.check(P.kkmx_set_scan_body(args.scan_body), P)
+        fn = lambda: K.check(P.kkmx_scan_incl_i32_traits(S.ptr, O.ptr, N, args.scan_traits), P)  # noqa: E731
+        t = _timeit(K, K.EventTimer(), fn, reps=3, warm=1)
         print(json.dumps({"scan_incl_i32_2^28": {"block": args.scan_shape[0], "items": args.scan_shape[1],
+                                                 "tile_body": args.scan_body, "input_traits": args.scan_traits,
                                                  "ms": round(t[0], 4), "gbps": round(8 * N / t[0] / 1e6, 1)}}))
         return
     if args.only:
@@ -673,6 +703,13 @@ def main():
         K.check(rt.mx_event_create(C.byref(e)))
         events.append(e)
 
+    # The scan's tile body is chosen per functor type from the runtime's own
+    # timings of its first six large launches (both bodies give identical
+    # results); let that happen before the timed steps, as it does early in
+    # any long-running program.
+    for _ in range(8):
+        chk(P.kkmx_scan_incl_i32(s.ptr, so.ptr, N))
+        K.fence()
     for _ in range(args.warmup):
         for f in calls:
             chk(f())
@@ -709,6 +746,7 @@ def main():
     checks = verify_step(K, P, calls, a, b, c, x, s, so, res)
     ok = all(checks.values())
 
+    scan_views = scan_view_traits(K, P, s, so, N)
     tuner_range = None if args.no_extra else tuner_in_situ_range(K, P, calls, [0, 0, 0, 0, 1, 1, 1, 2], N)
     peak = measured_copy_peak(K, a.ptr, b.ptr, N * 8, K.EventTimer())
     del a, b, c, x, s, so
@@ -767,6 +805,7 @@ def main():
         "pct_of_measured_peak": {k: round(100 * v / peak, 1) for k, v in op_gbps.items()},
         "results_bit_exact": bool(ok),
         "results_checked": checks,
+        "scan_incl_i32_by_input_view": scan_views,
         "wall_s": round(wall, 3),
         "device": {"arch": props["arch"], "cus": props["cu_count"]},
         "roofline": {

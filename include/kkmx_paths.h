@@ -107,8 +107,27 @@ int kkmx_reduce_minmaxsum_i64(const int64_t* x_dev, int64_t n, int64_t* out_dev)
 /* ---- parallel_scan ---- */
 /* Exclusive prefix sum of int32 input accumulated in int64; total_dev may be NULL. */
 int kkmx_scan_excl_i32_i64(const int32_t* in_dev, int64_t* out_dev, int64_t n, int64_t* total_dev);
-/* Inclusive prefix sum of int32 input in int32. */
+/* Inclusive prefix sum of int32 input in int32.  The input is read through a
+ * const RandomAccess View when it shares no memory with the output, else
+ * through a plain const View (in-place calls are allowed). */
 int kkmx_scan_incl_i32(const int32_t* in_dev, int32_t* out_dev, int64_t n);
+/* The same scan with the input View's memory traits chosen by the caller
+ * (core/src/Kokkos_MemoryTraits.hpp): 0 = Unmanaged, 1 = Unmanaged |
+ * RandomAccess, 2 = Unmanaged | Restrict.  1 and 2 promise the kernel never
+ * writes the input and are rejected (status -1) when the buffers overlap. */
+int kkmx_scan_incl_i32_traits(const int32_t* in_dev, int32_t* out_dev, int64_t n, int in_traits);
+/* Which register-resident tile body every Range scan runs: -1 = chosen per
+ * functor type from the runtime's own timings (the default), 0 = keep the
+ * per-round wave scans, 1 = keep the functor's contributions.  Both give
+ * bitwise-identical results; no counterpart in the reference (its scan has
+ * one kernel, core/src/HIP/Kokkos_HIP_ParallelScan_Range.hpp:29-388), used by
+ * tests and the benchmark to pin the body. */
+int kkmx_set_scan_body(int body);
+/* The tile body the automatic choice kept for kkmx_scan_incl_i32_traits'
+ * functor with these input traits: 0 or 1, or -2 while its trial launches
+ * are still running (the choice is made at the first large launch after the
+ * trials have completed). */
+int kkmx_scan_incl_i32_body(int in_traits);
 /* kkmx_scan_incl_i32 without any allocation: no counterpart in the reference.
  * For callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops).  Its scratch (a 256-byte control block and

@@ -94,7 +94,10 @@ int mx_event_create(mx_event_t* ev);
 int mx_event_destroy(mx_event_t ev);
 int mx_event_record(mx_event_t ev, mx_stream_t stream);
 int mx_event_sync(mx_event_t ev);
+int mx_event_query(mx_event_t ev); /* 0 = complete, 600 (hipErrorNotReady) = pending */
 int mx_event_elapsed_ms(float* ms, mx_event_t start, mx_event_t stop);
+/* *capturing = 1 while the stream records into a graph (stream capture). */
+int mx_stream_capturing(mx_stream_t stream, int* capturing);
 int mx_stream_wait_event(mx_stream_t stream, mx_event_t ev);
 
 /* Kernel launch: fn is the host-side handle of a __global__ function,

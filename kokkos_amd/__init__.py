@@ -70,6 +70,8 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "mx_event_destroy", [vp])
         _sig(lib, "mx_event_record", [vp, vp])
         _sig(lib, "mx_event_sync", [vp])
+        _sig(lib, "mx_event_query", [vp])
+        _sig(lib, "mx_stream_capturing", [vp, C.POINTER(C.c_int)])
         _sig(lib, "mx_event_elapsed_ms", [C.POINTER(C.c_float), vp, vp])
         _sig(lib, "mx_error_string", [C.c_int], C.c_char_p)
         _sig(lib, "mx_default_stream", [C.POINTER(vp)])
@@ -108,6 +110,9 @@ def _declare(name: str, lib: C.CDLL) -> None:
         _sig(lib, "kkmx_reduce_minmaxsum_i64", [vp, i64, vp])
         _sig(lib, "kkmx_scan_excl_i32_i64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_scan_incl_i32", [vp, vp, i64])
+        _sig(lib, "kkmx_scan_incl_i32_traits", [vp, vp, i64, C.c_int])
+        _sig(lib, "kkmx_set_scan_body", [C.c_int])
+        _sig(lib, "kkmx_scan_incl_i32_body", [C.c_int])
         _sig(lib, "kkmx_scan_incl_i32_noalloc", [vp, vp, i64])
         _sig(lib, "kkmx_scan_excl_i64", [vp, vp, i64, vp])
         _sig(lib, "kkmx_scan_excl_f64", [vp, vp, i64, vp])

@@ -35,6 +35,12 @@ COMMON = [
     "-I", str(ROOT / "include"),
     "-I", str(PKG / "include"),
     "-Wno-unused-command-line-argument",
+    # Let the memory-dependence analysis look far enough back for the Range
+    # scan's final pass to reuse its first pass's loads of invariant inputs
+    # (the two are hundreds of instructions apart).
+    "-mllvm", "-memdep-block-scan-limit=100000",
+    "-mllvm", "-memdep-block-number-limit=100000",
+    "-mllvm", "-gvn-max-num-deps=100000",
 ]
 
 # name -> (sources relative to the package, extra flags, link against runtime)

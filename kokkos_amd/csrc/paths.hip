@@ -143,24 +143,59 @@ struct MinMaxSum {
 };
 
 // ---------------------------------------------------------------- scans
-template <class In, class Acc>
+// The scan input is a View type parameter.  The entry points read it through
+// a const RandomAccess View when it shares no memory with the output: the
+// kernel never changes it, and those loads go through the constant address
+// space, which lets the scan's final pass reuse the first pass's values
+// instead of reading the input again.  Overlapping (e.g. in-place) calls use
+// a plain const View.
+template <class T>
+using ReadOnlyView = Kokkos::View<const T*, Kokkos::LayoutRight, Kokkos::MI355XSpace,
+                                  Kokkos::MemoryTraits<Kokkos::Unmanaged | Kokkos::RandomAccess>>;
+template <class T>
+using RestrictView = Kokkos::View<const T*, Kokkos::LayoutRight, Kokkos::MI355XSpace,
+                                  Kokkos::MemoryTraits<Kokkos::Unmanaged | Kokkos::Restrict>>;
+template <class In, class Acc, class InView = DevView<const In>>
 struct ExclusiveScan {
-  DevView<const In> in;
+  InView in;
   DevView<Acc> out;
   KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, Acc& update, const bool final) const {
     if (final) out(i) = update;
     update += static_cast<Acc>(in(i));
   }
 };
-template <class T>
+template <class T, class InView = DevView<const T>>
 struct InclusiveScan {
-  DevView<const T> in;
+  InView in;
   DevView<T> out;
   KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, T& update, const bool final) const {
     update += in(i);
     if (final) out(i) = update;
   }
 };
+inline bool buffers_overlap(const void* a, size_t abytes, const void* b, size_t bbytes) {
+  const auto pa = reinterpret_cast<uintptr_t>(a), pb = reinterpret_cast<uintptr_t>(b);
+  return pa < pb + bbytes && pb < pa + abytes;
+}
+
+// Exclusive scan of n values into out, with an optional device total.
+template <class In, class Acc, class InView>
+void exclusive_scan_run(const char* label, const In* in, Acc* out, int64_t n, Acc* total) {
+  ExclusiveScan<In, Acc, InView> f{InView(in, n), DevView<Acc>(out, n)};
+  if (total) {
+    DevScalar<Acc> t(total);
+    Kokkos::parallel_scan(label, Kokkos::RangePolicy<Kokkos::IndexType<int64_t>>(0, n), f, t);
+  } else {
+    Kokkos::parallel_scan(label, Kokkos::RangePolicy<Kokkos::IndexType<int64_t>>(0, n), f);
+  }
+}
+template <class In, class Acc>
+void exclusive_scan_any(const char* label, const In* in, Acc* out, int64_t n, Acc* total) {
+  if (buffers_overlap(in, sizeof(In) * size_t(n), out, sizeof(Acc) * size_t(n)))
+    exclusive_scan_run<In, Acc, DevView<const In>>(label, in, out, n, total);
+  else
+    exclusive_scan_run<In, Acc, ReadOnlyView<In>>(label, in, out, n, total);
+}
 
 
 // ---------------------------------------------------------------- stencil
@@ -944,22 +979,51 @@ int kkmx_reduce_minmaxsum_i64(const int64_t* x, int64_t n, int64_t* out) {
 
 int kkmx_scan_excl_i32_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* total) {
   if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_scan_excl_i32_i64: bad arguments");
+  return guarded([&] { exclusive_scan_any("kkmx::scan_excl_i32_i64", in, out, n, total); });
+}
+int kkmx_scan_incl_i32_traits(const int32_t* in, int32_t* out, int64_t n, int in_traits) {
+  if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_scan_incl_i32_traits: bad arguments");
+  if (in_traits < 0 || in_traits > 2) return reject("kkmx_scan_incl_i32_traits: in_traits must be 0, 1 or 2");
+  if (in_traits != 0 && n > 0 && buffers_overlap(in, 4 * size_t(n), out, 4 * size_t(n)))
+    return reject("kkmx_scan_incl_i32_traits: RandomAccess / Restrict input must not overlap the output");
   return guarded([&] {
-    ExclusiveScan<int32_t, int64_t> f{DevView<const int32_t>(in, n), DevView<int64_t>(out, n)};
-    if (total) {
-      DevScalar<int64_t> t(total);
-      Kokkos::parallel_scan("kkmx::scan_excl_i32_i64", Range64(0, n), f, t);
-    } else {
-      Kokkos::parallel_scan("kkmx::scan_excl_i32_i64", Range64(0, n), f);
-    }
+    if (in_traits == 1)
+      Kokkos::parallel_scan("kkmx::scan_incl_i32", Range64(0, n),
+                            InclusiveScan<int32_t, ReadOnlyView<int32_t>>{ReadOnlyView<int32_t>(in, n),
+                                                                          DevView<int32_t>(out, n)});
+    else if (in_traits == 2)
+      Kokkos::parallel_scan("kkmx::scan_incl_i32", Range64(0, n),
+                            InclusiveScan<int32_t, RestrictView<int32_t>>{RestrictView<int32_t>(in, n),
+                                                                          DevView<int32_t>(out, n)});
+    else
+      Kokkos::parallel_scan("kkmx::scan_incl_i32", Range64(0, n),
+                            InclusiveScan<int32_t>{DevView<const int32_t>(in, n), DevView<int32_t>(out, n)});
   });
 }
 int kkmx_scan_incl_i32(const int32_t* in, int32_t* out, int64_t n) {
   if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_scan_incl_i32: bad arguments");
-  return guarded([&] {
-    Kokkos::parallel_scan("kkmx::scan_incl_i32", Range64(0, n),
-                          InclusiveScan<int32_t>{DevView<const int32_t>(in, n), DevView<int32_t>(out, n)});
-  });
+  return kkmx_scan_incl_i32_traits(in, out, n, buffers_overlap(in, 4 * size_t(n), out, 4 * size_t(n)) ? 0 : 1);
+}
+int kkmx_scan_incl_i32_body(int in_traits) {
+  using namespace Kokkos::Impl;
+  using R = Kokkos::RangePolicy<Kokkos::IndexType<int64_t>>;
+  auto chosen = [](auto f) {
+    using CR = typename ScanTypes<R, decltype(f), void>::CR;
+    std::lock_guard<std::mutex> lk(scan_body_choice<CR>().m);
+    const int c = scan_body_choice<CR>().chosen;
+    return c < 0 ? -2 : c;
+  };
+  switch (in_traits) {
+    case 0: return chosen(InclusiveScan<int32_t>{});
+    case 1: return chosen(InclusiveScan<int32_t, ReadOnlyView<int32_t>>{});
+    case 2: return chosen(InclusiveScan<int32_t, RestrictView<int32_t>>{});
+    default: return reject("kkmx_scan_incl_i32_body: in_traits must be 0, 1 or 2");
+  }
+}
+int kkmx_set_scan_body(int body) {
+  if (body < -1 || body > 1) return reject("kkmx_set_scan_body: body must be -1, 0 or 1");
+  Kokkos::Impl::range_overrides().scan_body = body;
+  return 0;
 }
 int kkmx_scan_incl_i32_noalloc(const int32_t* in, int32_t* out, int64_t n) {
   if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_scan_incl_i32_noalloc: bad arguments");
@@ -996,27 +1060,11 @@ int kkmx_scan_incl_i32_noalloc(const int32_t* in, int32_t* out, int64_t n) {
 }
 int kkmx_scan_excl_i64(const int64_t* in, int64_t* out, int64_t n, int64_t* total) {
   if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_scan_excl_i64: bad arguments");
-  return guarded([&] {
-    ExclusiveScan<int64_t, int64_t> f{DevView<const int64_t>(in, n), DevView<int64_t>(out, n)};
-    if (total) {
-      DevScalar<int64_t> t(total);
-      Kokkos::parallel_scan("kkmx::scan_excl_i64", Range64(0, n), f, t);
-    } else {
-      Kokkos::parallel_scan("kkmx::scan_excl_i64", Range64(0, n), f);
-    }
-  });
+  return guarded([&] { exclusive_scan_any("kkmx::scan_excl_i64", in, out, n, total); });
 }
 int kkmx_scan_excl_f64(const double* in, double* out, int64_t n, double* total) {
   if (n < 0 || (n > 0 && (!in || !out))) return reject("kkmx_scan_excl_f64: bad arguments");
-  return guarded([&] {
-    ExclusiveScan<double, double> f{DevView<const double>(in, n), DevView<double>(out, n)};
-    if (total) {
-      DevScalar<double> t(total);
-      Kokkos::parallel_scan("kkmx::scan_excl_f64", Range64(0, n), f, t);
-    } else {
-      Kokkos::parallel_scan("kkmx::scan_excl_f64", Range64(0, n), f);
-    }
-  });
+  return guarded([&] { exclusive_scan_any("kkmx::scan_excl_f64", in, out, n, total); });
 }
 
 

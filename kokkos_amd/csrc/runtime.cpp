@@ -218,6 +218,19 @@ int mx_event_sync(mx_event_t ev) {
   return rc(hipEventSynchronize(static_cast<hipEvent_t>(ev)));
 }
 
+int mx_event_query(mx_event_t ev) {
+  const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(ev));
+  if (e == hipErrorNotReady) (void)hipGetLastError();  // pending is not an error
+  return rc(e);
+}
+
+int mx_stream_capturing(mx_stream_t stream, int* capturing) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  const int r = rc(hipStreamIsCapturing(static_cast<hipStream_t>(stream), &st));
+  if (capturing) *capturing = (st != hipStreamCaptureStatusNone);
+  return r;
+}
+
 int mx_event_elapsed_ms(float* ms, mx_event_t start, mx_event_t stop) {
   return rc(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
                                 static_cast<hipEvent_t>(stop)));

@@ -528,7 +528,7 @@ class BatchDispatch {
       t += d.ntiles;
       m_elements += d.n;
     }
-    if (t >= (int64_t(1) << 31)) Impl::throw_runtime_exception("Kokkos::Experimental::BatchDispatch: too many tiles");
+    if (t >= Impl::kScanMaxTiles) Impl::throw_runtime_exception("Kokkos::Experimental::BatchDispatch: too many tiles");
     m_tiles = t;
     if (m_ops.empty()) {
       m_planned = true;

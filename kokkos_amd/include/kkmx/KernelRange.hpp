@@ -251,6 +251,24 @@ inline void note_scan_launches(MI355XInstance* inst, uint64_t k, mx_stream_t s) 
 template <class T, int Kind = (sizeof(T) == 4 ? 1 : (sizeof(T) == 8 ? 2 : 0))>
 struct ScanTileState;
 
+// Status words are published with a buffer store carrying the agent-scope
+// coherence bit (sc1), the same memory operation a relaxed agent-scope
+// atomic store of the word compiles to.  As an intrinsic call rather than an
+// IR atomic, the compiler can see that it does not write memory that is
+// invariant for the kernel, so a functor's loads through const RandomAccess /
+// Restrict Views (constant address space) are not clobbered by a publish and
+// can be reused across the look-back.  Word offsets are 32-bit: the host
+// keeps the tile count below 2^27.
+typedef unsigned int scan_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void scan_publish_word(uint64_t* words, int64_t w, uint32_t flag, uint32_t bits) {
+  scan_u32x2 d;
+  d.x                        = bits;
+  d.y                        = flag;
+  __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(words, 0, 0x7fffffff, 0x00020000);
+  __builtin_amdgcn_raw_buffer_store_b64(d, rsrc, static_cast<int>(w * 8), 0, /*sc1*/ 16);
+}
+inline constexpr int64_t kScanMaxTiles = int64_t(1) << 27;
+
 // 4-byte values: one 64-bit word {flag, value}.
 template <class T>
 struct ScanTileState<T, 1> {
@@ -260,8 +278,7 @@ struct ScanTileState<T, 1> {
   __device__ void publish(int64_t t, uint32_t flag, const T& v) const {
     uint32_t bits;
     __builtin_memcpy(&bits, &v, 4);
-    __hip_atomic_store(&words[t], (static_cast<uint64_t>(flag) << 32) | bits, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    scan_publish_word(words, t, flag, bits);
   }
   __device__ uint32_t read(int64_t t, T& v) const {
     const uint64_t w = __hip_atomic_load(&words[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -281,11 +298,8 @@ struct ScanTileState<T, 2> {
   __device__ void publish(int64_t t, uint32_t flag, const T& v) const {
     uint64_t bits;
     __builtin_memcpy(&bits, &v, 8);
-    const uint64_t hi = static_cast<uint64_t>(flag) << 32;
-    __hip_atomic_store(&words[2 * t], hi | (bits & 0xffffffffu), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&words[2 * t + 1], hi | (bits >> 32), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    scan_publish_word(words, 2 * t, flag, static_cast<uint32_t>(bits));
+    scan_publish_word(words, 2 * t + 1, flag, static_cast<uint32_t>(bits >> 32));
   }
   __device__ uint32_t read(int64_t t, T& v) const {
     const uint64_t a = __hip_atomic_load(&words[2 * t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -599,12 +613,118 @@ __device__ __forceinline__ void scan_tile_rr(const CR& cr, const int64_t begin, 
   }
 }
 
+// The same tile with the functor's contributions kept in registers instead
+// of the per-round wave scans; the final pass recomputes each round's wave
+// scan from them (the same DPP steps in the same order, so the results are
+// bitwise those of scan_tile_rr).  What it buys: when the functor reads
+// through Views whose data the kernel cannot change (const RandomAccess /
+// Restrict, loaded through the constant address space) and the library is
+// built with the larger memory-dependence limits, the compiler replaces the
+// final pass's loads with the first pass's values, so the tile is read from
+// HBM once.  For that the first pass must dominate the final pass, so full
+// and ragged tiles run separate copies of the body (Full).  Index bases are
+// made opaque per chunk of 8 rounds, so every round's address is a chunk base
+// plus an immediate offset instead of a separately materialised 64-bit
+// address; the ragged copy re-derives its bases before the final pass, so its
+// bounds masks are not carried across the look-back.
+template <class CR, class Index, int IPT, bool WithTotal, bool Full>
+__device__ __forceinline__ void scan_tile_keep_body(const CR& cr, const int64_t end, const int64_t ntiles,
+                                                    const ScanTileState<typename CR::value_type>& st,
+                                                    const int64_t tile, const uint32_t epoch,
+                                                    typename CR::value_type* total_out,
+                                                    typename CR::value_type* s_wave, typename CR::value_type* s_red,
+                                                    int* s_pos, const int64_t base) {
+  using V         = typename CR::value_type;
+  using Tag       = typename CR::work_tag;
+  const auto join = [&cr](V& a, const V& b) { cr.join(a, b); };
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6, l = wave::lane();
+  V identity;
+  cr.init(identity);
+  constexpr int kChunk  = 8;
+  constexpr int kChunks = (IPT + kChunk - 1) / kChunk;
+  int64_t chunk_base[kChunks];
+#pragma unroll
+  for (int c = 0; c < kChunks; ++c) {
+    chunk_base[c] = base + c * (kChunk * 64);
+    asm volatile("" : "+v"(chunk_base[c]));
+  }
+  V v[IPT];
+  V wave_total = identity;
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) {
+    const int64_t i = chunk_base[r / kChunk] + (r % kChunk) * 64;
+    v[r]            = identity;
+    if (Full || i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), v[r], false);
+  }
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) join(wave_total, wave::reduce(v[r], identity, join));
+  if (l == 0) s_wave[w] = wave_total;
+  __syncthreads();
+  const V x    = (l < nw) ? s_wave[l] : identity;
+  const V xs   = wave::inclusive_scan(x, identity, join);
+  const V tagg = wave::readlane(xs, nw - 1);
+  const V wpre = (w == 0) ? identity : wave::readlane(xs, w - 1);
+
+  V prefix = identity;
+  if (tile == 0) {
+    if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanIncl), tagg);
+  } else {
+    if (threadIdx.x == 0) st.publish(tile, scan_flag(epoch, kScanAgg), tagg);
+    prefix = scan_lookback(st, tile, epoch, identity, join, s_red, s_pos);
+    if (threadIdx.x == 0) {
+      V t = prefix;
+      join(t, tagg);
+      st.publish(tile, scan_flag(epoch, kScanIncl), t);
+    }
+  }
+  if constexpr (WithTotal) {
+    if (threadIdx.x == 0 && tile == ntiles - 1) {
+      V tot = prefix;
+      join(tot, tagg);
+      *total_out = tot;
+    }
+  }
+  V running = prefix;
+  join(running, wpre);
+  if constexpr (!Full) {
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) asm volatile("" : "+v"(chunk_base[c]));
+  }
+#pragma unroll
+  for (int r = 0; r < IPT; ++r) {
+    const V inc = wave::inclusive_scan(v[r], identity, join);
+    V u         = running;
+    join(u, wave::shift_up(inc, identity));
+    join(running, wave::readlane(inc, 63));
+    const int64_t i = chunk_base[r / kChunk] + (r % kChunk) * 64;
+    if (Full || i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), u, true);
+  }
+}
+
+template <class CR, class Index, int IPT, bool WithTotal>
+__device__ __forceinline__ void scan_tile_keep(const CR& cr, const int64_t begin, const int64_t end,
+                                               const int64_t ntiles, const ScanTileState<typename CR::value_type>& st,
+                                               const int64_t tile, const uint32_t epoch,
+                                               typename CR::value_type* total_out, typename CR::value_type* s_wave,
+                                               typename CR::value_type* s_red, int* s_pos) {
+  const int64_t tile_elems = static_cast<int64_t>(blockDim.x) * IPT;
+  const int64_t tbase      = begin + tile * tile_elems;
+  const int64_t base       = tbase + static_cast<int64_t>(threadIdx.x >> 6) * (64 * IPT) + wave::lane();
+  if (tbase + tile_elems <= end)
+    scan_tile_keep_body<CR, Index, IPT, WithTotal, true>(cr, end, ntiles, st, tile, epoch, total_out, s_wave, s_red,
+                                                         s_pos, base);
+  else
+    scan_tile_keep_body<CR, Index, IPT, WithTotal, false>(cr, end, ntiles, st, tile, epoch, total_out, s_wave,
+                                                          s_red, s_pos, base);
+}
+
 // One tile per workgroup, claimed in launch order from control[2..3] (one
 // 64-bit word {tile counter (low), epoch (high)}: a single relaxed fetch_add
 // returns a consistent pair, so the claim needs no fence).  The workgroup of
 // the last tile returns the word to {0, epoch + 1}: every workgroup has
 // claimed by then.
-template <class CR, class Index, int IPT, bool WithTotal>
+// Keep selects the tile body: 0 = scan_tile_rr, 1 = scan_tile_keep.
+template <class CR, class Index, int IPT, bool WithTotal, int Keep = 0>
 __global__ __launch_bounds__(1024) void k_range_scan_rr(
     const CR cr, const int64_t begin, const int64_t end, const int64_t ntiles,
     ScanTileState<typename CR::value_type> st, unsigned* control,
@@ -624,8 +744,12 @@ __global__ __launch_bounds__(1024) void k_range_scan_rr(
   if (threadIdx.x == 0 && tile == ntiles - 1)
     __hip_atomic_store(claim_word, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
-  scan_tile_rr<CR, Index, IPT, WithTotal>(cr, begin, end, ntiles, st, tile, epoch, total_out, s_wave.get(),
-                                          s_red.get(), &s_pos);
+  if constexpr (Keep == 1)
+    scan_tile_keep<CR, Index, IPT, WithTotal>(cr, begin, end, ntiles, st, tile, epoch, total_out, s_wave.get(),
+                                              s_red.get(), &s_pos);
+  else
+    scan_tile_rr<CR, Index, IPT, WithTotal>(cr, begin, end, ntiles, st, tile, epoch, total_out, s_wave.get(),
+                                            s_red.get(), &s_pos);
 }
 
 // ------------------------------------------------------------------ host helpers
@@ -693,7 +817,8 @@ inline int64_t range_grid(int64_t n, const RangeConfig& c) {
 // answer for one dispatch takes precedence.
 struct RangeOverrides {
   RangeConfig cfg[3];
-  bool set[3] = {false, false, false};
+  bool set[3]   = {false, false, false};
+  int scan_body = -1;  // >= 0: every scan runs this tile body (no trials)
 };
 inline RangeOverrides& range_overrides() {
   static RangeOverrides o;
@@ -1001,8 +1126,63 @@ class ParallelReduce<CR, RangePolicy<Traits...>, MI355X> {
   bool m_has_config = false;
 };
 
+// ------------------------------------------------------------------ scan tile body choice
+// Which tile body a scan functor type runs (k_range_scan_rr's Keep, for 16
+// and 32 values per lane).  Both give bitwise-identical results, so the
+// choice is made from the runtime's own timings: the first 2 * kScanTrials
+// launches of a functor type over at least kScanTrialElems indices alternate
+// the two bodies, each bracketed by events on the launching stream; at a
+// later launch, once every trial has completed (checked without blocking),
+// the body with the lower best time per index is kept for the type.
+// Launches captured into a graph, size-only passes and smaller launches use
+// the choice made so far (body 0 until then).
+inline constexpr int kScanTrials         = 3;
+inline constexpr int64_t kScanTrialElems = int64_t(1) << 21;
+struct ScanBodyChoice {
+  std::mutex m;
+  int chosen   = -1;
+  int launched = 0;
+  mx_event_t ev[2 * kScanTrials][2] = {};
+  int64_t elems[2 * kScanTrials]   = {};
+};
+template <class CR>
+ScanBodyChoice& scan_body_choice() {
+  static ScanBodyChoice c;
+  return c;
+}
+// The body for a launch of n indices on stream s; for a trial launch,
+// *start / *stop are the events to record around it (else null).
+inline int pick_scan_body(ScanBodyChoice& c, int64_t n, mx_stream_t s, mx_event_t* start, mx_event_t* stop) {
+  *start = *stop = nullptr;
+  if (range_overrides().scan_body >= 0) return range_overrides().scan_body;
+  std::lock_guard<std::mutex> lk(c.m);
+  if (c.chosen >= 0) return c.chosen;
+  if (n < kScanTrialElems || g_size_only_dispatch) return 0;
+  int capturing = 0;
+  if (mx_stream_capturing(s, &capturing) != 0 || capturing) return 0;
+  if (c.launched < 2 * kScanTrials) {
+    const int k = c.launched++;
+    for (int e = 0; e < 2; ++e)
+      if (!c.ev[k][e]) KKMX_SAFE_CALL(mx_event_create(&c.ev[k][e]));
+    c.elems[k] = n;
+    *start     = c.ev[k][0];
+    *stop      = c.ev[k][1];
+    return k & 1;
+  }
+  for (int k = 0; k < 2 * kScanTrials; ++k)
+    if (mx_event_query(c.ev[k][1]) != 0) return 0;
+  double best[2] = {1e300, 1e300};
+  for (int k = 0; k < 2 * kScanTrials; ++k) {
+    float ms = 0.f;
+    if (mx_event_elapsed_ms(&ms, c.ev[k][0], c.ev[k][1]) != 0) continue;
+    best[k & 1] = std::min(best[k & 1], double(ms) / double(c.elems[k]));
+  }
+  c.chosen = best[1] < best[0] ? 1 : 0;
+  return c.chosen;
+}
+
 template <class CR, class Index, bool WithTotal>
-const void* range_scan_kernel(int ipt) {
+const void* range_scan_kernel(int ipt, int keep = 0) {
   // Register-resident while IPT values take at most 256 bytes per lane
   // (measured: 32 int64 values per lane at 1024 lanes still run at 5 waves
   // per SIMD and beat the two-pass tile by 2x).
@@ -1010,10 +1190,14 @@ const void* range_scan_kernel(int ipt) {
   switch (ipt >= 64 ? 64 : ipt >= 32 ? 32 : ipt >= 16 ? 16 : 8) {
     case 64: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 64, WithTotal>);
     case 32:
-      if constexpr (rr_max >= 32) return reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 32, WithTotal>);
+      if constexpr (rr_max >= 32)
+        return keep ? reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 32, WithTotal, 1>)
+                    : reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 32, WithTotal>);
       else return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 32, WithTotal>);
     case 16:
-      if constexpr (rr_max >= 16) return reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 16, WithTotal>);
+      if constexpr (rr_max >= 16)
+        return keep ? reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 16, WithTotal, 1>)
+                    : reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 16, WithTotal>);
       else return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 16, WithTotal>);
     default:
       if constexpr (rr_max >= 8) return reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 8, WithTotal>);
@@ -1055,6 +1239,7 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
   const int ipt = c.items >= 64 ? 64 : c.items >= 32 ? 32 : c.items >= 16 ? 16 : 8;
   const int64_t tile   = static_cast<int64_t>(c.block) * ipt;
   const int64_t ntiles = ceil_div(e - b, tile);
+  if (ntiles >= kScanMaxTiles) throw_runtime_exception("Kokkos::parallel_scan: extent needs too many tiles");
   ScanTileState<V> st;
   void* status = scan_state_kind<V>::value == 0 ? inst->scan_flags_space(ScanTileState<V>::bytes(ntiles))
                                                 : inst->scan_status_space(ScanTileState<V>::bytes(ntiles));
@@ -1070,10 +1255,14 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
   }
   int64_t bb = b, ee = e, nt = ntiles;
   unsigned* control = inst->control;
-  const void* k     = range_scan_kernel<CR, Index, WithTotal>(ipt);
+  mx_event_t ev_start = nullptr, ev_stop = nullptr;
+  const int keep    = ipt >= 16 ? pick_scan_body(scan_body_choice<CR>(), e - b, inst->stream, &ev_start, &ev_stop) : 0;
+  const void* k     = range_scan_kernel<CR, Index, WithTotal>(ipt, keep);
   if (!g_size_only_dispatch) note_scan_launches(inst, 1, inst->stream);
+  if (ev_start) KKMX_SAFE_CALL(mx_event_record(ev_start, inst->stream));
   mx_launch_kernel(k, static_cast<unsigned>(ntiles), c.block, 0, inst->stream, cr, bb, ee, nt, st, control,
                    tot_dst);
+  if (ev_stop) KKMX_SAFE_CALL(mx_event_record(ev_stop, inst->stream));
   if (copy_back) {
     KKMX_SAFE_CALL(mx_stream_sync(inst->stream));
     std::memcpy(static_cast<void*>(total), tot_dst, sizeof(V));

@@ -187,6 +187,92 @@ def test_scan_state_reuse_across_launches(paths, oracle):
         assert np.array_equal(o.download(np.int32, n), oracle.scan_incl_i32(x)), n
 
 
+@pytest.fixture
+def scan_body(paths):
+    """Pin the scan tile body for one test; the automatic choice afterwards."""
+    def pin(body):
+        K.check(paths.kkmx_set_scan_body(body), paths)
+    yield pin
+    K.check(paths.kkmx_set_scan_body(-1), paths)
+
+
+@pytest.mark.parametrize("n", [1, 64, 4097, 100003, (1 << 21) + 17, 1 << 22, (1 << 24) + 5])
+def test_scan_bodies_and_view_traits_bit_exact(paths, oracle, scan_body, n):
+    """Both register-resident tile bodies (keep the wave scans / keep the
+    contributions), with the input read through a plain, a RandomAccess and a
+    Restrict View, equal the Serial scan bit for bit; ragged last tiles
+    included (n not a multiple of any tile)."""
+    P = paths
+    x = oracle.fill_uniform_i32(n, 40 + n % 7, -100, 100)
+    ref = oracle.scan_incl_i32(x)
+    X, o = dev(x), K.DeviceBuffer(n * 4)
+    for body in (0, 1):
+        scan_body(body)
+        for traits in (0, 1, 2):
+            o.upload(np.zeros(n, np.int32))
+            run(P, P.kkmx_scan_incl_i32_traits(X.ptr, o.ptr, n, traits))
+            assert np.array_equal(o.download(np.int32, n), ref), (body, traits)
+
+
+def test_scan_bodies_f64(paths, oracle, scan_body):
+    """Both tile bodies on FP64 exclusive scans with totals.  Integer-valued
+    doubles add exactly, so both equal the Serial scan bit for bit; for
+    U[0,1) values every prefix is within 1e-12 of the compensated scan.  (The
+    look-back combines the nearest predecessor that has published an
+    inclusive prefix, which depends on timing, so rounded FP64 scans are not
+    bitwise reproducible from run to run under either body.)"""
+    P = paths
+    n = (1 << 22) + 333
+    xi = oracle.fill_uniform_i32(n, 12, 0, 100).astype(np.float64)
+    y = oracle.fill_uniform_f64(n, 11, 0.0, 1.0)
+    XI, Y, o, t = dev(xi), dev(y), K.DeviceBuffer(n * 8), K.DeviceBuffer(8)
+    ref_i, rt_i = oracle.scan_excl_f64(xi)
+    ref, rt = oracle.scan_excl_f64_compensated(y)
+    for body in (0, 1):
+        scan_body(body)
+        run(P, P.kkmx_scan_excl_f64(XI.ptr, o.ptr, n, t.ptr))
+        assert np.array_equal(o.download(np.float64, n), ref_i) and t.download(np.float64, 1)[0] == rt_i, body
+        run(P, P.kkmx_scan_excl_f64(Y.ptr, o.ptr, n, t.ptr))
+        got = o.download(np.float64, n)
+        err = np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0))
+        terr = abs(t.download(np.float64, 1)[0] - rt) / rt
+        print(f"scan_f64 body {body}: max rel err vs compensated {err:.3e}, total {terr:.3e}")
+        assert err < 1e-12 and terr < 1e-12, body
+
+
+def test_scan_in_place_and_overlap_rules(paths, oracle):
+    """kkmx_scan_incl_i32 in place (in == out) reads through a plain View and
+    is exact; asking for a RandomAccess / Restrict input that overlaps the
+    output is rejected before anything runs."""
+    P = paths
+    for n in (5, 70001, (1 << 22) + 3):
+        x = oracle.fill_uniform_i32(n, 50, 0, 9)
+        B = dev(x)
+        run(P, P.kkmx_scan_incl_i32(B.ptr, B.ptr, n))
+        assert np.array_equal(B.download(np.int32, n), oracle.scan_incl_i32(x)), n
+        for traits in (1, 2):
+            assert P.kkmx_scan_incl_i32_traits(B.ptr, B.ptr, n, traits) == -1
+    assert P.kkmx_scan_incl_i32_traits(B.ptr, B.ptr, 4, 3) == -1
+    assert P.kkmx_set_scan_body(2) == -1
+
+
+def test_scan_automatic_body_choice_exact_every_call(paths, oracle):
+    """With the automatic choice the first large launches of a functor type
+    alternate the two bodies before one is kept: every call is exact."""
+    P = paths
+    n = (1 << 21) + 1000
+    for k in range(10):
+        x = oracle.fill_uniform_i32(n, 60 + k, -5, 50)
+        X, o = dev(x), K.DeviceBuffer(n * 4)
+        run(P, P.kkmx_scan_incl_i32(X.ptr, o.ptr, n))
+        assert np.array_equal(o.download(np.int32, n), oracle.scan_incl_i32(x)), k
+        y = oracle.fill_uniform_i64(n, 70 + k, -(1 << 40), 1 << 40)
+        Y, p, t = dev(y), K.DeviceBuffer(n * 8), K.DeviceBuffer(8)
+        run(P, P.kkmx_scan_excl_i64(Y.ptr, p.ptr, n, t.ptr))
+        ref, rt = oracle.scan_excl_i64(y)
+        assert np.array_equal(p.download(np.int64, n), ref) and int(t.download(np.int64, 1)[0]) == int(rt), k
+
+
 def test_full_size_config2(paths, oracle):
     """2^28 elements: int64 reduce max/min/sum (seed 3) and int32 inclusive
     scan (seed 4), generated on the device, compared bit for bit."""
