@@ -1188,7 +1188,10 @@ const void* range_scan_kernel(int ipt, int keep = 0) {
   // per SIMD and beat the two-pass tile by 2x).
   constexpr int rr_max = int(256 / sizeof(typename CR::value_type));
   switch (ipt >= 64 ? 64 : ipt >= 32 ? 32 : ipt >= 16 ? 16 : 8) {
-    case 64: return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 64, WithTotal>);
+    case 64:
+      if constexpr (rr_max >= 64)
+        if (keep) return reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 64, WithTotal, 1>);
+      return reinterpret_cast<const void*>(&k_range_scan<CR, Index, 64, WithTotal>);
     case 32:
       if constexpr (rr_max >= 32)
         return keep ? reinterpret_cast<const void*>(&k_range_scan_rr<CR, Index, 32, WithTotal, 1>)
@@ -1235,8 +1238,20 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
                       ? range_overrides().cfg[2]
                       : tuned_range_config<typename CR::functor_type>(2, e - b, sizeof(V), std::is_floating_point<V>::value,
                                                                      default_scan_config(e - b));
-  c.block       = normalise_block(c.block);
-  const int ipt = c.items >= 64 ? 64 : c.items >= 32 ? 32 : c.items >= 16 ? 16 : 8;
+  c.block = normalise_block(c.block);
+  int ipt = c.items >= 64 ? 64 : c.items >= 32 ? 32 : c.items >= 16 ? 16 : 8;
+  // The tile body (trial launches alternate the two), then the shape that
+  // suits it: at the default shape for large extents (1024 lanes x 32) the
+  // contribution-keeping body takes 64 values per lane while they fit its
+  // register budget, since it no longer relies on the L2 to serve a re-read
+  // and larger tiles halve the claims and look-backs.
+  mx_event_t ev_start = nullptr, ev_stop = nullptr;
+  const int keep = ipt >= 16 ? pick_scan_body(scan_body_choice<CR>(), e - b, inst->stream, &ev_start, &ev_stop) : 0;
+  constexpr bool keep64_fits = sizeof(V) <= 4 && scan_state_kind<V>::value != 0;
+  if (keep64_fits && keep == 1 && !cfg && !range_overrides().set[2] && c.block == 1024 && ipt == 32) {
+    const RangeConfig d = default_scan_config(e - b);
+    if (d.block == 1024 && d.items == 32) ipt = 64;
+  }
   const int64_t tile   = static_cast<int64_t>(c.block) * ipt;
   const int64_t ntiles = ceil_div(e - b, tile);
   if (ntiles >= kScanMaxTiles) throw_runtime_exception("Kokkos::parallel_scan: extent needs too many tiles");
@@ -1255,8 +1270,6 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
   }
   int64_t bb = b, ee = e, nt = ntiles;
   unsigned* control = inst->control;
-  mx_event_t ev_start = nullptr, ev_stop = nullptr;
-  const int keep    = ipt >= 16 ? pick_scan_body(scan_body_choice<CR>(), e - b, inst->stream, &ev_start, &ev_stop) : 0;
   const void* k     = range_scan_kernel<CR, Index, WithTotal>(ipt, keep);
   if (!g_size_only_dispatch) note_scan_launches(inst, 1, inst->stream);
   if (ev_start) KKMX_SAFE_CALL(mx_event_record(ev_start, inst->stream));
