@@ -265,10 +265,44 @@ struct PlusReduceFunctor {
 template <class Ex>
 using Range64 = RangePolicy<Ex, IndexType<int64_t>>;
 
+// Input of a scan whose source range shares no memory with its destination:
+// the kernel never writes it, so its loads go through the constant address
+// space, like those of a const RandomAccess View, and the scan's final pass
+// can reuse the first pass's values instead of reading the input again.
+template <class T>
+struct InvariantInput {
+  const T* ptr;
+  ptrdiff_t stride;
+  KOKKOS_FUNCTION T operator[](ptrdiff_t i) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    using invariant_ptr = const __attribute__((address_space(4))) T*;
+    return *(invariant_ptr)(ptr + i * stride);
+#else
+    return ptr[i * stride];
+#endif
+  }
+};
+
+template <class It>
+struct is_device_view_iterator : std::false_type {};
+template <class D, class... P>
+struct is_device_view_iterator<RandomAccessIterator<View<D, P...>>>
+    : std::bool_constant<std::is_same<typename View<D, P...>::memory_space, MI355XSpace>::value &&
+                         std::is_arithmetic<typename View<D, P...>::non_const_value_type>::value> {};
+
+// Whether [a, a + n*sa) and [b, b + n*sb) (element strides, bytes ea / eb)
+// share memory.
+inline bool ranges_overlap(const void* a, ptrdiff_t sa, size_t ea, const void* b, ptrdiff_t sb, size_t eb,
+                           int64_t n) {
+  const auto pa = reinterpret_cast<uintptr_t>(a), pb = reinterpret_cast<uintptr_t>(b);
+  const uintptr_t la = uintptr_t(n - 1) * uintptr_t(sa < 0 ? -sa : sa) * ea + ea;
+  const uintptr_t lb = uintptr_t(n - 1) * uintptr_t(sb < 0 ? -sb : sb) * eb + eb;
+  if (sa < 0 || sb < 0) return true;
+  return pa < pb + lb && pb < pa + la;
+}
+
 template <class T, bool Inclusive, bool HasInit, class Ex, class InIt, class OutIt, class Op, class UOp>
-OutIt scan_impl(const std::string& label, const Ex& ex, InIt first, InIt last, OutIt dest, Op op, UOp u, T init) {
-  const int64_t n = static_cast<int64_t>(last - first);
-  if (n <= 0) return dest;
+void scan_run(const std::string& label, const Ex& ex, InIt first, OutIt dest, int64_t n, Op op, UOp u, T init) {
   if constexpr (std::is_same<Op, PlusOp>::value) {
     parallel_scan(label, Range64<Ex>(ex, 0, n),
                   PlusScanFunctor<T, InIt, OutIt, UOp, Inclusive>{first, dest, u, HasInit ? init : T(0)});
@@ -276,6 +310,23 @@ OutIt scan_impl(const std::string& label, const Ex& ex, InIt first, InIt last, O
     parallel_scan(label, Range64<Ex>(ex, 0, n), ScanFunctor<T, InIt, OutIt, Op, UOp, Inclusive, HasInit>{
                                                     first, dest, op, u, init});
   }
+}
+
+template <class T, bool Inclusive, bool HasInit, class Ex, class InIt, class OutIt, class Op, class UOp>
+OutIt scan_impl(const std::string& label, const Ex& ex, InIt first, InIt last, OutIt dest, Op op, UOp u, T init) {
+  const int64_t n = static_cast<int64_t>(last - first);
+  if (n <= 0) return dest;
+  if constexpr (std::is_same<Ex, MI355X>::value && is_device_view_iterator<InIt>::value &&
+                is_device_view_iterator<OutIt>::value) {
+    using IT = typename InIt::value_type;
+    using OT = typename OutIt::value_type;
+    if (!ranges_overlap(first.data(), first.stride(), sizeof(IT), dest.data(), dest.stride(), sizeof(OT), n)) {
+      scan_run<T, Inclusive, HasInit>(label, ex, InvariantInput<IT>{first.data(), first.stride()}, dest, n, op, u,
+                                      init);
+      return dest + n;
+    }
+  }
+  scan_run<T, Inclusive, HasInit>(label, ex, first, dest, n, op, u, init);
   return dest + n;
 }
 
