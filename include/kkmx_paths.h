@@ -172,12 +172,16 @@ int kkmx_spmv_csr(const int64_t* rowptr_dev, const int32_t* col_dev, const doubl
  * kkmx_histogram: mode 0 = global atomics, 1 = bins [0, lds_bins) privatised
  *   in team scratch (LDS atomics), the rest global; 2 = as 1 with 16-byte key
  *   loads and the lowest 256 bins replicated 16 times in LDS (keys 16-byte
- *   aligned; otherwise mode 1 runs); 3 = as 2 for the LDS part, with bins at
- *   or above lds_bins counted in a per-team row of a workspace and the team's
- *   LDS counts stored into the same row, the rows then summed into bins by a
- *   second kernel (no device-scope atomics; the workspace, teams x nbins
- *   int32, is kept between calls and released by kkmx_finalize).  Modes 2
- *   and 3 lower lds_bins to what fits in LDS beside the replicas.
+ *   aligned; otherwise mode 1 runs); 3 = every bin counted in LDS when they
+ *   fit (up to about 70 K bins; lds_bins is then not used): the lowest 256
+ *   bins replicated 16 times as 32-bit counters, the others as 16-bit halves
+ *   of LDS words with exact carry-out, each team's counts stored into its row
+ *   of a workspace and the rows summed into bins by a second kernel; with
+ *   more bins (or more than 2^24 keys per CU) as 2 for the LDS part, with
+ *   bins at or above lds_bins counted in the team's row.  No device-scope
+ *   atomics in mode 3; the workspace, teams x nbins int32, is kept between
+ *   calls and released by kkmx_finalize.  Modes 2 and 3 lower lds_bins to
+ *   what fits in LDS beside the replicas.  Every mode adds into bins.
  * kkmx_scatter_add_f64: bins[idx[i]] += vals[i] with FP64 atomic_add. */
 int kkmx_xorshift64_stream(uint64_t* out_dev, int64_t n, uint64_t seed);
 int kkmx_gups(int64_t* table_dev, int64_t table_size, const uint64_t* stream_dev, int64_t nupdates);

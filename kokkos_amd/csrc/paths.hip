@@ -530,6 +530,140 @@ struct HistRows {
     });
   }
 };
+// Mode 3 when every bin fits in LDS (up to about 70 K bins): no key leaves
+// the CU until the team's row is written.
+//  * Bins below 256 (the head of skewed key distributions) are 32-bit
+//    counters replicated 16 times, lane l adding to copy l mod 16.
+//  * Every other bin is a 16-bit half of a 32-bit LDS word, incremented with
+//    a returning LDS add of 1 or 1 << 16.  An add whose returned half ends in
+//    0x3fff carries the half across a multiple of 16384; the lane that sees
+//    it subtracts 16384 from the half again and appends the bin to the team's
+//    carry list, which becomes one 16384 row atomic per entry at the end.  A
+//    half stays below 65536 while each carry's subtraction lands within 49151
+//    further adds to it.  A wavefront issues its subtraction right after its
+//    own 16 adds return, so at most two rounds of the team's adds come in
+//    between (2 x 16 wavefronts x 16 adds x 64 lanes = 32768).  The team
+//    handles fewer than 1024 x 16384 keys, so the list never fills (larger
+//    extents take the team-row path below).
+//  * The 16 adds of a lane's four 16-byte key loads are issued back to back;
+//    their returns are checked together, with one branch per 16 keys that is
+//    taken only for a carry.
+// Keys past the end of the team's share count into a spare word per lane.
+// The time goes to streaming the keys: 2^28 Zipf keys take 0.24 ms, against
+// 0.21 ms for the loads alone, and 0.38 ms when 3 % of the keys (bins above
+// the 32 K that fit as 32-bit counters) left the loop for the team row.
+struct HistPacked {
+  static constexpr int kCopies = 16;
+  static constexpr int kHot    = 256;
+  static constexpr int kList   = 1024;
+  static constexpr uint32_t kCarry = 16384;
+  static constexpr int64_t kMaxKeysPerTeam = int64_t(kList) * kCarry;
+  DevView<const int32_t> keys;
+  Kokkos::View<int32_t**, Kokkos::LayoutRight, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>> rows;
+  int64_t n;
+  int32_t nbins;
+  using member_type = Kokkos::TeamPolicy<Kokkos::MI355X>::member_type;
+  using Lds = Kokkos::View<uint32_t*, Kokkos::MI355X::scratch_memory_space, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+  KOKKOS_INLINE_FUNCTION static int32_t packed_words(int32_t nbins) { return (nbins - kHot + 1) / 2; }
+  static size_t scratch_words(int32_t nbins) {
+    return size_t(kCopies) * (kHot + 1) + size_t(packed_words(nbins)) + 64 + 1 + kList;
+  }
+  KOKKOS_INLINE_FUNCTION void operator()(const member_type& m) const {
+    constexpr int kLoads   = HistWide::kWideLoads;
+    const int32_t hot_w    = kCopies * (kHot + 1);
+    const int32_t pw       = packed_words(nbins);
+    const int32_t dummy0   = hot_w + pw;
+    const int32_t list_cnt = dummy0 + 64;
+    const int32_t list0    = list_cnt + 1;
+    const int32_t words    = list0 + kList;
+    int32_t* row           = &rows(m.league_rank(), 0);
+    Lds h(m.team_scratch(0), words);
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, words), [&](const int b) { h(b) = 0u; });
+    m.team_barrier();
+    const int lane        = m.team_rank();
+    const int T           = m.team_size();
+    const int wl          = lane & 63;
+    const int32_t hot_off = (lane % kCopies) * (kHot + 1);
+    const int64_t groups  = n / 4;
+    const int64_t nteams  = m.league_size();
+    const int64_t chunk   = (groups + nteams - 1) / nteams;
+    const int64_t g0      = chunk * m.league_rank();
+    const int64_t g1      = g0 + chunk < groups ? g0 + chunk : groups;
+    const int4* kv        = reinterpret_cast<const int4*>(keys.data());
+    // LDS word and increment of key k: a hot replica (1), a packed half
+    // (1 or 1 << 16), or this lane's spare word for an empty slot (k < 0).
+    auto slot = [&](int32_t k, int32_t& w, uint32_t& inc, uint32_t& sh) {
+      const int32_t j = k - kHot;
+      sh              = (k >= kHot) ? uint32_t(j & 1) * 16u : 0u;
+      w               = k < 0 ? dummy0 + wl : (k < kHot ? hot_off + k : hot_w + (j >> 1));
+      inc             = 1u << sh;
+    };
+    auto carry = [&](int32_t k, int32_t w, uint32_t sh) {
+      Kokkos::atomic_sub(&h(w), kCarry << sh);
+      const uint32_t s2 = Kokkos::atomic_fetch_add(&h(list_cnt), 1u);
+      h(list0 + int32_t(s2)) = uint32_t(k);
+    };
+    for (int64_t gb = g0; gb < g1; gb += int64_t(kLoads) * T) {
+      int32_t k[4 * kLoads];
+#pragma unroll
+      for (int u = 0; u < kLoads; ++u) {
+        const int64_t gi = gb + lane + int64_t(u) * T;
+        const int4 q     = gi < g1 ? kv[gi] : int4{-1, -1, -1, -1};
+        k[4 * u] = q.x, k[4 * u + 1] = q.y, k[4 * u + 2] = q.z, k[4 * u + 3] = q.w;
+      }
+      uint32_t old[4 * kLoads];
+#pragma unroll
+      for (int r = 0; r < 4 * kLoads; ++r) {
+        int32_t w;
+        uint32_t inc, sh;
+        slot(k[r], w, inc, sh);
+        old[r] = Kokkos::atomic_fetch_add(&h(w), inc);
+      }
+      bool any = false;
+#pragma unroll
+      for (int r = 0; r < 4 * kLoads; ++r) {
+        int32_t w;
+        uint32_t inc, sh;
+        slot(k[r], w, inc, sh);
+        any = any || (k[r] >= kHot && ((old[r] >> sh) & (kCarry - 1)) == kCarry - 1);
+      }
+      if (__ballot(any)) {
+#pragma unroll
+        for (int r = 0; r < 4 * kLoads; ++r) {
+          int32_t w;
+          uint32_t inc, sh;
+          slot(k[r], w, inc, sh);
+          if (k[r] >= kHot && ((old[r] >> sh) & (kCarry - 1)) == kCarry - 1) carry(k[r], w, sh);
+        }
+      }
+    }
+    if (m.league_rank() == 0 && lane < n - groups * 4) {
+      int32_t w;
+      uint32_t inc, sh;
+      const int32_t kk = keys(groups * 4 + lane);
+      slot(kk, w, inc, sh);
+      const uint32_t o = Kokkos::atomic_fetch_add(&h(w), inc);
+      if (kk >= kHot && ((o >> sh) & (kCarry - 1)) == kCarry - 1) carry(kk, w, sh);
+    }
+    m.team_barrier();
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, nbins), [&](const int b) {
+      uint32_t c;
+      if (b < kHot) {
+        c = 0;
+        for (int r = 0; r < kCopies; ++r) c += h(r * (kHot + 1) + b);
+      } else {
+        const int32_t j = b - kHot;
+        c               = (h(hot_w + (j >> 1)) >> ((j & 1) * 16)) & 0xffffu;
+      }
+      row[b] = int32_t(c);
+    });
+    m.team_barrier();
+    const int32_t nlist = int32_t(h(list_cnt));
+    Kokkos::parallel_for(Kokkos::TeamVectorRange(m, nlist), [&](const int s2) {
+      desul::atomic_add(row + int32_t(h(list0 + s2)), int32_t(kCarry), desul::MemoryOrderRelaxed(), desul::MemoryScopeCore());
+    });
+  }
+};
 // One team of 8 wavefronts per 64 consecutive bins: wavefront w sums rows
 // w, w+8, ... of the team's bins (64 lanes = 64 consecutive bins, so each
 // load is one contiguous 256-byte run), and the 8 partial sums meet in LDS.
@@ -1203,6 +1337,22 @@ int kkmx_histogram(const int32_t* keys, int64_t n, int32_t* bins, int32_t nbins,
       const size_t budget = Kokkos::TeamPolicy<Kokkos::MI355X>(cus, 1024).scratch_size_max(0);
       const int64_t fit   = int64_t(budget / 4) - int64_t(HistWide::kHotCopies) * (hot0 + 1);
       if (lb > fit) lb = int32_t(fit);
+    }
+    if (mode == 3 && reinterpret_cast<uintptr_t>(keys) % 16 == 0 && nbins > HistPacked::kHot) {
+      const int cus_ = Kokkos::MI355X::impl_cu_count();
+      const size_t budget = Kokkos::TeamPolicy<Kokkos::MI355X>(cus_, 1024).scratch_size_max(0);
+      if (HistPacked::scratch_words(nbins) * 4 <= budget && n / cus_ < HistPacked::kMaxKeysPerTeam - 4096) {
+        int32_t* rows = hist_rows_space(size_t(cus_) * nbins);
+        using Rows = Kokkos::View<int32_t**, Kokkos::LayoutRight, Kokkos::MI355XSpace, Kokkos::MemoryTraits<Kokkos::Unmanaged>>;
+        auto pol = Kokkos::TeamPolicy<Kokkos::MI355X>(cus_, 1024)
+                       .set_scratch_size(0, Kokkos::PerTeam(HistPacked::scratch_words(nbins) * 4));
+        Kokkos::parallel_for("kkmx::hist_packed", pol, HistPacked{K, Rows(rows, cus_, nbins), n, nbins});
+        const int sum_teams = (nbins + HistRowSum::kBins - 1) / HistRowSum::kBins;
+        auto sum_pol = Kokkos::TeamPolicy<Kokkos::MI355X>(sum_teams, HistRowSum::kBins * HistRowSum::kWaves)
+                           .set_scratch_size(0, Kokkos::PerTeam(size_t(HistRowSum::kBins) * HistRowSum::kWaves * 4));
+        Kokkos::parallel_for("kkmx::hist_row_sum", sum_pol, HistRowSum{decltype(HistRowSum::rows)(rows, cus_, nbins), B});
+        return;
+      }
     }
     if (mode == 3 && reinterpret_cast<uintptr_t>(keys) % 16 == 0) {
       int32_t lb3       = lds_bins > 0 ? lds_bins : 32768;

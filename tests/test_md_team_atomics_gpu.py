@@ -217,6 +217,42 @@ def test_histogram_rows_adversarial_keys(paths, oracle):
         assert np.array_equal(Bb.download(np.int32, nb), oracle.histogram(keys, nb)), key
 
 
+@pytest.mark.parametrize("pattern", ["one_packed_bin", "word_pair", "last_bin", "hot_bin"])
+def test_histogram_packed_carries(paths, oracle, pattern):
+    """Mode 3 with every bin in LDS on keys that drive one 16-bit counter far
+    past its range (2^26 keys, about 16 carries of 16384 per team), two
+    counters sharing one LDS word, the upper half of the last word, and a
+    replicated hot bin."""
+    P = paths
+    n, nb = 1 << 26, 65536
+    if pattern == "one_packed_bin":
+        keys = np.full(n, 300, np.int32)
+    elif pattern == "word_pair":
+        keys = np.tile(np.array([300, 301], np.int32), n // 2)
+    elif pattern == "last_bin":
+        keys = np.full(n, nb - 1, np.int32)
+    else:
+        keys = np.full(n, 7, np.int32)
+    Kb, Bb = dev(keys), K.DeviceBuffer(nb * 4)
+    K.load("kkmx_rt").mx_memset_async(Bb.ptr, 0, nb * 4, P.kkmx_stream())
+    run(P, P.kkmx_histogram(Kb.ptr, n, Bb.ptr, nb, 3, 0))
+    assert np.array_equal(Bb.download(np.int32, nb), oracle.histogram(keys, nb)), pattern
+
+
+@pytest.mark.parametrize("nb", [257, 1001, 69000, 70000, 1 << 17])
+def test_histogram_mode3_bin_counts(paths, oracle, nb):
+    """Mode 3 across bin counts: 257 and 1001 (odd packed tails), 69000 (near
+    the LDS limit of the packed layout, 69725 bins), and 70000 and 2^17 (too
+    many bins: team rows for the bins above the LDS range)."""
+    P = paths
+    n = (1 << 22) + 13
+    keys = oracle.fill_uniform_i32(n, 90 + nb % 11, 0, nb - 1)
+    Kb, Bb = dev(keys), K.DeviceBuffer(nb * 4)
+    K.load("kkmx_rt").mx_memset_async(Bb.ptr, 0, nb * 4, P.kkmx_stream())
+    run(P, P.kkmx_histogram(Kb.ptr, n, Bb.ptr, nb, 3, 32768))
+    assert np.array_equal(Bb.download(np.int32, nb), oracle.histogram(keys, nb)), nb
+
+
 # ---------------------------------------------------------------- FP64 scatter
 @pytest.mark.parametrize("n,nbins", [(1000, 16), (1 << 20, 1 << 10), (1 << 26, 1 << 20)])
 def test_scatter_f64_within_1e10(paths, oracle, n, nbins):
