@@ -422,8 +422,9 @@ def leg_atomics(K, P, timer):
         "gups_2^30_table_2^28_updates": {"ms": round(g[0], 3), "gups": round(U / g[0] / 1e6, 3)},
         "histogram_zipf_2^28_keys": {"ms": round(h[0], 3), "gbps": round(4 * nk / h[0] / 1e6, 1),
                                      "keys_per_s": round(nk / h[0] * 1e3, 1),
-                                     "kernel": "LDS bins with the 256 hottest replicated 16x, 16-byte key loads, team "
-                                               "rows summed by a second kernel (no device-scope atomics)"},
+                                     "kernel": "every bin in LDS: the 256 hottest as 16 replicated 32-bit counters, "
+                                               "the rest as 16-bit halves with exact carry-out; 16-byte key loads; "
+                                               "team rows summed by a second kernel (no device-scope atomics)"},
         "histogram_zipf_2^28_keys_global_atomic_flush": {"ms": round(h2[0], 3), "gbps": round(4 * nk / h2[0] / 1e6, 1)},
         "histogram_zipf_2^28_keys_one_key_per_load": {"ms": round(h1[0], 3), "gbps": round(4 * nk / h1[0] / 1e6, 1)},
         "scatter_f64_2^26_into_2^20": {"ms": round(sc[0], 3), "updates_per_s": round(ns / sc[0] * 1e3, 1),
