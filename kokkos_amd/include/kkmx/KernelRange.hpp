@@ -752,6 +752,57 @@ __global__ __launch_bounds__(1024) void k_range_scan_rr(
                                             s_red.get(), &s_pos);
 }
 
+// The contribution-keeping body in a loop: a grid of resident workgroups,
+// each claiming its next tile from the same word once its current tile's
+// final calls are issued, so a workgroup's end and a new one's start (the
+// dispatch between them) are paid once per workgroup instead of once per tile
+// (3 % at 2^28 int32 elements with 4096 tiles).
+//   * Claims are counted past the last tile: every workgroup stops on its
+//     first id >= ntiles, so the launch makes exactly ntiles + gridDim.x
+//     claims and the one that receives the last id returns the word to
+//     {0, epoch + 1} after every other claim of the launch.
+//   * The claim is taken after the look-back of the current tile, so a
+//     workgroup holding an unstarted tile only has its own final pass to
+//     finish: the lowest unfinished tile is always running or about to run.
+//     A workgroup that is not resident holds no tile, so the launch needs no
+//     co-residency.
+//   * The claim is made after the last final call, so the compiler still
+//     replaces the final pass's loads with the first pass's values.
+template <class CR, class Index, int IPT, bool WithTotal>
+__global__ __launch_bounds__(1024) void k_range_scan_keep_loop(
+    const CR cr, const int64_t begin, const int64_t end, const int64_t ntiles,
+    ScanTileState<typename CR::value_type> st, unsigned* control,
+    typename CR::value_type* total_out) {
+  using V = typename CR::value_type;
+  __shared__ LdsSlots16<V> s_wave;
+  __shared__ LdsSlots16<V> s_red;
+  __shared__ uint64_t s_claim;
+  __shared__ int s_pos;
+  uint64_t* claim_word   = reinterpret_cast<uint64_t*>(control + 2);
+  const uint32_t last_id = static_cast<uint32_t>(ntiles + static_cast<int64_t>(gridDim.x) - 1);
+  if (threadIdx.x == 0)
+    s_claim = __hip_atomic_fetch_add(claim_word, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  uint64_t claim = s_claim;
+  while (true) {
+    const uint32_t id    = static_cast<uint32_t>(claim);
+    const uint32_t epoch = static_cast<uint32_t>(claim >> 32);
+    if (threadIdx.x == 0 && id == last_id)
+      __hip_atomic_store(claim_word, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    if (static_cast<int64_t>(id) >= ntiles) break;
+    scan_tile_keep<CR, Index, IPT, WithTotal>(cr, begin, end, ntiles, st, static_cast<int64_t>(id), epoch, total_out,
+                                              s_wave.get(), s_red.get(), &s_pos);
+    uint64_t next = 0;
+    if (threadIdx.x == 0)
+      next = __hip_atomic_fetch_add(claim_word, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();  // the tile's LDS reads are done before s_claim changes
+    if (threadIdx.x == 0) s_claim = next;
+    __syncthreads();
+    claim = s_claim;
+  }
+}
+
 // ------------------------------------------------------------------ host helpers
 template <class V>
 struct ResultSink {
@@ -1271,10 +1322,20 @@ void range_scan_execute(const CR& cr_in, const Policy& policy, typename CR::valu
   int64_t bb = b, ee = e, nt = ntiles;
   unsigned* control = inst->control;
   const void* k     = range_scan_kernel<CR, Index, WithTotal>(ipt, keep);
+  unsigned grid     = static_cast<unsigned>(ntiles);
+  if constexpr (keep64_fits) {
+    // 1024 lanes holding 64 values each need more than 64 registers per lane
+    // (118 for an int32 sum), so one workgroup fills a CU, and a grid of one
+    // workgroup per CU loops over the tiles.
+    if (keep == 1 && ipt == 64 && c.block == 1024) {
+      k                 = reinterpret_cast<const void*>(&k_range_scan_keep_loop<CR, Index, 64, WithTotal>);
+      const int64_t cus = MI355X::impl_cu_count() > 0 ? MI355X::impl_cu_count() : 256;
+      grid              = static_cast<unsigned>(ntiles < cus ? ntiles : cus);
+    }
+  }
   if (!g_size_only_dispatch) note_scan_launches(inst, 1, inst->stream);
   if (ev_start) KKMX_SAFE_CALL(mx_event_record(ev_start, inst->stream));
-  mx_launch_kernel(k, static_cast<unsigned>(ntiles), c.block, 0, inst->stream, cr, bb, ee, nt, st, control,
-                   tot_dst);
+  mx_launch_kernel(k, grid, c.block, 0, inst->stream, cr, bb, ee, nt, st, control, tot_dst);
   if (ev_stop) KKMX_SAFE_CALL(mx_event_record(ev_stop, inst->stream));
   if (copy_back) {
     KKMX_SAFE_CALL(mx_stream_sync(inst->stream));

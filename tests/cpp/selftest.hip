@@ -1021,6 +1021,52 @@ KKMX_TEST(scan_epoch_wrap_clears_status) {
   CHECK_EQ(epoch_now, alias + 1u);
 }
 
+// The contribution-keeping tile body at 64 values per lane runs as a loop of
+// one resident workgroup per CU over the tiles: totals, a ragged last tile, a
+// non-zero begin, more tiles than workgroups, back-to-back launches of both
+// bodies, and the claim word left at {0, epoch + 1} by every launch.
+struct ScanIntTotal {
+  Kokkos::View<const int*, Exec, Kokkos::MemoryTraits<Kokkos::RandomAccess>> a;
+  Kokkos::View<int*, Exec> out;
+  int64_t begin;
+  KOKKOS_INLINE_FUNCTION void operator()(const int64_t i, int& u, const bool final) const {
+    u += a(i - begin);
+    if (final) out(i - begin) = u;
+  }
+};
+KKMX_TEST(scan_keep_loop_totals_and_claims) {
+  Exec ex;
+  Kokkos::Impl::MI355XInstance* inst = ex.impl_internal_space_instance();
+  const int saved = Kokkos::Impl::range_overrides().scan_body;
+  for (int body : {1, 0, 1}) {
+    Kokkos::Impl::range_overrides().scan_body = body;
+    for (int64_t n : {(int64_t(1) << 22) + 12345, (int64_t(1) << 25) + 12345, 3 * (int64_t(1) << 24) + 1}) {
+      const int64_t b = 77;
+      Kokkos::View<int*, Exec> a("a", n), out("out", n);
+      Kokkos::parallel_for(Kokkos::RangePolicy<Exec, Kokkos::IndexType<int64_t>>(ex, 0, n),
+                           KOKKOS_LAMBDA(const int64_t i) { a(i) = int((uint64_t(i) * 2654435761u) % 201) - 100; });
+      unsigned before[2], after[2];
+      KKMX_SAFE_CALL(mx_memcpy(before, inst->control + 2, sizeof(before)));
+      int total = 0;
+      Kokkos::parallel_scan(Kokkos::RangePolicy<Exec, Kokkos::IndexType<int64_t>>(ex, b, b + n),
+                            ScanIntTotal{a, out, b}, total);
+      ex.fence();
+      KKMX_SAFE_CALL(mx_memcpy(after, inst->control + 2, sizeof(after)));
+      auto h = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), out);
+      int64_t bad = 0, acc = 0;
+      for (int64_t k = 0; k < n; ++k) {
+        acc += int64_t((uint64_t(k) * 2654435761u) % 201) - 100;
+        bad += (h(k) != acc);
+      }
+      CHECK_EQ(bad, int64_t(0));
+      CHECK_EQ(int64_t(total), acc);
+      CHECK_EQ(after[0], 0u);
+      CHECK_EQ(after[1], before[1] + 1u);
+    }
+  }
+  Kokkos::Impl::range_overrides().scan_body = saved;
+}
+
 // A scan whose value is a struct (generic tile state) and a scan with a tag
 // and a non-zero begin.
 struct Pair64 {
