@@ -627,7 +627,14 @@ __device__ __forceinline__ void scan_tile_rr(const CR& cr, const int64_t begin, 
 // plus an immediate offset instead of a separately materialised 64-bit
 // address; the ragged copy re-derives its bases before the final pass, so its
 // bounds masks are not carried across the look-back.
-template <class CR, class Index, int IPT, bool WithTotal, bool Full>
+//   K > 1 (value types that are not arithmetic scalars, whose joins cost
+// many instructions): each round gives a lane K consecutive indices, which
+// it runs through the functor in order with one running update, as a serial
+// scan does; the wave scans then combine one value per lane and round, K
+// times fewer than with one index per lane.  A lane's K loads are adjacent,
+// so the compiler merges them into one wider load.  Floating-point joins
+// then associate differently from K = 1 (integer results are identical).
+template <class CR, class Index, int IPT, bool WithTotal, bool Full, int K = 1>
 __device__ __forceinline__ void scan_tile_keep_body(const CR& cr, const int64_t end, const int64_t ntiles,
                                                     const ScanTileState<typename CR::value_type>& st,
                                                     const int64_t tile, const uint32_t epoch,
@@ -640,24 +647,34 @@ __device__ __forceinline__ void scan_tile_keep_body(const CR& cr, const int64_t 
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6, l = wave::lane();
   V identity;
   cr.init(identity);
+  static_assert(K >= 1 && IPT % K == 0, "K must divide the values per lane");
+  constexpr int kRounds = IPT / K;
   constexpr int kChunk  = 8;
   constexpr int kChunks = (IPT + kChunk - 1) / kChunk;
   int64_t chunk_base[kChunks];
 #pragma unroll
   for (int c = 0; c < kChunks; ++c) {
     chunk_base[c] = base + c * (kChunk * 64);
-    asm volatile("" : "+v"(chunk_base[c]));
+    if constexpr (K == 1) asm volatile("" : "+v"(chunk_base[c]));
   }
-  V v[IPT];
+  // Index of round r's k-th call in this lane.
+  const auto index = [&](int r, int k) -> int64_t {
+    if constexpr (K == 1) return chunk_base[r / kChunk] + (r % kChunk) * 64;
+    else return base + static_cast<int64_t>(r) * (64 * K) + k;
+  };
+  V v[kRounds];
   V wave_total = identity;
 #pragma unroll
-  for (int r = 0; r < IPT; ++r) {
-    const int64_t i = chunk_base[r / kChunk] + (r % kChunk) * 64;
-    v[r]            = identity;
-    if (Full || i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), v[r], false);
+  for (int r = 0; r < kRounds; ++r) {
+    v[r] = identity;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t i = index(r, k);
+      if (Full || i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), v[r], false);
+    }
   }
 #pragma unroll
-  for (int r = 0; r < IPT; ++r) join(wave_total, wave::reduce(v[r], identity, join));
+  for (int r = 0; r < kRounds; ++r) join(wave_total, wave::reduce(v[r], identity, join));
   if (l == 0) s_wave[w] = wave_total;
   __syncthreads();
   const V x    = (l < nw) ? s_wave[l] : identity;
@@ -686,20 +703,29 @@ __device__ __forceinline__ void scan_tile_keep_body(const CR& cr, const int64_t 
   }
   V running = prefix;
   join(running, wpre);
-  if constexpr (!Full) {
+  if constexpr (!Full && K == 1) {
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) asm volatile("" : "+v"(chunk_base[c]));
   }
 #pragma unroll
-  for (int r = 0; r < IPT; ++r) {
+  for (int r = 0; r < kRounds; ++r) {
     const V inc = wave::inclusive_scan(v[r], identity, join);
     V u         = running;
     join(u, wave::shift_up(inc, identity));
     join(running, wave::readlane(inc, 63));
-    const int64_t i = chunk_base[r / kChunk] + (r % kChunk) * 64;
-    if (Full || i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), u, true);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t i = index(r, k);
+      if (Full || i < end) call_functor<Tag>(cr.functor(), static_cast<Index>(i), u, true);
+    }
   }
 }
+
+// Consecutive indices per lane and round in the contribution-keeping body:
+// 1 for arithmetic values, 4 for other value types (structs, wrapped values
+// with an emptiness flag), whose joins dominate the tile's time.
+template <class V>
+inline constexpr int scan_keep_k = std::is_arithmetic<V>::value ? 1 : 4;
 
 template <class CR, class Index, int IPT, bool WithTotal>
 __device__ __forceinline__ void scan_tile_keep(const CR& cr, const int64_t begin, const int64_t end,
@@ -707,15 +733,16 @@ __device__ __forceinline__ void scan_tile_keep(const CR& cr, const int64_t begin
                                                const int64_t tile, const uint32_t epoch,
                                                typename CR::value_type* total_out, typename CR::value_type* s_wave,
                                                typename CR::value_type* s_red, int* s_pos) {
+  constexpr int K          = scan_keep_k<typename CR::value_type>;
   const int64_t tile_elems = static_cast<int64_t>(blockDim.x) * IPT;
   const int64_t tbase      = begin + tile * tile_elems;
-  const int64_t base       = tbase + static_cast<int64_t>(threadIdx.x >> 6) * (64 * IPT) + wave::lane();
+  const int64_t base = tbase + static_cast<int64_t>(threadIdx.x >> 6) * (64 * IPT) + int64_t(wave::lane()) * K;
   if (tbase + tile_elems <= end)
-    scan_tile_keep_body<CR, Index, IPT, WithTotal, true>(cr, end, ntiles, st, tile, epoch, total_out, s_wave, s_red,
-                                                         s_pos, base);
+    scan_tile_keep_body<CR, Index, IPT, WithTotal, true, K>(cr, end, ntiles, st, tile, epoch, total_out, s_wave,
+                                                            s_red, s_pos, base);
   else
-    scan_tile_keep_body<CR, Index, IPT, WithTotal, false>(cr, end, ntiles, st, tile, epoch, total_out, s_wave,
-                                                          s_red, s_pos, base);
+    scan_tile_keep_body<CR, Index, IPT, WithTotal, false, K>(cr, end, ntiles, st, tile, epoch, total_out, s_wave,
+                                                             s_red, s_pos, base);
 }
 
 // One tile per workgroup, claimed in launch order from control[2..3] (one

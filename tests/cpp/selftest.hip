@@ -1068,7 +1068,8 @@ KKMX_TEST(scan_keep_loop_totals_and_claims) {
 }
 
 // A scan whose value is a struct (generic tile state) and a scan with a tag
-// and a non-zero begin.
+// and a non-zero begin, with both tile bodies (at 2^21 + 9 indices the
+// contribution-keeping body gives each lane 4 consecutive indices a round).
 struct Pair64 {
   int64_t a, b;
   KOKKOS_INLINE_FUNCTION Pair64& operator+=(const Pair64& o) {
@@ -1091,7 +1092,10 @@ struct StructScan {
   }
 };
 KKMX_TEST(scan_struct_value_with_tag_and_offset) {
-  for (int64_t n : {int64_t(1), int64_t(777), int64_t(1000003)}) {
+  const int saved = Kokkos::Impl::range_overrides().scan_body;
+  for (int body : {0, 1})
+  for (int64_t n : {int64_t(1), int64_t(777), int64_t(1000003), (int64_t(1) << 21) + 9}) {
+    Kokkos::Impl::range_overrides().scan_body = body;
     const int64_t b = 1000;
     Kokkos::View<int64_t*, Exec> a("a", n), c("c", n);
     Pair64 total{0, 0};
@@ -1107,6 +1111,7 @@ KKMX_TEST(scan_struct_value_with_tag_and_offset) {
     CHECK_EQ(total.a, n);
     CHECK_EQ(total.b, ((b + n - 1) * (b + n)) / 2 - (b * (b - 1)) / 2);
   }
+  Kokkos::Impl::range_overrides().scan_body = saved;
 }
 
 // =========================================================================

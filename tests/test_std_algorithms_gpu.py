@@ -54,6 +54,32 @@ def test_inclusive_max_scan_no_neutral_element(paths, oracle, n):
     assert np.array_equal(D.download(np.int32, n), np.maximum.accumulate(x))
 
 
+@pytest.fixture
+def scan_body(paths):
+    """Pin the scan tile body for one test; the automatic choice afterwards."""
+    def pin(body):
+        K.check(paths.kkmx_set_scan_body(body), paths)
+    yield pin
+    K.check(paths.kkmx_set_scan_body(-1), paths)
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 4099, 100003, (1 << 21) + 17, (1 << 22) + 5, (1 << 24) + 3])
+@pytest.mark.parametrize("body", [0, 1])
+def test_max_scan_both_tile_bodies(paths, oracle, scan_body, body, n):
+    """The running maximum's value type wraps the int with an emptiness flag,
+    so the contribution-keeping body gives each lane 4 consecutive indices
+    per round; both bodies, ragged tiles and in-place calls equal numpy."""
+    P = paths
+    scan_body(body)
+    x = oracle.fill_uniform_i32(n, 55 + n % 5, -(1 << 30), 1 << 30)
+    want = np.maximum.accumulate(x)
+    S, D = K.DeviceBuffer.from_array(x), K.DeviceBuffer(n * 4)
+    run(P, P.kkmx_std_inclusive_scan_max_i32(S.ptr, D.ptr, n))
+    assert np.array_equal(D.download(np.int32, n), want)
+    run(P, P.kkmx_std_inclusive_scan_max_i32(S.ptr, S.ptr, n))  # in place
+    assert np.array_equal(S.download(np.int32, n), want)
+
+
 @pytest.mark.parametrize("n", [1, 1000, 1 << 20])
 def test_transform_exclusive_scan_squares_f64(paths, oracle, n):
     P = paths
