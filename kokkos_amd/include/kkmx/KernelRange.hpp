@@ -631,9 +631,10 @@ __device__ __forceinline__ void scan_tile_rr(const CR& cr, const int64_t begin, 
 // many instructions): each round gives a lane K consecutive indices, which
 // it runs through the functor in order with one running update, as a serial
 // scan does; the wave scans then combine one value per lane and round, K
-// times fewer than with one index per lane.  A lane's K loads are adjacent,
-// so the compiler merges them into one wider load.  Floating-point joins
-// then associate differently from K = 1 (integer results are identical).
+// times fewer than with one index per lane.  A lane's K indices are
+// adjacent, so its K loads of a unit-stride input fall in one or two cache
+// lines.  Floating-point joins then associate differently from K = 1
+// (integer results are identical).
 template <class CR, class Index, int IPT, bool WithTotal, bool Full, int K = 1>
 __device__ __forceinline__ void scan_tile_keep_body(const CR& cr, const int64_t end, const int64_t ntiles,
                                                     const ScanTileState<typename CR::value_type>& st,
