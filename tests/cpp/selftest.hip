@@ -1067,6 +1067,52 @@ KKMX_TEST(scan_keep_loop_totals_and_claims) {
   Kokkos::Impl::range_overrides().scan_body = saved;
 }
 
+// A graph holding the looping contribution-keeping scan (2^24 + 5 int32
+// indices, body pinned): every replay equals the host scan, and so does an
+// eager scan between replays, which leaves its instance's claim word at
+// {0, epoch + 1} (graph nodes run on instances of their own).
+KKMX_TEST(graph_scan_keep_loop_replays) {
+  Exec ex;
+  Kokkos::Impl::MI355XInstance* inst = ex.impl_internal_space_instance();
+  const int saved = Kokkos::Impl::range_overrides().scan_body;
+  Kokkos::Impl::range_overrides().scan_body = 1;
+  const int64_t n = (int64_t(1) << 24) + 5;
+  Kokkos::View<int*, Exec> a("a", n), out("out", n);
+  Kokkos::parallel_for(Kokkos::RangePolicy<Exec, Kokkos::IndexType<int64_t>>(ex, 0, n),
+                       KOKKOS_LAMBDA(const int64_t i) { a(i) = int(i % 13) - 6; });
+  auto graph = Kokkos::Experimental::create_graph(ex, [&](auto root) {
+    root.then_parallel_scan("scan", Kokkos::RangePolicy<Exec, Kokkos::IndexType<int64_t>>(0, n),
+                            ScanIntTotal{a, out, 0});
+  });
+  std::vector<int> want(n);
+  int acc = 0;
+  for (int64_t k = 0; k < n; ++k) want[k] = (acc += int(k % 13) - 6);
+  Kokkos::View<int*, Exec> out2("out2", n);
+  const auto mismatches = [&](const Kokkos::View<int*, Exec>& v) {
+    auto h      = Kokkos::create_mirror_view_and_copy(Kokkos::HostSpace(), v);
+    int64_t bad = 0;
+    for (int64_t k = 0; k < n; ++k) bad += (h(k) != want[k]);
+    return bad;
+  };
+  for (int rep = 0; rep < 4; ++rep) {
+    Kokkos::deep_copy(ex, out, -1);
+    Kokkos::deep_copy(ex, out2, -1);
+    ex.fence();
+    graph.submit();
+    ex.fence();
+    CHECK_EQ(mismatches(out), int64_t(0));
+    unsigned before[2], after[2];
+    KKMX_SAFE_CALL(mx_memcpy(before, inst->control + 2, sizeof(before)));
+    Kokkos::parallel_scan(Kokkos::RangePolicy<Exec, Kokkos::IndexType<int64_t>>(ex, 0, n), ScanIntTotal{a, out2, 0});
+    ex.fence();
+    KKMX_SAFE_CALL(mx_memcpy(after, inst->control + 2, sizeof(after)));
+    CHECK_EQ(mismatches(out2), int64_t(0));
+    CHECK_EQ(after[0], 0u);
+    CHECK_EQ(after[1], before[1] + 1u);
+  }
+  Kokkos::Impl::range_overrides().scan_body = saved;
+}
+
 // A scan whose value is a struct (generic tile state) and a scan with a tag
 // and a non-zero begin, with both tile bodies (at 2^21 + 9 indices the
 // contribution-keeping body gives each lane 4 consecutive indices a round).
