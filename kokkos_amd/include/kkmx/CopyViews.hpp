@@ -99,15 +99,16 @@ struct ElementCopy {
 };
 
 // ViewCopy between LayoutLeft and LayoutRight (both contiguous, rank >= 2)
-// as an LDS-tiled transpose on gfx950: each 256-lane workgroup moves a
+// as an LDS-tiled transpose on gfx950: each 512-lane workgroup moves a
 // 64 x 64 tile of (first index, last index) for one multi-index of the
 // middle dimensions.  It reads along the source's contiguous dimension and
 // writes along the destination's, both coalesced; the tile sits in LDS with
 // a 65-element row pitch so neither pass has bank conflicts.  Consecutive
 // workgroups take consecutive tiles of the last index, and the destination
 // is written with nontemporal stores (a copy's output is not read back by
-// the copy): 0.523 ms for 512^3 doubles against 0.549 ms with the first
-// index fastest and ordinary stores.
+// the copy).  512^3 doubles: 0.452 ms; 0.522 ms with 256 lanes, 0.483 ms
+// with 1024, 0.549 ms with 256 lanes, the first index fastest and ordinary
+// stores.
 //   LayoutLeft  offset of (i, mid, k) = i + e0 * (midL + M * k)
 //   LayoutRight offset of (i, mid, k) = (i * M + midR) * e_last + k
 // midL / midR: the middle indices flattened left-first / right-first, M
@@ -119,7 +120,7 @@ struct TransposeShape {
   int64_t tiles0, tileslast;
 };
 template <class W, bool SrcLeft>
-__global__ __launch_bounds__(256) void k_layout_transpose(W* __restrict__ dst, const W* __restrict__ src,
+__global__ __launch_bounds__(512) void k_layout_transpose(W* __restrict__ dst, const W* __restrict__ src,
                                                           const TransposeShape sh) {
   __shared__ W tile[64][65];
   const int64_t b  = blockIdx.x;
@@ -139,8 +140,8 @@ __global__ __launch_bounds__(256) void k_layout_transpose(W* __restrict__ dst, c
   const int64_t i0 = t0 * 64, k0 = tk * 64;
   // load along the source's contiguous dimension
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int q = row + 4 * r;
+  for (int r = 0; r < 8; ++r) {
+    const int q = row + 8 * r;
     if (SrcLeft) {
       const int64_t i = i0 + lane, k = k0 + q;
       if (i < sh.e0 && k < sh.elast) tile[q][lane] = src[i + sh.e0 * (midL + sh.M * k)];
@@ -152,8 +153,8 @@ __global__ __launch_bounds__(256) void k_layout_transpose(W* __restrict__ dst, c
   __syncthreads();
   // store along the destination's contiguous dimension
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int q = row + 4 * r;
+  for (int r = 0; r < 8; ++r) {
+    const int q = row + 8 * r;
     if (SrcLeft) {  // destination LayoutRight: k fastest; tile[k_local][i_local]
       const int64_t k = k0 + lane, i = i0 + q;
       if (i < sh.e0 && k < sh.elast) __builtin_nontemporal_store(tile[lane][q], &dst[(i * sh.M + midR) * sh.elast + k]);
@@ -205,7 +206,7 @@ bool layout_transpose(const MI355X& exec, const DV& dst, const SV& src) {
     const W* s_ = reinterpret_cast<const W*>(src.data());
     const void* k = lr ? reinterpret_cast<const void*>(&k_layout_transpose<W, true>)
                        : reinterpret_cast<const void*>(&k_layout_transpose<W, false>);
-    mx_launch_kernel(k, static_cast<unsigned>(blocks), 256, 0, exec.stream(), d, s_, sh);
+    mx_launch_kernel(k, static_cast<unsigned>(blocks), 512, 0, exec.stream(), d, s_, sh);
     return true;
   }
 }
